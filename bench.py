@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the strip list-mode TOF MLEM reconstruction.
+
+A step is one MLEM iteration over every resident event of the workload
+(forward projection + back-projection + image update).  The metric is
+event-updates/s = events x iterations / time, with the events already
+resident in HBM when the timed region starts.
+
+Prints one JSON line.  Default workload: 2e7 seeded events on a 1000 x 400
+grid of 1 mm pixels (R = 400 mm, L = 1000 mm, sigma_z = 10 mm,
+sigma_dl = 42 mm, 10-ellipse head phantom), 20 iterations.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "j-pet-mlem-old_amd")
+for p in (ROOT, PKG):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+# published MI355X peak for FP32 (TFLOP/s); the kernels use non-packed FP32,
+# whose issue rate is half of it
+PEAK_FP32_TFLOPS = 157.3
+# algorithmic FP32 operations per kernel evaluation (forward + back-projection
+# of one in-ellipse pixel): Q = rowA + dz*(rowB + C*dz) (2 FMA = 4 flop),
+# exp2 (1), K = pre*e (1), D += K*rho (2); again Q (4), exp2 (1), K (1),
+# K*inv (1) and the image add (1)
+FLOPS_PER_KERNEL_EVAL = 16
+
+
+def cpu_baseline(workload, n_sample, threads):
+    """Times the reference CPU reconstruction (oracle/_ref, the reference's own
+    Reconstruction<float, GaussianKernel<float>>) -- or the oracle port where
+    the reference build is absent -- on a bounded sample of the workload."""
+    import oracle
+    events = workload.events(n_sample, seed=workload.seed + 1)
+    g = oracle.geometry(*workload.oracle_geometry_tuple())
+    kind = "reference" if oracle.ref_available() else "port"
+    fn = oracle.ref_mlem if kind == "reference" else oracle.mlem
+    fn(g, events[: min(2000, n_sample)], iterations=1, n_threads=threads)
+    t0 = time.perf_counter()
+    fn(g, events, iterations=1, n_threads=threads)
+    dt = time.perf_counter() - t0
+    return {
+        "value": n_sample / dt,
+        "unit": "event-updates/s",
+        "cores": threads,
+        "kind": kind,
+        "sample": f"{n_sample} events of the same workload (seed "
+                  f"{workload.seed + 1}), 1 iteration, {dt:.2f} s wall",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="strip-1000x400-1mm-2e7")
+    ap.add_argument("--events", type=int, default=0,
+                    help="override the event count (0 = the workload's)")
+    ap.add_argument("--algorithm", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=4_000_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for "
+              f"{args.gpus}); the multi-GPU sharding is described in "
+              f"DESIGN.md but not launched", file=sys.stderr)
+        sys.exit(2)
+
+    import pystrip
+    from pystrip import phantom
+
+    w = phantom.WORKLOADS[args.workload]
+    n_events = args.events or w.n_events
+    t_gen = time.perf_counter()
+    events = w.events(n_events)
+    t_gen = time.perf_counter() - t_gen
+
+    try:
+        import torch
+        have_torch = torch.cuda.is_available()
+    except Exception:
+        have_torch = False
+
+    rec = pystrip.Reconstruction(w.geometry(), pystrip.MODE_CPU, device=0)
+    if args.algorithm:
+        rec.set_algorithm(args.algorithm)
+    t_up = time.perf_counter()
+    rec.set_events(events)
+    t_up = time.perf_counter() - t_up
+    n_evals = rec.kernel_evaluations()
+
+    rec.iterate(args.warmup)
+    rec.synchronize()
+    if have_torch:
+        torch.cuda.synchronize()
+    rec.set_timing(True)
+    t0 = time.perf_counter()
+    rec.iterate_async(args.steps)
+    rec.synchronize()
+    if have_torch:
+        torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    fwd_ms, bwd_ms, iter_ms, n_timed = rec.get_timing_detail()
+    rec.set_timing(False)
+    kernel_ms = fwd_ms + bwd_ms
+
+    value = n_events * args.steps / elapsed
+    ms_per_step = elapsed * 1e3 / args.steps
+    n_timed = max(n_timed, 1)
+    fwd_avg, bwd_avg = fwd_ms / n_timed, bwd_ms / n_timed
+    two_kernels = bwd_avg > 0
+    if two_kernels:
+        # the dominant kernel of the two carries half the algorithmic work
+        dom_name = ("back-projection" if bwd_avg >= fwd_avg
+                    else "forward projection")
+        dom_ms = max(fwd_avg, bwd_avg)
+        dom_flops = n_evals * FLOPS_PER_KERNEL_EVAL / 2
+    else:
+        dom_name = "event pass (forward + back-projection)"
+        dom_ms = fwd_avg
+        dom_flops = n_evals * FLOPS_PER_KERNEL_EVAL
+    kernel_avg_ms = kernel_ms / n_timed
+    achieved_tflops = dom_flops / (dom_ms * 1e-3) / 1e12
+
+    rho = rec.rho
+    mass = float(rho.astype(np.float64).sum())
+
+    line = {
+        "metric": "MLEM event-updates/s on 1 GPU vs host CPU",
+        "value": value,
+        "unit": "event-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": w.name,
+            "n_events": n_events,
+            "grid": f"{w.n_z}x{w.n_y}",
+            "pixel_mm": w.pixel,
+            "R_mm": w.radius,
+            "L_mm": w.length,
+            "sigma_z_mm": w.sigma_z,
+            "sigma_dl_mm": w.sigma_dl,
+            "algorithm": rec.algorithm,
+            "kernel_evaluations_per_iteration": n_evals,
+            "mean_kernel_evaluations_per_event": n_evals / n_events,
+            "device": rec.device_name(),
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved_tflops,
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / PEAK_FP32_TFLOPS,
+            "traffic": None,
+            "kernel": dom_name,
+            "kernel_avg_ms": dom_ms,
+            "forward_avg_ms": fwd_avg,
+            "backprojection_avg_ms": bwd_avg,
+            "event_kernels_avg_ms": kernel_avg_ms,
+            "iteration_avg_ms": iter_ms / n_timed,
+            "kernel_evaluations_per_s": n_evals / (kernel_avg_ms * 1e-3),
+            "flops_per_kernel_evaluation": FLOPS_PER_KERNEL_EVAL,
+        },
+        "cpu_baseline": None,
+        "timing_s": {"event_generation": t_gen, "upload_and_setup": t_up},
+        "check": {"image_mass": mass},
+    }
+    if not args.no_cpu:
+        threads = min(os.cpu_count() or 1, 16)
+        line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, threads)
+    rec.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,197 @@
+/*
+ * strip_mlem.h — C ABI of the MI355X (gfx950) J-PET 2D strip list-mode TOF
+ * MLEM reconstruction.
+ *
+ * This is the drop-in boundary for the reference's `--gpu` path:
+ *   template <typename F> void PET2D::Strip::GPU::Reconstruction::run(
+ *       Scanner<F,S>&, Response<F>*, int, Output&, int, int,
+ *       delegate<void(int, const Output&)>, delegate<void(int, bool)>,
+ *       int device, int n_blocks, int n_threads_per_block,
+ *       delegate<void(const char*)>)
+ *   (src/2d/strip/cuda/reconstruction.h:31-44, .cu:31-113)
+ * Plain pointers and sizes only; no C++ or torch types cross it.  A C++
+ * wrapper that reproduces the templated `run<F>` call on top of this ABI is
+ * in strip_gpu_reconstruction.hpp.
+ *
+ * Error convention: every function returns STRIP_MLEM_OK (0) or a negative
+ * status; strip_mlem_last_error() gives a message for the calling thread.
+ * (The reference aborts on any CUDA error, src/util/cuda/debug.h:17-55; the
+ * C++ wrapper restores that behaviour by throwing.)
+ *
+ * Threading: a context is single-threaded (one host thread at a time); all
+ * device work is issued on the context's own HIP stream.  Several contexts
+ * may coexist in one process (the reference's global textures made its run()
+ * non-reentrant, cuda/reconstruction.cu:12-13; this ABI is not).
+ */
+#ifndef STRIP_MLEM_H
+#define STRIP_MLEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STRIP_MLEM_OK 0
+#define STRIP_MLEM_ERR_ARGUMENT (-1)
+#define STRIP_MLEM_ERR_HIP (-2)
+#define STRIP_MLEM_ERR_STATE (-3)
+#define STRIP_MLEM_ERR_ALLOC (-4)
+#define STRIP_MLEM_ERR_UNSUPPORTED (-5)
+
+/* Scanner geometry, the arguments of Scanner<F,S>(R, L, n_y, n_z, h, w,
+ * sigma_z, sigma_dl, center_y, center_z) (src/2d/strip/scanner.h:39-67).
+ * All lengths in the same unit (the CLI uses millimetres). */
+typedef struct strip_mlem_geometry {
+  double radius;              /* R, half distance between the strips */
+  double scintillator_length; /* L */
+  int n_z_pixels;             /* image width  (z) */
+  int n_y_pixels;             /* image height (y) */
+  double pixel_width;         /* along z */
+  double pixel_height;        /* along y */
+  double sigma_z;             /* TOF sigma along z (--s-z) */
+  double sigma_dl;            /* TOF sigma of delta-l (--s-dl) */
+  double center_y;            /* 0 in the CLI */
+  double center_z;            /* 0 in the CLI */
+} strip_mlem_geometry;
+
+/* Reconstruction semantics.
+ * STRIP_MLEM_MODE_CPU reproduces the reference CPU path (the graded parity
+ * mode: GaussianKernel, no sensitivity, zero-denominator guard,
+ * src/2d/strip/reconstruction.h:220-298).  STRIP_MLEM_MODE_LEGACY_GPU
+ * reproduces the old CUDA warp kernel (AnalyticKernel, 5-point pixel
+ * sensitivity in the denominator, no guard,
+ * cuda/reconstruction/warp_granularity.cuh:88-152). */
+typedef struct strip_mlem_mode {
+  int kernel;          /* 0 = Gaussian, 1 = Analytic */
+  int sensitivity;     /* 0 = none, 1 = multiplies the denominator,
+                          2 = divides the back-projected numerator */
+  int sensitivity_map; /* 0 = sensitivity(pixel centre), 1 = 5-point */
+  int guard_zero;      /* 1 = inv = D>0 ? 1/D : 0 ;  0 = inv = 1/D */
+} strip_mlem_mode;
+
+#define STRIP_MLEM_KERNEL_GAUSSIAN 0
+#define STRIP_MLEM_KERNEL_ANALYTIC 1
+
+typedef struct strip_mlem_ctx strip_mlem_ctx;
+
+/* Creates a context on HIP device `device` (the reference's -D/--cuda-device,
+ * src/common/options.cpp:8).  mode may be NULL (= CPU parity mode).  The
+ * image starts at rho = 1 (reconstruction.h:59). */
+int strip_mlem_create(const strip_mlem_geometry* geometry,
+                      const strip_mlem_mode* mode, int device,
+                      strip_mlem_ctx** out);
+void strip_mlem_destroy(strip_mlem_ctx* ctx);
+
+/* Replaces the event list.  responses: host AoS triples (z_u, z_d, dl), the
+ * layout of Response<float> (src/2d/strip/response.h:13-20) and of the
+ * binary event file.  The events are copied to HBM and their per-event
+ * ellipse parameters precomputed once (the reference recomputes them every
+ * iteration, reconstruction.h:132-134).  (Replaces ResponsesSOA,
+ * cuda/gpu_events_soa.h:15-44.) */
+int strip_mlem_set_events(strip_mlem_ctx* ctx, const float* responses,
+                          int64_t n_responses);
+/* Same, from AoS triples already resident in device memory (not retained). */
+int strip_mlem_set_events_device(strip_mlem_ctx* ctx,
+                                 const float* device_responses,
+                                 int64_t n_responses);
+int64_t strip_mlem_n_events(const strip_mlem_ctx* ctx);
+
+/* Start image / current image, n_y*n_z floats row-major (index y*n_z+z,
+ * src/2d/geometry/pixel.h:29).  (Replaces output_rho.copy_to_device /
+ * copy_from_device, cuda/reconstruction.cu:73-74,107.) */
+int strip_mlem_set_rho(strip_mlem_ctx* ctx, const float* rho);
+int strip_mlem_get_rho(strip_mlem_ctx* ctx, float* rho);
+/* Device-resident variants (pointers to n_y*n_z floats in device memory;
+ * the copies are ordered on the context stream). */
+int strip_mlem_set_rho_device(strip_mlem_ctx* ctx, const float* device_rho);
+int strip_mlem_get_rho_device(strip_mlem_ctx* ctx, float* device_rho);
+
+/* Sensitivity map used by the mode (centre or 5-point), n_y*n_z floats.
+ * (Replaces fill_with_sensitivity, cuda/reconstruction.cu:115-126, and the
+ * CPU map of reconstruction.h:67-73.) */
+int strip_mlem_get_sensitivity(strip_mlem_ctx* ctx, int kind,
+                               float* sensitivity);
+
+/* Runs n MLEM iterations on the resident events (one iteration of
+ * Reconstruction::operator(), reconstruction.h:117-152, per step) and
+ * waits for them to finish. */
+int strip_mlem_iterate(strip_mlem_ctx* ctx, int n_iterations);
+/* Same, without waiting (work is queued on the context stream). */
+int strip_mlem_iterate_async(strip_mlem_ctx* ctx, int n_iterations);
+int strip_mlem_synchronize(strip_mlem_ctx* ctx);
+
+/* Launch shape, replacing -B/--cuda-blocks and -W/--cuda-threads
+ * (src/common/options.cpp:9-11).  0 keeps the automatic choice.
+ * threads_per_block must be a multiple of 64 (one wavefront per event). */
+int strip_mlem_set_launch(strip_mlem_ctx* ctx, int n_blocks,
+                          int threads_per_block);
+/* Selects the hot-path implementation: 0 = automatic (row band when the
+ * mode and grid allow it), 1 = bbox wavefront kernel (every mode), 2 =
+ * row-band LDS kernels (Gaussian kernel, sensitivity none or numerator).
+ * Returns STRIP_MLEM_ERR_UNSUPPORTED when the requested one cannot run the
+ * context's mode or grid.  strip_mlem_get_algorithm returns the one in use. */
+int strip_mlem_set_algorithm(strip_mlem_ctx* ctx, int algorithm);
+int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx);
+
+/* Exact number of kernel evaluations of one iteration, sum_e P_e, where P_e
+ * counts the pixels of event e's clamped bounding box that pass the 3-sigma
+ * ellipse test (gaussian_kernel.h:131-137).  Computed on the device. */
+int strip_mlem_kernel_evaluations(strip_mlem_ctx* ctx, int64_t* count);
+
+/* Per-event forward projections D_e = sum_p K_e(p) rho(p) [s(p)] of the
+ * current image (host array of n_events floats). */
+int strip_mlem_forward(strip_mlem_ctx* ctx, float* denominators);
+
+/* Timing of the hot kernel(s) with HIP events on the context stream.
+ * After strip_mlem_set_timing(ctx, 1), every iteration records the start and
+ * end of its event-loop kernel(s); strip_mlem_get_timing returns the summed
+ * milliseconds of those kernels, of whole iterations, and the iteration
+ * count, then resets them. */
+int strip_mlem_set_timing(strip_mlem_ctx* ctx, int enable);
+int strip_mlem_get_timing(strip_mlem_ctx* ctx, double* kernel_ms,
+                          double* iteration_ms, int64_t* n_iterations);
+/* Same, with the forward-projection and back-projection kernels apart (the
+ * bbox algorithm runs both in one kernel: all of it is reported as
+ * forward). */
+int strip_mlem_get_timing_detail(strip_mlem_ctx* ctx, double* forward_ms,
+                                 double* backward_ms, double* iteration_ms,
+                                 int64_t* n_iterations);
+
+/* Name of the device (the reference's device_name callback,
+ * cuda/reconstruction.cu:54-57). */
+int strip_mlem_device_name(strip_mlem_ctx* ctx, char* buffer, int size);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* strip_mlem_last_error(void);
+
+/* ---- One-call driver with the reference's output protocol ---------------
+ * Mirrors GPU::Reconstruction::run (cuda/reconstruction.cu:31-113):
+ *  - output(user, -1, sensitivity) once, with the 5-point pixel sensitivity;
+ *  - per iteration: progress(user, iteration, 0) ... progress(user, it, 1);
+ *  - in block 0, output(user, i+1, rho) after iterations i < 5 and
+ *    i = 9, 14, 19, 29, 49, 99 unless i is the block's last iteration;
+ *  - output(user, (block+1)*n_iterations_in_block, rho) after each block;
+ *  - a final progress(user, n_blocks*n_iterations_in_block, 0).
+ * rho_inout holds the start image and receives the final one.  The image
+ * pointer passed to `output` is only valid during the callback.  The mode
+ * decides the arithmetic (NULL = CPU parity mode). */
+typedef void (*strip_mlem_output_fn)(void* user, int iteration,
+                                     const float* image, int n_z, int n_y);
+typedef void (*strip_mlem_progress_fn)(void* user, int completed,
+                                       int finished);
+typedef void (*strip_mlem_device_name_fn)(void* user, const char* name);
+
+int strip_mlem_run(const strip_mlem_geometry* geometry,
+                   const strip_mlem_mode* mode, const float* responses,
+                   int n_responses, float* rho_inout, int n_iteration_blocks,
+                   int n_iterations_in_block, strip_mlem_output_fn output,
+                   strip_mlem_progress_fn progress, void* user, int device,
+                   int n_blocks, int n_threads_per_block,
+                   strip_mlem_device_name_fn device_name);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,971 @@
+// MI355X (gfx950) J-PET 2D strip list-mode TOF MLEM: kernels + C ABI.
+//
+// One MLEM iteration (src/2d/strip/reconstruction.h:117-152) is
+//   D_e   = sum_{p in E_e} K_e(p) rho(p)                      (forward)
+//   bp(p) = sum_{e : p in E_e} K_e(p) * inv_e,  inv_e = D_e>0 ? 1/D_e : 0
+//   rho'(p) = rho(p) * bp(p)                                  (finalize)
+// which equals the reference's  rho'(p) = sum_e (K_e(p) rho(p)) inv_e
+// (reconstruction.h:277-296) with rho(p) factored out of the event sum.
+// E_e is the set of pixels of the event's clamped bounding box whose centre
+// passes the 3-sigma ellipse test (reconstruction.h:255-264).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "strip_band.hpp"
+#include "strip_device.hpp"
+#include "strip_mlem.h"
+
+using namespace strip;
+
+namespace strip {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& message) {
+  g_last_error = message;
+  return code;
+}
+
+}  // namespace strip
+
+#define HIP_TRY(expr)                                                     \
+  do {                                                                    \
+    hipError_t _err = (expr);                                             \
+    if (_err != hipSuccess)                                               \
+      return strip::fail(STRIP_MLEM_ERR_HIP, std::string(#expr) + ": " +  \
+                                                 hipGetErrorString(_err)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Per-event precompute from AoS (z_u, z_d, dl) triples.
+__global__ __launch_bounds__(256) void setup_events_kernel(
+    DeviceGeometry g, const float* __restrict__ responses, int64_t n,
+    float4* __restrict__ ev_a, float4* __restrict__ ev_b,
+    float4* __restrict__ ev_c, uint2* __restrict__ ev_box) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 a, b, c;
+  uint2 box;
+  setup_event_exact(g, responses[3 * i], responses[3 * i + 1],
+                    responses[3 * i + 2], a, b, c, box);
+  ev_a[i] = a;
+  ev_b[i] = b;
+  ev_c[i] = c;
+  ev_box[i] = box;
+}
+
+// Sensitivity map: kind 0 = Scanner::sensitivity(pixel centre)
+// (reconstruction.h:67-73), kind 1 = Scanner::pixel_sensitivity 5-point
+// rule (scanner.h:174-185, cuda/reconstruction.cu:115-126).
+__global__ __launch_bounds__(256) void sensitivity_kernel(DeviceGeometry g,
+                                                          int kind,
+                                                          float* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n_z * g.n_y) return;
+  int ix = i % g.n_z, iy = i / g.n_z;
+  float z = pixel_center_z(g, ix), y = pixel_center_y(g, iy);
+  float v;
+  if (kind == 0) {
+    v = scanner_sensitivity(g, z, y);
+  } else {
+#pragma clang fp contract(off)
+    float hw = g.pixel_width / 2.0f, hh = g.pixel_height / 2.0f;
+    v = scanner_sensitivity(g, z, y) / 3.0f +
+        scanner_sensitivity(g, z + hw, y + hh) / 6.0f +
+        scanner_sensitivity(g, z - hw, y - hh) / 6.0f +
+        scanner_sensitivity(g, z + (-hw), y + hh) / 6.0f +
+        scanner_sensitivity(g, z - (-hw), y - hh) / 6.0f;
+  }
+  out[i] = v;
+}
+
+struct ModeFlags {
+  int kernel;       // 0 gaussian, 1 analytic
+  int sensitivity;  // 0 none, 1 denominator, 2 numerator
+  int guard_zero;
+};
+
+// Bounding-box wavefront kernel ("bbox" algorithm): one wave64 per event,
+// grid-stride over events.  Lanes walk the clamped bbox linearised row-major
+// like the reference CPU loop (reconstruction.h:255-288).  Pass 1 forms D_e
+// with a wavefront reduction; pass 2 recomputes K and adds K*inv_e into the
+// back-projection image with L2 float atomics.
+//  FAST = true: Gaussian kernel, no sensitivity, guarded inverse (CPU
+//  parity mode) with the row-hoisted prefactor and hardware exp2.
+//  FAST = false: generic semantics (legacy --gpu mode and variants).
+template <bool FAST, bool BACKPROJECT>
+__global__ __launch_bounds__(256) void bbox_event_kernel(
+    DeviceGeometry g, ModeFlags m, const float4* __restrict__ ev_a,
+    const float4* __restrict__ ev_b, const uint2* __restrict__ ev_box,
+    int64_t n_events, const float* __restrict__ rho,
+    const float* __restrict__ sens, float* __restrict__ bp,
+    float* __restrict__ denominators) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves_per_block = blockDim.x >> 6;
+  const int64_t n_waves = (int64_t)gridDim.x * waves_per_block;
+  const bool legacy_split = !FAST && m.kernel == 1;
+  for (int64_t e = (int64_t)blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+       e < n_events; e += n_waves) {
+    const float4 a = ev_a[e];
+    const float4 b = ev_b[e];
+    const uint2 box = ev_box[e];
+    const int x0 = box.x & 0xffff, x1 = box.x >> 16;
+    const int y0 = box.y & 0xffff, y1 = box.y >> 16;
+    const int width = x1 - x0, height = y1 - y0;
+    if (width <= 0 || height <= 0) {
+      if (!BACKPROJECT && lane == 0) denominators[e] = 0.0f;
+      continue;
+    }
+    const int n = width * height;
+    const float z = a.x, y = a.y, tan = a.z, sec = a.w;
+    const float inv_width_f = 1.0f / (float)width;
+
+    float acc = 0.0f;
+    for (int idx = lane; idx < n; idx += 64) {
+      int r, c;
+      if (legacy_split) {
+        // warp_granularity.cuh:162-176 (float reciprocal row split)
+        r = (int)(short)(int)((float)idx * inv_width_f);
+        c = idx - width * r;
+      } else {
+        r = idx / width;
+        c = idx - r * width;
+      }
+      const int ix = x0 + c, iy = y0 + r;
+      // the float split can step one column outside the bbox; keep every
+      // access inside the image
+      if (!FAST && (ix < 0 || ix >= g.n_z || iy >= g.n_y)) continue;
+      const float dz = pixel_center_z(g, ix) - z;
+      const float dy = pixel_center_y(g, iy) - y;
+      const float q = ellipse_q_exact(b.x, b.y, b.z, dy, dz);
+      if (q <= 9.0f) {
+        const int p = iy * g.n_z + ix;
+        float k;
+        if (FAST || m.kernel == 0)
+          k = gauss_row_prefactor(g, dy + y, tan, sec) * gauss_exp(q);
+        else
+          k = analytic_kernel(g, y, tan, sec, dz, dy);
+        float kr = k * rho[p];
+        if (!FAST && m.sensitivity == 1) kr *= sens[p];
+        acc += kr;
+      }
+    }
+    const float d = wave_sum(acc);
+    if (!BACKPROJECT) {
+      if (lane == 0) denominators[e] = d;
+      continue;
+    }
+    float inv;
+    if (FAST || m.guard_zero)
+      inv = (d > 0.0f) ? 1.0f / d : 0.0f;
+    else
+      inv = 1.0f / d;
+
+    for (int idx = lane; idx < n; idx += 64) {
+      int r, c;
+      if (legacy_split) {
+        r = (int)(short)(int)((float)idx * inv_width_f);
+        c = idx - width * r;
+      } else {
+        r = idx / width;
+        c = idx - r * width;
+      }
+      const int ix = x0 + c, iy = y0 + r;
+      // the float split can step one column outside the bbox; keep every
+      // access inside the image
+      if (!FAST && (ix < 0 || ix >= g.n_z || iy >= g.n_y)) continue;
+      const float dz = pixel_center_z(g, ix) - z;
+      const float dy = pixel_center_y(g, iy) - y;
+      const float q = ellipse_q_exact(b.x, b.y, b.z, dy, dz);
+      if (q <= 9.0f) {
+        const int p = iy * g.n_z + ix;
+        float k;
+        if (FAST || m.kernel == 0)
+          k = gauss_row_prefactor(g, dy + y, tan, sec) * gauss_exp(q);
+        else
+          k = analytic_kernel(g, y, tan, sec, dz, dy);
+        unsafeAtomicAdd(&bp[p], k * inv);
+      }
+    }
+  }
+}
+
+// rho'(p) = rho(p) * bp(p) [/ s(p) in numerator mode]; clears bp for the
+// next iteration (a fill kernel rather than hipMemsetAsync, so iterations can
+// be captured in a HIP graph).
+__global__ __launch_bounds__(256) void finalize_kernel(
+    int n_pixels, const float* __restrict__ rho, float* __restrict__ bp,
+    const float* __restrict__ sens, int divide_by_sens,
+    float* __restrict__ rho_next) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_pixels;
+       i += gridDim.x * blockDim.x) {
+    float v = rho[i] * bp[i];
+    if (divide_by_sens) v = v / sens[i];
+    rho_next[i] = v;
+    bp[i] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ out,
+                                                   int64_t n, float value) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = value;
+}
+
+// Exact kernel-evaluation count sum_e P_e: one wave per event, same support
+// test as the iteration kernels.
+__global__ __launch_bounds__(256) void count_support_kernel(
+    DeviceGeometry g, const float4* __restrict__ ev_a,
+    const float4* __restrict__ ev_b, const uint2* __restrict__ ev_box,
+    int64_t n_events, unsigned long long* __restrict__ total) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves_per_block = blockDim.x >> 6;
+  const int64_t n_waves = (int64_t)gridDim.x * waves_per_block;
+  unsigned long long mine = 0;
+  for (int64_t e = (int64_t)blockIdx.x * waves_per_block + (threadIdx.x >> 6);
+       e < n_events; e += n_waves) {
+    const float4 a = ev_a[e];
+    const float4 b = ev_b[e];
+    const uint2 box = ev_box[e];
+    const int x0 = box.x & 0xffff, x1 = box.x >> 16;
+    const int y0 = box.y & 0xffff, y1 = box.y >> 16;
+    const int width = x1 - x0, height = y1 - y0;
+    if (width <= 0 || height <= 0) continue;
+    const int n = width * height;
+    int cnt = 0;
+    for (int idx = lane; idx < n; idx += 64) {
+      int r = idx / width, c = idx - r * width;
+      const float dz = pixel_center_z(g, x0 + c) - a.x;
+      const float dy = pixel_center_y(g, y0 + r) - a.y;
+      cnt += ellipse_q_exact(b.x, b.y, b.z, dy, dz) <= 9.0f;
+    }
+    mine += (unsigned long long)wave_sum_int(cnt);
+  }
+  if (lane == 0 && mine) atomicAdd(total, mine);
+}
+
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+namespace strip {
+
+// Scanner<float,short> constants (scanner.h:39-67) and GaussianKernel
+// constants (gaussian_kernel.h:27-33) in FP32, as the reference computes
+// them on the host.
+DeviceGeometry make_device_geometry(const strip_mlem_geometry& geo) {
+#pragma clang fp contract(off)
+  DeviceGeometry g;
+  const float R = (float)geo.radius, L = (float)geo.scintillator_length;
+  const float w = (float)geo.pixel_width, h = (float)geo.pixel_height;
+  const float cy = (float)geo.center_y, cz = (float)geo.center_z;
+  const short ny = (short)geo.n_y_pixels, nz = (short)geo.n_z_pixels;
+  const float size_y = (float)ny * h;  // scanner.h:61
+  const float size_z = (float)nz * w;  // scanner.h:62
+  g.radius = R;
+  g.half_length = L / 2.0f;
+  g.pixel_width = w;
+  g.pixel_height = h;
+  g.tl_y = cy + size_y / 2.0f;
+  g.tl_z = cz - size_z / 2.0f;
+  g.tl_y_half_h = g.tl_y - h / 2.0f;
+  g.tl_z_half_w = g.tl_z + w / 2.0f;
+  const float sz = (float)geo.sigma_z, sdl = (float)geo.sigma_dl;
+  g.inv_pow_sigma_z = 1.0f / (sz * sz);
+  g.inv_pow_sigma_dl = 1.0f / (sdl * sdl);
+  // 1/(2 pi^2) folded in double (M_PI is a double), sqrt taken in float
+  g.norm_c = (float)(1.0 / (2.0 * M_PI * M_PI) *
+                     (double)std::sqrt(g.inv_pow_sigma_z * g.inv_pow_sigma_z *
+                                       g.inv_pow_sigma_dl));
+  g.n_z = nz;
+  g.n_y = ny;
+  return g;
+}
+
+}  // namespace strip
+
+struct strip_mlem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  strip_mlem_geometry geometry{};
+  strip_mlem_mode mode{};
+  DeviceGeometry g{};
+  int n_pixels = 0;
+  int64_t n_events = 0, cap_events = 0;
+  EventSoA ev{nullptr, nullptr, nullptr, nullptr};
+  float* rho[2] = {nullptr, nullptr};
+  int cur = 0;
+  float* bp = nullptr;
+  float* sens = nullptr;  // map used by the mode (only when it needs one)
+  float* sens_out = nullptr;  // scratch for strip_mlem_get_sensitivity
+  float* staging = nullptr;   // AoS staging for host uploads
+  int64_t staging_cap = 0;
+  float* denominators = nullptr;
+  int64_t denominators_cap = 0;
+  unsigned long long* counter = nullptr;
+  int n_blocks = 0, threads = 256;
+  int algorithm = 0;  // requested: 0 = automatic, 1 = bbox, 2 = row band
+  int cu_count = 0;
+  strip::BandPlan band;
+  bool band_dirty = true;
+  char name[256] = {0};
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  size_t event_used = 0;
+  double forward_ms = 0.0, backward_ms = 0.0, iteration_ms = 0.0;
+  int64_t timed_iterations = 0;
+};
+
+namespace {
+
+#define CTX_CHECK(ctx)                                               \
+  do {                                                               \
+    if (!(ctx)) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL"); \
+    HIP_TRY(hipSetDevice((ctx)->device));                            \
+  } while (0)
+
+int check_geometry(const strip_mlem_geometry* g) {
+  if (!g) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "geometry is NULL");
+  if (g->n_z_pixels <= 0 || g->n_y_pixels <= 0 || g->n_z_pixels > 32767 ||
+      g->n_y_pixels > 32767)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "pixel counts must be in [1, 32767]");
+  if (!(g->pixel_width > 0) || !(g->pixel_height > 0))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "pixel size must be > 0");
+  if (!(g->sigma_z > 0) || !(g->sigma_dl > 0))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "sigmas must be > 0");
+  if (!(g->radius > 0))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "radius must be > 0");
+  return STRIP_MLEM_OK;
+}
+
+int check_mode(const strip_mlem_mode* m) {
+  if (m->kernel < 0 || m->kernel > 1 || m->sensitivity < 0 ||
+      m->sensitivity > 2 || m->sensitivity_map < 0 || m->sensitivity_map > 1)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "invalid mode");
+  return STRIP_MLEM_OK;
+}
+
+bool is_fast_mode(const strip_mlem_mode& m) {
+  return m.kernel == 0 && m.sensitivity == 0 && m.guard_zero != 0;
+}
+
+int grid_for_events(const strip_mlem_ctx* ctx, int64_t n_events) {
+  const int waves_per_block = ctx->threads / 64;
+  int64_t needed = (n_events + waves_per_block - 1) / waves_per_block;
+  int64_t cap = ctx->n_blocks > 0 ? ctx->n_blocks
+                                  : (int64_t)std::max(ctx->cu_count, 1) * 16;
+  return (int)std::max<int64_t>(1, std::min(needed, cap));
+}
+
+hipEvent_t next_event(strip_mlem_ctx* ctx) {
+  if (ctx->event_used == ctx->event_pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->event_pool.push_back(e);
+  }
+  return ctx->event_pool[ctx->event_used++];
+}
+
+// Sums the recorded (start, after forward, after back-projection, end)
+// quadruples.
+int collect_timing(strip_mlem_ctx* ctx) {
+  if (ctx->event_used == 0) return STRIP_MLEM_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (size_t i = 0; i + 4 <= ctx->event_used; i += 4) {
+    float f = 0.f, b = 0.f, t = 0.f;
+    HIP_TRY(hipEventElapsedTime(&f, ctx->event_pool[i], ctx->event_pool[i + 1]));
+    HIP_TRY(hipEventElapsedTime(&b, ctx->event_pool[i + 1], ctx->event_pool[i + 2]));
+    HIP_TRY(hipEventElapsedTime(&t, ctx->event_pool[i], ctx->event_pool[i + 3]));
+    ctx->forward_ms += f;
+    ctx->backward_ms += b;
+    ctx->iteration_ms += t;
+    ctx->timed_iterations += 1;
+  }
+  ctx->event_used = 0;
+  return STRIP_MLEM_OK;
+}
+
+int ensure_mode_sensitivity(strip_mlem_ctx* ctx) {
+  if (ctx->mode.sensitivity == 0 || ctx->sens) return STRIP_MLEM_OK;
+  HIP_TRY(hipMalloc(&ctx->sens, sizeof(float) * ctx->n_pixels));
+  const int blocks = (ctx->n_pixels + 255) / 256;
+  hipLaunchKernelGGL(sensitivity_kernel, dim3(blocks), dim3(256), 0,
+                     ctx->stream, ctx->g, ctx->mode.sensitivity_map,
+                     ctx->sens);
+  HIP_TRY(hipGetLastError());
+  return STRIP_MLEM_OK;
+}
+
+int reserve_events(strip_mlem_ctx* ctx, int64_t n) {
+  if (n <= ctx->cap_events) return STRIP_MLEM_OK;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ctx->ev.a) HIP_TRY(hipFree(ctx->ev.a));
+  if (ctx->ev.b) HIP_TRY(hipFree(ctx->ev.b));
+  if (ctx->ev.c) HIP_TRY(hipFree(ctx->ev.c));
+  if (ctx->ev.box) HIP_TRY(hipFree(ctx->ev.box));
+  ctx->ev = EventSoA{nullptr, nullptr, nullptr, nullptr};
+  ctx->cap_events = 0;
+  if (hipMalloc(&ctx->ev.a, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&ctx->ev.b, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&ctx->ev.c, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&ctx->ev.box, sizeof(uint2) * n) != hipSuccess)
+    return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate event arrays");
+  ctx->cap_events = n;
+  return STRIP_MLEM_OK;
+}
+
+int run_setup(strip_mlem_ctx* ctx, const float* d_responses, int64_t offset,
+              int64_t n) {
+  if (n <= 0) return STRIP_MLEM_OK;
+  const int64_t blocks = (n + 255) / 256;
+  hipLaunchKernelGGL(setup_events_kernel, dim3((unsigned)blocks), dim3(256),
+                     0, ctx->stream, ctx->g, d_responses, n,
+                     ctx->ev.a + offset, ctx->ev.b + offset,
+                     ctx->ev.c + offset, ctx->ev.box + offset);
+  HIP_TRY(hipGetLastError());
+  return STRIP_MLEM_OK;
+}
+
+ModeFlags mode_flags(const strip_mlem_mode& m) {
+  return ModeFlags{m.kernel, m.sensitivity, m.guard_zero};
+}
+
+// One event pass: forward + back-projection of every resident event into bp.
+int launch_event_pass(strip_mlem_ctx* ctx, const float* rho) {
+  if (ctx->n_events == 0) return STRIP_MLEM_OK;
+  const int grid = grid_for_events(ctx, ctx->n_events);
+  if (is_fast_mode(ctx->mode)) {
+    hipLaunchKernelGGL((bbox_event_kernel<true, true>), dim3(grid),
+                       dim3(ctx->threads), 0, ctx->stream, ctx->g,
+                       mode_flags(ctx->mode), ctx->ev.a, ctx->ev.b,
+                       ctx->ev.box, ctx->n_events, rho, ctx->sens, ctx->bp,
+                       nullptr);
+  } else {
+    hipLaunchKernelGGL((bbox_event_kernel<false, true>), dim3(grid),
+                       dim3(ctx->threads), 0, ctx->stream, ctx->g,
+                       mode_flags(ctx->mode), ctx->ev.a, ctx->ev.b,
+                       ctx->ev.box, ctx->n_events, rho, ctx->sens, ctx->bp,
+                       nullptr);
+  }
+  HIP_TRY(hipGetLastError());
+  return STRIP_MLEM_OK;
+}
+
+bool band_supported(const strip_mlem_ctx* ctx) {
+  return ctx->mode.kernel == 0 && ctx->mode.sensitivity != 1 &&
+         strip::band_rows_for(ctx->g) != 0;
+}
+
+int effective_algorithm(const strip_mlem_ctx* ctx) {
+  if (ctx->algorithm == 1) return 1;
+  if (ctx->algorithm == 2) return 2;
+  return band_supported(ctx) ? 2 : 1;
+}
+
+int record_marks(strip_mlem_ctx* ctx, hipEvent_t* marks) {
+  for (int i = 0; i < 4; ++i) {
+    marks[i] = next_event(ctx);
+    if (!marks[i])
+      return strip::fail(STRIP_MLEM_ERR_HIP, "hipEventCreate failed");
+  }
+  return STRIP_MLEM_OK;
+}
+
+int launch_iteration(strip_mlem_ctx* ctx) {
+  const float* rho = ctx->rho[ctx->cur];
+  float* rho_next = ctx->rho[ctx->cur ^ 1];
+  hipEvent_t marks[4] = {nullptr, nullptr, nullptr, nullptr};
+  if (ctx->timing) {
+    int rc = record_marks(ctx, marks);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(marks[0], ctx->stream));
+  }
+  int rc = launch_event_pass(ctx, rho);
+  if (rc) return rc;
+  if (ctx->timing) {
+    HIP_TRY(hipEventRecord(marks[1], ctx->stream));
+    HIP_TRY(hipEventRecord(marks[2], ctx->stream));
+  }
+  const int blocks = std::max(1, std::min((ctx->n_pixels + 255) / 256, 4096));
+  hipLaunchKernelGGL(finalize_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                     ctx->n_pixels, rho, ctx->bp, ctx->sens,
+                     ctx->mode.sensitivity == 2 ? 1 : 0, rho_next);
+  HIP_TRY(hipGetLastError());
+  if (ctx->timing) HIP_TRY(hipEventRecord(marks[3], ctx->stream));
+  ctx->cur ^= 1;
+  return STRIP_MLEM_OK;
+}
+
+// Row-band plan for the resident events (rebuilt after set_events).
+int ensure_band(strip_mlem_ctx* ctx) {
+  if (!ctx->band_dirty && ctx->band.ready) return STRIP_MLEM_OK;
+  std::string err;
+  if (strip::band_prepare(ctx->band, ctx->g, ctx->ev, ctx->n_events,
+                          ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, "band plan: " + err);
+  ctx->band_dirty = false;
+  return STRIP_MLEM_OK;
+}
+
+int band_iterations(strip_mlem_ctx* ctx, int n_iterations) {
+  int rc = ensure_band(ctx);
+  if (rc) return rc;
+  std::string err;
+  if (strip::band_load_images(ctx->band, ctx->g, ctx->rho[ctx->cur],
+                              ctx->mode.sensitivity == 2 ? ctx->sens : nullptr,
+                              ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  for (int i = 0; i < n_iterations; ++i) {
+    hipEvent_t marks[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (ctx->timing) {
+      if ((rc = record_marks(ctx, marks))) return rc;
+      HIP_TRY(hipEventRecord(marks[0], ctx->stream));
+    }
+    if (strip::band_iteration(ctx->band, ctx->g, ctx->ev,
+                              ctx->mode.sensitivity, ctx->mode.guard_zero,
+                              ctx->stream, marks[1], marks[2], &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    if (ctx->timing) HIP_TRY(hipEventRecord(marks[3], ctx->stream));
+  }
+  if (strip::band_store_image(ctx->band, ctx->g, ctx->rho[ctx->cur],
+                              ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  return STRIP_MLEM_OK;
+}
+
+int fill(strip_mlem_ctx* ctx, float* ptr, int64_t n, float value) {
+  int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
+                     ptr, n, value);
+  HIP_TRY(hipGetLastError());
+  return STRIP_MLEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* strip_mlem_last_error(void) { return strip::g_last_error.c_str(); }
+
+int strip_mlem_create(const strip_mlem_geometry* geometry,
+                      const strip_mlem_mode* mode, int device,
+                      strip_mlem_ctx** out) {
+  strip::g_last_error.clear();
+  if (!out) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int rc = check_geometry(geometry);
+  if (rc) return rc;
+  strip_mlem_mode m = {0, 0, 0, 1};
+  if (mode) {
+    rc = check_mode(mode);
+    if (rc) return rc;
+    m = *mode;
+  }
+  int n_devices = 0;
+  HIP_TRY(hipGetDeviceCount(&n_devices));
+  if (device < 0 || device >= n_devices)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "device index out of range (" +
+                           std::to_string(n_devices) + " devices)");
+  HIP_TRY(hipSetDevice(device));
+  strip_mlem_ctx* ctx = new strip_mlem_ctx;
+  ctx->device = device;
+  ctx->geometry = *geometry;
+  ctx->mode = m;
+  ctx->g = strip::make_device_geometry(*geometry);
+  ctx->n_pixels = geometry->n_z_pixels * geometry->n_y_pixels;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    ctx->cu_count = prop.multiProcessorCount;
+    std::snprintf(ctx->name, sizeof(ctx->name), "%s (%s)", prop.name,
+                  prop.gcnArchName);
+  }
+  auto cleanup = [&](int code) {
+    strip_mlem_destroy(ctx);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) !=
+      hipSuccess)
+    return cleanup(strip::fail(STRIP_MLEM_ERR_HIP, "hipStreamCreate failed"));
+  if (hipMalloc(&ctx->rho[0], sizeof(float) * ctx->n_pixels) != hipSuccess ||
+      hipMalloc(&ctx->rho[1], sizeof(float) * ctx->n_pixels) != hipSuccess ||
+      hipMalloc(&ctx->bp, sizeof(float) * ctx->n_pixels) != hipSuccess ||
+      hipMalloc(&ctx->counter, sizeof(unsigned long long)) != hipSuccess)
+    return cleanup(strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate images"));
+  if ((rc = fill(ctx, ctx->rho[0], ctx->n_pixels, 1.0f)) ||
+      (rc = fill(ctx, ctx->bp, ctx->n_pixels, 0.0f)) ||
+      (rc = ensure_mode_sensitivity(ctx)))
+    return cleanup(rc);
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return cleanup(strip::fail(STRIP_MLEM_ERR_HIP, "initialisation failed"));
+  *out = ctx;
+  return STRIP_MLEM_OK;
+}
+
+void strip_mlem_destroy(strip_mlem_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
+  (void)hipFree(ctx->ev.a);
+  (void)hipFree(ctx->ev.b);
+  (void)hipFree(ctx->ev.c);
+  (void)hipFree(ctx->ev.box);
+  strip::band_release(ctx->band);
+  (void)hipFree(ctx->rho[0]);
+  (void)hipFree(ctx->rho[1]);
+  (void)hipFree(ctx->bp);
+  (void)hipFree(ctx->sens);
+  (void)hipFree(ctx->sens_out);
+  (void)hipFree(ctx->staging);
+  (void)hipFree(ctx->denominators);
+  (void)hipFree(ctx->counter);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+int strip_mlem_set_events(strip_mlem_ctx* ctx, const float* responses,
+                          int64_t n) {
+  CTX_CHECK(ctx);
+  if (n < 0 || (n > 0 && !responses))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad event array");
+  int rc = reserve_events(ctx, n);
+  if (rc) return rc;
+  // upload in chunks through one device staging buffer
+  const int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 24);
+  if (chunk > ctx->staging_cap) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->staging) HIP_TRY(hipFree(ctx->staging));
+    ctx->staging = nullptr;
+    ctx->staging_cap = 0;
+    if (hipMalloc(&ctx->staging, sizeof(float) * 3 * chunk) != hipSuccess)
+      return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate staging");
+    ctx->staging_cap = chunk;
+  }
+  for (int64_t off = 0; off < n; off += chunk) {
+    const int64_t m = std::min(chunk, n - off);
+    HIP_TRY(hipMemcpyAsync(ctx->staging, responses + 3 * off,
+                           sizeof(float) * 3 * m, hipMemcpyHostToDevice,
+                           ctx->stream));
+    if ((rc = run_setup(ctx, ctx->staging, off, m))) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  ctx->n_events = n;
+  ctx->band_dirty = true;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_events_device(strip_mlem_ctx* ctx,
+                                 const float* device_responses, int64_t n) {
+  CTX_CHECK(ctx);
+  if (n < 0 || (n > 0 && !device_responses))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad event array");
+  int rc = reserve_events(ctx, n);
+  if (rc) return rc;
+  if ((rc = run_setup(ctx, device_responses, 0, n))) return rc;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->n_events = n;
+  ctx->band_dirty = true;
+  return STRIP_MLEM_OK;
+}
+
+int64_t strip_mlem_n_events(const strip_mlem_ctx* ctx) {
+  return ctx ? ctx->n_events : -1;
+}
+
+int strip_mlem_set_rho(strip_mlem_ctx* ctx, const float* rho) {
+  CTX_CHECK(ctx);
+  if (!rho) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "rho is NULL");
+  HIP_TRY(hipMemcpyAsync(ctx->rho[ctx->cur], rho,
+                         sizeof(float) * ctx->n_pixels, hipMemcpyHostToDevice,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_rho(strip_mlem_ctx* ctx, float* rho) {
+  CTX_CHECK(ctx);
+  if (!rho) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "rho is NULL");
+  HIP_TRY(hipMemcpyAsync(rho, ctx->rho[ctx->cur],
+                         sizeof(float) * ctx->n_pixels, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_rho_device(strip_mlem_ctx* ctx, const float* device_rho) {
+  CTX_CHECK(ctx);
+  if (!device_rho) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "rho is NULL");
+  HIP_TRY(hipMemcpyAsync(ctx->rho[ctx->cur], device_rho,
+                         sizeof(float) * ctx->n_pixels,
+                         hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_rho_device(strip_mlem_ctx* ctx, float* device_rho) {
+  CTX_CHECK(ctx);
+  if (!device_rho) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "rho is NULL");
+  HIP_TRY(hipMemcpyAsync(device_rho, ctx->rho[ctx->cur],
+                         sizeof(float) * ctx->n_pixels,
+                         hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_sensitivity(strip_mlem_ctx* ctx, int kind,
+                               float* sensitivity) {
+  CTX_CHECK(ctx);
+  if (!sensitivity || kind < 0 || kind > 1)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad sensitivity request");
+  if (!ctx->sens_out)
+    HIP_TRY(hipMalloc(&ctx->sens_out, sizeof(float) * ctx->n_pixels));
+  const int blocks = (ctx->n_pixels + 255) / 256;
+  hipLaunchKernelGGL(sensitivity_kernel, dim3(blocks), dim3(256), 0,
+                     ctx->stream, ctx->g, kind, ctx->sens_out);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(sensitivity, ctx->sens_out,
+                         sizeof(float) * ctx->n_pixels, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_iterate_async(strip_mlem_ctx* ctx, int n_iterations) {
+  CTX_CHECK(ctx);
+  if (n_iterations < 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "negative iteration count");
+  if (n_iterations == 0) return STRIP_MLEM_OK;
+  if (effective_algorithm(ctx) == 2) return band_iterations(ctx, n_iterations);
+  for (int i = 0; i < n_iterations; ++i) {
+    int rc = launch_iteration(ctx);
+    if (rc) return rc;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_synchronize(strip_mlem_ctx* ctx) {
+  CTX_CHECK(ctx);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_iterate(strip_mlem_ctx* ctx, int n_iterations) {
+  int rc = strip_mlem_iterate_async(ctx, n_iterations);
+  if (rc) return rc;
+  return strip_mlem_synchronize(ctx);
+}
+
+int strip_mlem_set_launch(strip_mlem_ctx* ctx, int n_blocks,
+                          int threads_per_block) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (n_blocks < 0 || threads_per_block < 0 || threads_per_block % 64 ||
+      threads_per_block > 256)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "threads per block must be a multiple of 64, <= 256");
+  ctx->n_blocks = n_blocks;
+  ctx->threads = threads_per_block ? threads_per_block : 256;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_algorithm(strip_mlem_ctx* ctx, int algorithm) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (algorithm < 0 || algorithm > 2)
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED, "unknown algorithm");
+  if (algorithm == 2 && !band_supported(ctx))
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "the row-band algorithm needs the Gaussian kernel, no "
+                       "sensitivity in the denominator and n_z <= 8192");
+  ctx->algorithm = algorithm;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx) {
+  return ctx ? effective_algorithm(ctx) : -1;
+}
+
+int strip_mlem_kernel_evaluations(strip_mlem_ctx* ctx, int64_t* count) {
+  CTX_CHECK(ctx);
+  if (!count) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "count is NULL");
+  HIP_TRY(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long),
+                         ctx->stream));
+  if (ctx->n_events > 0 && effective_algorithm(ctx) == 2) {
+    int rc = ensure_band(ctx);
+    if (rc) return rc;
+    std::string err;
+    if (strip::band_count(ctx->band, ctx->g, ctx->ev, ctx->counter,
+                          ctx->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  } else if (ctx->n_events > 0) {
+    const int grid = grid_for_events(ctx, ctx->n_events);
+    hipLaunchKernelGGL(count_support_kernel, dim3(grid), dim3(ctx->threads), 0,
+                       ctx->stream, ctx->g, ctx->ev.a, ctx->ev.b, ctx->ev.box,
+                       ctx->n_events, ctx->counter);
+    HIP_TRY(hipGetLastError());
+  }
+  unsigned long long v = 0;
+  HIP_TRY(hipMemcpyAsync(&v, ctx->counter, sizeof(v), hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  *count = (int64_t)v;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_forward(strip_mlem_ctx* ctx, float* denominators) {
+  CTX_CHECK(ctx);
+  if (!denominators && ctx->n_events)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "denominators is NULL");
+  if (ctx->n_events == 0) return STRIP_MLEM_OK;
+  if (ctx->denominators_cap < ctx->n_events) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->denominators) HIP_TRY(hipFree(ctx->denominators));
+    ctx->denominators = nullptr;
+    ctx->denominators_cap = 0;
+    HIP_TRY(hipMalloc(&ctx->denominators, sizeof(float) * ctx->n_events));
+    ctx->denominators_cap = ctx->n_events;
+  }
+  if (effective_algorithm(ctx) == 2) {
+    int rc = ensure_band(ctx);
+    if (rc) return rc;
+    std::string err;
+    if (strip::band_load_images(ctx->band, ctx->g, ctx->rho[ctx->cur],
+                                nullptr, ctx->stream, &err) ||
+        strip::band_forward_only(ctx->band, ctx->g, ctx->ev,
+                                 ctx->denominators, ctx->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    HIP_TRY(hipMemcpyAsync(denominators, ctx->denominators,
+                           sizeof(float) * ctx->n_events,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return STRIP_MLEM_OK;
+  }
+  const int grid = grid_for_events(ctx, ctx->n_events);
+  const float* rho = ctx->rho[ctx->cur];
+  if (is_fast_mode(ctx->mode))
+    hipLaunchKernelGGL((bbox_event_kernel<true, false>), dim3(grid),
+                       dim3(ctx->threads), 0, ctx->stream, ctx->g,
+                       mode_flags(ctx->mode), ctx->ev.a, ctx->ev.b,
+                       ctx->ev.box, ctx->n_events, rho, ctx->sens, ctx->bp,
+                       ctx->denominators);
+  else
+    hipLaunchKernelGGL((bbox_event_kernel<false, false>), dim3(grid),
+                       dim3(ctx->threads), 0, ctx->stream, ctx->g,
+                       mode_flags(ctx->mode), ctx->ev.a, ctx->ev.b,
+                       ctx->ev.box, ctx->n_events, rho, ctx->sens, ctx->bp,
+                       ctx->denominators);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(denominators, ctx->denominators,
+                         sizeof(float) * ctx->n_events, hipMemcpyDeviceToHost,
+                         ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_timing(strip_mlem_ctx* ctx, int enable) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  int rc = collect_timing(ctx);
+  if (rc) return rc;
+  ctx->timing = enable != 0;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_timing(strip_mlem_ctx* ctx, double* kernel_ms,
+                          double* iteration_ms, int64_t* n_iterations) {
+  double f = 0, b = 0;
+  int rc = strip_mlem_get_timing_detail(ctx, &f, &b, iteration_ms,
+                                        n_iterations);
+  if (rc) return rc;
+  if (kernel_ms) *kernel_ms = f + b;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_timing_detail(strip_mlem_ctx* ctx, double* forward_ms,
+                                 double* backward_ms, double* iteration_ms,
+                                 int64_t* n_iterations) {
+  CTX_CHECK(ctx);
+  int rc = collect_timing(ctx);
+  if (rc) return rc;
+  if (forward_ms) *forward_ms = ctx->forward_ms;
+  if (backward_ms) *backward_ms = ctx->backward_ms;
+  if (iteration_ms) *iteration_ms = ctx->iteration_ms;
+  if (n_iterations) *n_iterations = ctx->timed_iterations;
+  ctx->forward_ms = ctx->backward_ms = ctx->iteration_ms = 0.0;
+  ctx->timed_iterations = 0;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_device_name(strip_mlem_ctx* ctx, char* buffer, int size) {
+  if (!ctx || !buffer || size <= 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad arguments");
+  std::snprintf(buffer, (size_t)size, "%s", ctx->name);
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_run(const strip_mlem_geometry* geometry,
+                   const strip_mlem_mode* mode, const float* responses,
+                   int n_responses, float* rho_inout, int n_iteration_blocks,
+                   int n_iterations_in_block, strip_mlem_output_fn output,
+                   strip_mlem_progress_fn progress, void* user, int device,
+                   int n_blocks, int n_threads_per_block,
+                   strip_mlem_device_name_fn device_name) {
+  if (!rho_inout)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "rho_inout is NULL");
+  if (n_iteration_blocks < 0 || n_iterations_in_block < 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "negative iteration counts");
+  strip_mlem_ctx* ctx = nullptr;
+  int rc = strip_mlem_create(geometry, mode, device, &ctx);
+  if (rc) return rc;
+  struct Guard {
+    strip_mlem_ctx* c;
+    ~Guard() { strip_mlem_destroy(c); }
+  } guard{ctx};
+  if (device_name) device_name(user, ctx->name);
+  if (n_threads_per_block > 256) n_threads_per_block = 256;
+  n_threads_per_block -= n_threads_per_block % 64;
+  if ((rc = strip_mlem_set_launch(ctx, n_blocks > 0 ? n_blocks : 0,
+                                  n_threads_per_block > 0 ? n_threads_per_block
+                                                          : 0)))
+    return rc;
+  const int nz = geometry->n_z_pixels, ny = geometry->n_y_pixels;
+  // cuda/reconstruction.cu:63-70: the 5-point sensitivity goes out first
+  std::vector<float> host_image((size_t)nz * ny);
+  if ((rc = strip_mlem_get_sensitivity(ctx, 1, host_image.data()))) return rc;
+  if (output) output(user, -1, host_image.data(), nz, ny);
+  if ((rc = strip_mlem_set_rho(ctx, rho_inout))) return rc;
+  if ((rc = strip_mlem_set_events(ctx, responses, n_responses))) return rc;
+  for (int block = 0; block < n_iteration_blocks; ++block) {
+    for (int i = 0; i < n_iterations_in_block; ++i) {
+      const int iteration = block * n_iterations_in_block + i;
+      if (progress) progress(user, iteration, 0);
+      if ((rc = strip_mlem_iterate(ctx, 1))) return rc;
+      if (progress) progress(user, iteration, 1);
+      // cuda/reconstruction.cu:98-104: early outputs of block 0
+      if (!block && i < n_iterations_in_block - 1 &&
+          (i < 5 || i == 9 || i == 14 || i == 19 || i == 29 || i == 49 ||
+           i == 99)) {
+        if ((rc = strip_mlem_get_rho(ctx, rho_inout))) return rc;
+        if (output) output(user, i + 1, rho_inout, nz, ny);
+      }
+    }
+    if ((rc = strip_mlem_get_rho(ctx, rho_inout))) return rc;
+    if (output)
+      output(user, (block + 1) * n_iterations_in_block, rho_inout, nz, ny);
+  }
+  if (progress) progress(user, n_iteration_blocks * n_iterations_in_block, 0);
+  return STRIP_MLEM_OK;
+}
+
+}  // extern "C"

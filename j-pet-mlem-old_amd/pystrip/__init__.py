@@ -1,0 +1,349 @@
+"""Python binding over the C ABI of the MI355X strip MLEM library.
+
+This is a thin ctypes layer over ``include/strip_mlem.h``; every call runs the
+HIP kernels in ``lib/libstrip_mlem.so``.  There is no CPU fallback: if the
+library is missing, importing :class:`Reconstruction` raises.
+
+The class mirrors the reference's ``PET2D::Strip::Reconstruction`` usage
+(src/2d/strip/reconstruction.h:57-154, reconstruction_cmd.cpp:136-292):
+construct it from a scanner geometry, load responses (z_u, z_d, dl), run
+iteration blocks, read ``rho``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PACKAGE_ROOT = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PACKAGE_ROOT, "lib", "libstrip_mlem.so")
+
+STRIP_MLEM_OK = 0
+
+
+class StripMLEMError(RuntimeError):
+    """A C-ABI call returned a non-zero status."""
+
+
+class Geometry(ctypes.Structure):
+    """strip_mlem_geometry: Scanner(R, L, n_y, n_z, h, w, sigma_z, sigma_dl)
+    (src/2d/strip/scanner.h:39-67)."""
+
+    _fields_ = [
+        ("radius", ctypes.c_double),
+        ("scintillator_length", ctypes.c_double),
+        ("n_z_pixels", ctypes.c_int),
+        ("n_y_pixels", ctypes.c_int),
+        ("pixel_width", ctypes.c_double),
+        ("pixel_height", ctypes.c_double),
+        ("sigma_z", ctypes.c_double),
+        ("sigma_dl", ctypes.c_double),
+        ("center_y", ctypes.c_double),
+        ("center_z", ctypes.c_double),
+    ]
+
+    @classmethod
+    def make(cls, radius, length, n_z, n_y, pixel_size, sigma_z, sigma_dl,
+             pixel_height=None, center_y=0.0, center_z=0.0):
+        return cls(float(radius), float(length), int(n_z), int(n_y),
+                   float(pixel_size),
+                   float(pixel_size if pixel_height is None else pixel_height),
+                   float(sigma_z), float(sigma_dl), float(center_y),
+                   float(center_z))
+
+    def as_tuple(self):
+        return tuple(getattr(self, f) for f, _ in self._fields_)
+
+
+class Mode(ctypes.Structure):
+    """strip_mlem_mode: reconstruction semantics."""
+
+    _fields_ = [
+        ("kernel", ctypes.c_int),
+        ("sensitivity", ctypes.c_int),
+        ("sensitivity_map", ctypes.c_int),
+        ("guard_zero", ctypes.c_int),
+    ]
+
+
+# Reference CPU path (graded parity mode), reconstruction.h:220-298
+MODE_CPU = (0, 0, 0, 1)
+# Legacy CUDA warp kernel, cuda/reconstruction/warp_granularity.cuh:88-152
+MODE_LEGACY_GPU = (1, 1, 1, 0)
+
+_OUTPUT_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int,
+                              ctypes.POINTER(ctypes.c_float), ctypes.c_int,
+                              ctypes.c_int)
+_PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int,
+                                ctypes.c_int)
+_NAME_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_char_p)
+
+_lib = None
+
+
+def library_path() -> str:
+    return LIB_PATH
+
+
+def load_library():
+    """Loads lib/libstrip_mlem.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise StripMLEMError(
+            f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    i64 = ctypes.c_int64
+    fptr = ctypes.POINTER(ctypes.c_float)
+    sigs = {
+        "strip_mlem_create": (ctypes.c_int, [ctypes.POINTER(Geometry),
+                                             ctypes.POINTER(Mode), ctypes.c_int,
+                                             ctypes.POINTER(P)]),
+        "strip_mlem_destroy": (None, [P]),
+        "strip_mlem_set_events": (ctypes.c_int, [P, fptr, i64]),
+        "strip_mlem_set_events_device": (ctypes.c_int, [P, P, i64]),
+        "strip_mlem_n_events": (i64, [P]),
+        "strip_mlem_set_rho": (ctypes.c_int, [P, fptr]),
+        "strip_mlem_get_rho": (ctypes.c_int, [P, fptr]),
+        "strip_mlem_set_rho_device": (ctypes.c_int, [P, P]),
+        "strip_mlem_get_rho_device": (ctypes.c_int, [P, P]),
+        "strip_mlem_get_sensitivity": (ctypes.c_int, [P, ctypes.c_int, fptr]),
+        "strip_mlem_iterate": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_iterate_async": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_synchronize": (ctypes.c_int, [P]),
+        "strip_mlem_set_launch": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+        "strip_mlem_set_algorithm": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_get_algorithm": (ctypes.c_int, [P]),
+        "strip_mlem_kernel_evaluations": (ctypes.c_int,
+                                          [P, ctypes.POINTER(i64)]),
+        "strip_mlem_forward": (ctypes.c_int, [P, fptr]),
+        "strip_mlem_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_get_timing": (ctypes.c_int, [P,
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(i64)]),
+        "strip_mlem_get_timing_detail": (ctypes.c_int, [
+            P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(i64)]),
+        "strip_mlem_device_name": (ctypes.c_int, [P, ctypes.c_char_p,
+                                                  ctypes.c_int]),
+        "strip_mlem_last_error": (ctypes.c_char_p, []),
+        "strip_mlem_run": (ctypes.c_int, [ctypes.POINTER(Geometry),
+                                          ctypes.POINTER(Mode), fptr,
+                                          ctypes.c_int, fptr, ctypes.c_int,
+                                          ctypes.c_int, _OUTPUT_FN,
+                                          _PROGRESS_FN, P, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int,
+                                          _NAME_FN]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names of every C-ABI entry point the binding expects."""
+    return [
+        "strip_mlem_create", "strip_mlem_destroy", "strip_mlem_set_events",
+        "strip_mlem_set_events_device", "strip_mlem_n_events",
+        "strip_mlem_set_rho", "strip_mlem_get_rho", "strip_mlem_set_rho_device",
+        "strip_mlem_get_rho_device", "strip_mlem_get_sensitivity",
+        "strip_mlem_iterate", "strip_mlem_iterate_async",
+        "strip_mlem_synchronize", "strip_mlem_set_launch",
+        "strip_mlem_set_algorithm", "strip_mlem_get_algorithm",
+        "strip_mlem_kernel_evaluations", "strip_mlem_forward",
+        "strip_mlem_set_timing", "strip_mlem_get_timing",
+        "strip_mlem_get_timing_detail",
+        "strip_mlem_device_name", "strip_mlem_last_error", "strip_mlem_run",
+    ]
+
+
+def _check(rc):
+    if rc != STRIP_MLEM_OK:
+        msg = _lib.strip_mlem_last_error().decode(errors="replace")
+        raise StripMLEMError(f"status {rc}: {msg}")
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _as_events(responses) -> np.ndarray:
+    ev = np.ascontiguousarray(responses, dtype=np.float32)
+    if ev.ndim != 2 or ev.shape[1] != 3:
+        raise ValueError("responses must be an (N, 3) array of z_u, z_d, dl")
+    return ev
+
+
+class Reconstruction:
+    """GPU reconstruction context (one HIP stream, resident events/images)."""
+
+    def __init__(self, geometry: Geometry, mode=MODE_CPU, device: int = 0):
+        self._lib = load_library()
+        self.geometry = geometry
+        self.mode = Mode(*mode)
+        self._ctx = ctypes.c_void_p()
+        _check(self._lib.strip_mlem_create(ctypes.byref(geometry),
+                                           ctypes.byref(self.mode), device,
+                                           ctypes.byref(self._ctx)))
+        self.shape = (geometry.n_y_pixels, geometry.n_z_pixels)
+
+    def close(self):
+        if self._ctx:
+            self._lib.strip_mlem_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- events / images --------------------------------------------------
+    def set_events(self, responses):
+        ev = _as_events(responses)
+        _check(self._lib.strip_mlem_set_events(self._ctx, _fptr(ev),
+                                               ev.shape[0]))
+
+    def set_events_device(self, device_ptr: int, n: int):
+        _check(self._lib.strip_mlem_set_events_device(
+            self._ctx, ctypes.c_void_p(device_ptr), int(n)))
+
+    @property
+    def n_events(self) -> int:
+        return int(self._lib.strip_mlem_n_events(self._ctx))
+
+    @property
+    def rho(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float32)
+        _check(self._lib.strip_mlem_get_rho(self._ctx, _fptr(out)))
+        return out
+
+    @rho.setter
+    def rho(self, value):
+        v = np.ascontiguousarray(value, dtype=np.float32).reshape(self.shape)
+        _check(self._lib.strip_mlem_set_rho(self._ctx, _fptr(v)))
+
+    def set_rho_device(self, device_ptr: int):
+        _check(self._lib.strip_mlem_set_rho_device(self._ctx,
+                                                   ctypes.c_void_p(device_ptr)))
+
+    def get_rho_device(self, device_ptr: int):
+        _check(self._lib.strip_mlem_get_rho_device(self._ctx,
+                                                   ctypes.c_void_p(device_ptr)))
+
+    def sensitivity(self, kind: int = 0) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float32)
+        _check(self._lib.strip_mlem_get_sensitivity(self._ctx, kind,
+                                                    _fptr(out)))
+        return out
+
+    # -- iterations --------------------------------------------------------
+    def iterate(self, n: int = 1):
+        _check(self._lib.strip_mlem_iterate(self._ctx, int(n)))
+
+    def iterate_async(self, n: int = 1):
+        _check(self._lib.strip_mlem_iterate_async(self._ctx, int(n)))
+
+    def synchronize(self):
+        _check(self._lib.strip_mlem_synchronize(self._ctx))
+
+    def set_launch(self, n_blocks: int = 0, threads_per_block: int = 0):
+        _check(self._lib.strip_mlem_set_launch(self._ctx, n_blocks,
+                                               threads_per_block))
+
+    def set_algorithm(self, algorithm: int):
+        _check(self._lib.strip_mlem_set_algorithm(self._ctx, algorithm))
+
+    @property
+    def algorithm(self) -> int:
+        return int(self._lib.strip_mlem_get_algorithm(self._ctx))
+
+    def kernel_evaluations(self) -> int:
+        v = ctypes.c_int64()
+        _check(self._lib.strip_mlem_kernel_evaluations(self._ctx,
+                                                       ctypes.byref(v)))
+        return int(v.value)
+
+    def forward(self) -> np.ndarray:
+        out = np.empty(self.n_events, dtype=np.float32)
+        _check(self._lib.strip_mlem_forward(self._ctx, _fptr(out)))
+        return out
+
+    def set_timing(self, enable: bool):
+        _check(self._lib.strip_mlem_set_timing(self._ctx, int(bool(enable))))
+
+    def get_timing(self):
+        k = ctypes.c_double()
+        t = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(self._lib.strip_mlem_get_timing(self._ctx, ctypes.byref(k),
+                                               ctypes.byref(t),
+                                               ctypes.byref(n)))
+        return k.value, t.value, n.value
+
+    def get_timing_detail(self):
+        """(forward_ms, backward_ms, iteration_ms, n_iterations) summed over
+        the timed iterations since the last call."""
+        f = ctypes.c_double()
+        b = ctypes.c_double()
+        t = ctypes.c_double()
+        n = ctypes.c_int64()
+        _check(self._lib.strip_mlem_get_timing_detail(
+            self._ctx, ctypes.byref(f), ctypes.byref(b), ctypes.byref(t),
+            ctypes.byref(n)))
+        return f.value, b.value, t.value, n.value
+
+    def device_name(self) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(self._lib.strip_mlem_device_name(self._ctx, buf, 256))
+        return buf.value.decode(errors="replace")
+
+
+def run(geometry: Geometry, responses, rho, n_blocks: int,
+        n_iterations_in_block: int, mode=MODE_CPU, device: int = 0,
+        output=None, progress=None, device_name=None, cuda_blocks: int = 0,
+        cuda_threads: int = 0):
+    """The reference's GPU::Reconstruction::run protocol
+    (src/2d/strip/cuda/reconstruction.cu:31-113) over strip_mlem_run.
+    ``output(iteration, image)`` receives a copy of each output image."""
+    lib = load_library()
+    ev = _as_events(responses)
+    r = np.ascontiguousarray(rho, dtype=np.float32).reshape(
+        geometry.n_y_pixels, geometry.n_z_pixels).copy()
+
+    def _out(_user, it, img, nz, ny):
+        if output is not None:
+            arr = np.ctypeslib.as_array(img, shape=(ny * nz,)).reshape(ny, nz)
+            output(it, arr.copy())
+
+    def _prog(_user, completed, finished):
+        if progress is not None:
+            progress(completed, bool(finished))
+
+    def _name(_user, name):
+        if device_name is not None:
+            device_name(name.decode(errors="replace"))
+
+    cb_out, cb_prog, cb_name = _OUTPUT_FN(_out), _PROGRESS_FN(_prog), \
+        _NAME_FN(_name)
+    m = Mode(*mode)
+    _check(lib.strip_mlem_run(ctypes.byref(geometry), ctypes.byref(m),
+                              _fptr(ev), ev.shape[0], _fptr(r), n_blocks,
+                              n_iterations_in_block, cb_out, cb_prog, None,
+                              device, cuda_blocks, cuda_threads, cb_name))
+    return r

@@ -1,0 +1,251 @@
+"""ctypes front-end of the CPU oracle (TEST INFRASTRUCTURE ONLY).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module, and only as the checker.  The product path never imports it.
+
+``liboracle.so`` is the C restatement of the reference's CPU strip
+reconstruction (strip_oracle.c).  ``_ref/libstrip_ref.so`` is the reference's
+own code compiled from its headers (ref_harness.cpp); it exists only where
+the reference tree was available when the oracle was built.
+
+Parity status of each oracle function (what pins it) is listed in the header
+of strip_oracle.c; every function is pinned by a test in tests/.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libstrip_ref.so")
+
+
+class OracleGeometry(ctypes.Structure):
+    _fields_ = [
+        ("radius", ctypes.c_double),
+        ("scintillator_length", ctypes.c_double),
+        ("n_z_pixels", ctypes.c_int),
+        ("n_y_pixels", ctypes.c_int),
+        ("pixel_width", ctypes.c_double),
+        ("pixel_height", ctypes.c_double),
+        ("sigma_z", ctypes.c_double),
+        ("sigma_dl", ctypes.c_double),
+        ("center_y", ctypes.c_double),
+        ("center_z", ctypes.c_double),
+    ]
+
+
+class OracleMode(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_int),
+        ("sensitivity", ctypes.c_int),
+        ("sensitivity_map", ctypes.c_int),
+        ("guard_zero", ctypes.c_int),
+    ]
+
+
+MODE_CPU = (0, 0, 0, 1)
+MODE_LEGACY_GPU = (1, 1, 1, 0)
+
+_lib = None
+_ref = None
+
+
+def build(reference: bool = False):
+    """Compiles liboracle.so (and _ref/ when asked and possible)."""
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    if reference and os.path.isdir("/root/reference/src"):
+        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+
+
+def geometry(radius, length, n_z, n_y, pixel_w, pixel_h, sigma_z, sigma_dl,
+             center_y=0.0, center_z=0.0):
+    return OracleGeometry(float(radius), float(length), int(n_z), int(n_y),
+                          float(pixel_w), float(pixel_h), float(sigma_z),
+                          float(sigma_dl), float(center_y), float(center_z))
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        G = ctypes.POINTER(OracleGeometry)
+        M = ctypes.POINTER(OracleMode)
+        f = ctypes.c_float
+        d = ctypes.c_double
+        fp = ctypes.POINTER(f)
+        dp = ctypes.POINTER(d)
+        ip = ctypes.POINTER(ctypes.c_int)
+        i64 = ctypes.c_int64
+        i64p = ctypes.POINTER(i64)
+        sig = {
+            "oracle_event_params_f32": (None, [G, f, f, f, fp, ip]),
+            "oracle_event_params_f64": (None, [G, d, d, d, dp, ip]),
+            "oracle_gaussian_kernel_f32": (f, [f] * 8),
+            "oracle_gaussian_kernel_f64": (d, [d] * 8),
+            "oracle_analytic_kernel_f32": (f, [f] * 8),
+            "oracle_analytic_kernel_f64": (d, [d] * 8),
+            "oracle_bb_y_f64": (d, [d, d, d]),
+            "oracle_bb_z_f64": (d, [d, d, d]),
+            "oracle_pixel_at_f32": (None, [G, f, f, ip, ip]),
+            "oracle_pixel_at_f64": (None, [G, d, d, ip, ip]),
+            "oracle_pixel_center_f32": (None, [G, ctypes.c_int, ctypes.c_int, fp, fp]),
+            "oracle_pixel_center_f64": (None, [G, ctypes.c_int, ctypes.c_int, dp, dp]),
+            "oracle_sensitivity_f32": (f, [G, f, f]),
+            "oracle_sensitivity_f64": (d, [G, d, d]),
+            "oracle_sensitivity_map_f32": (None, [G, ctypes.c_int, fp]),
+            "oracle_sensitivity_map_f64": (None, [G, ctypes.c_int, dp]),
+            "oracle_to_projection_f64": (None, [d, d, d, d, dp, dp, dp]),
+            "oracle_mlem_f32": (ctypes.c_int, [G, M, fp, i64, fp, ctypes.c_int, ctypes.c_int]),
+            "oracle_mlem_f64": (ctypes.c_int, [G, M, dp, i64, dp, ctypes.c_int, ctypes.c_int]),
+            "oracle_forward_f32": (ctypes.c_int, [G, M, fp, i64, fp, fp, ctypes.c_int]),
+            "oracle_forward_f64": (ctypes.c_int, [G, M, dp, i64, dp, dp, ctypes.c_int]),
+            "oracle_support_counts_f32": (ctypes.c_int, [G, fp, i64, i64p, i64p, ctypes.c_int]),
+            "oracle_loglik_f64": (d, [G, M, dp, i64, dp, dp, ctypes.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref():
+    """The reference's own code (raises if oracle/_ref was not built)."""
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("oracle/_ref/libstrip_ref.so not built")
+        L = ctypes.CDLL(REF_LIB)
+        G = ctypes.POINTER(OracleGeometry)
+        f = ctypes.c_float
+        d = ctypes.c_double
+        fp = ctypes.POINTER(f)
+        dp = ctypes.POINTER(d)
+        ip = ctypes.POINTER(ctypes.c_int)
+        i64 = ctypes.c_int64
+        sig = {
+            "ref_mlem_f32": (ctypes.c_int, [G, fp, i64, fp, ctypes.c_int, ctypes.c_int]),
+            "ref_mlem_f64": (ctypes.c_int, [G, dp, i64, dp, ctypes.c_int, ctypes.c_int]),
+            "ref_sensitivity_map_f32": (ctypes.c_int, [G, fp]),
+            "ref_pixel_sensitivity_map_f32": (ctypes.c_int, [G, fp]),
+            "ref_gaussian_kernel_f32": (f, [f] * 8),
+            "ref_gaussian_kernel_f64": (d, [d] * 8),
+            "ref_analytic_kernel_f32": (f, [f] * 8),
+            "ref_analytic_kernel_f64": (d, [d] * 8),
+            "ref_event_params_f32": (None, [G, f, f, f, fp]),
+            "ref_pixel_at_f32": (None, [G, f, f, ip, ip]),
+            "ref_sensitivity_f32": (f, [G, f, f]),
+            "ref_load_txt_events": (i64, [ctypes.c_char_p, ctypes.c_int, fp, i64]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _ref = L
+    return _ref
+
+
+def _p(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def mlem(g: OracleGeometry, events, rho0=None, iterations=1, mode=MODE_CPU,
+         n_threads=0, dtype=np.float32):
+    """Runs the oracle MLEM; returns the image (n_y, n_z)."""
+    ev = np.ascontiguousarray(events, dtype=dtype)
+    shape = (g.n_y_pixels, g.n_z_pixels)
+    rho = (np.ones(shape, dtype) if rho0 is None
+           else np.ascontiguousarray(rho0, dtype=dtype).reshape(shape).copy())
+    m = OracleMode(*mode)
+    if dtype == np.float32:
+        rc = lib().oracle_mlem_f32(ctypes.byref(g), ctypes.byref(m),
+                                   _p(ev, ctypes.c_float), ev.shape[0],
+                                   _p(rho, ctypes.c_float), iterations,
+                                   n_threads)
+    else:
+        rc = lib().oracle_mlem_f64(ctypes.byref(g), ctypes.byref(m),
+                                   _p(ev, ctypes.c_double), ev.shape[0],
+                                   _p(rho, ctypes.c_double), iterations,
+                                   n_threads)
+    if rc != 0:
+        raise RuntimeError("oracle_mlem failed")
+    return rho
+
+
+def forward(g, events, rho, mode=MODE_CPU, n_threads=0, dtype=np.float32):
+    ev = np.ascontiguousarray(events, dtype=dtype)
+    r = np.ascontiguousarray(rho, dtype=dtype)
+    out = np.empty(ev.shape[0], dtype=dtype)
+    m = OracleMode(*mode)
+    t = ctypes.c_float if dtype == np.float32 else ctypes.c_double
+    fn = lib().oracle_forward_f32 if dtype == np.float32 else lib().oracle_forward_f64
+    fn(ctypes.byref(g), ctypes.byref(m), _p(ev, t), ev.shape[0], _p(r, t),
+       _p(out, t), n_threads)
+    return out
+
+
+def support_counts(g, events, n_threads=0):
+    ev = np.ascontiguousarray(events, dtype=np.float32)
+    nb = np.empty(ev.shape[0], dtype=np.int64)
+    ne = np.empty(ev.shape[0], dtype=np.int64)
+    lib().oracle_support_counts_f32(ctypes.byref(g), _p(ev, ctypes.c_float),
+                                    ev.shape[0], _p(nb, ctypes.c_int64),
+                                    _p(ne, ctypes.c_int64), n_threads)
+    return nb, ne
+
+
+def sensitivity_map(g, kind=0, dtype=np.float32):
+    out = np.empty((g.n_y_pixels, g.n_z_pixels), dtype=dtype)
+    if dtype == np.float32:
+        lib().oracle_sensitivity_map_f32(ctypes.byref(g), kind,
+                                         _p(out, ctypes.c_float))
+    else:
+        lib().oracle_sensitivity_map_f64(ctypes.byref(g), kind,
+                                         _p(out, ctypes.c_double))
+    return out
+
+
+def event_params(g, z_u, z_d, dl, dtype=np.float32):
+    t = ctypes.c_float if dtype == np.float32 else ctypes.c_double
+    out = (t * 9)()
+    box = (ctypes.c_int * 4)()
+    fn = lib().oracle_event_params_f32 if dtype == np.float32 else \
+        lib().oracle_event_params_f64
+    fn(ctypes.byref(g), z_u, z_d, dl, out, box)
+    return np.array(list(out), dtype=dtype), tuple(box)
+
+
+def ref_mlem(g, events, rho0=None, iterations=1, n_threads=0,
+             dtype=np.float32):
+    ev = np.ascontiguousarray(events, dtype=dtype)
+    shape = (g.n_y_pixels, g.n_z_pixels)
+    rho = (np.ones(shape, dtype) if rho0 is None
+           else np.ascontiguousarray(rho0, dtype=dtype).reshape(shape).copy())
+    if dtype == np.float32:
+        ref().ref_mlem_f32(ctypes.byref(g), _p(ev, ctypes.c_float),
+                           ev.shape[0], _p(rho, ctypes.c_float), iterations,
+                           n_threads)
+    else:
+        ref().ref_mlem_f64(ctypes.byref(g), _p(ev, ctypes.c_double),
+                           ev.shape[0], _p(rho, ctypes.c_double), iterations,
+                           n_threads)
+    return rho
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))

@@ -1,0 +1,166 @@
+// Harness around the REFERENCE's own CPU strip reconstruction.
+// TEST INFRASTRUCTURE ONLY.  Compiled by oracle/Makefile (target `ref`)
+// directly from the reference's headers where they lie under
+// /root/reference/src; the output goes to oracle/_ref/ (git-ignored).  No
+// reference source is copied into this repository: this file only
+// instantiates the reference templates and exposes them through extern "C"
+// entry points so tests can compare the oracle restatement with them.
+//
+// Instantiated reference code:
+//   PET2D::Strip::Reconstruction<F, GaussianKernel<F>>   reconstruction.h:32-344
+//   PET2D::Strip::Scanner<F, short>                      scanner.h:26-291
+//   PET2D::Strip::GaussianKernel<F>, AnalyticKernel<F>   gaussian_kernel.h, analytic_kernel.h
+//   util::text_parser (fast .txt event loading)          util/text_parser.h:29-175
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "2d/strip/reconstruction.h"
+#include "2d/strip/gaussian_kernel.h"
+#include "2d/strip/analytic_kernel.h"
+#include "2d/strip/scanner.h"
+
+#include "strip_oracle.h"  // oracle_geometry only (plain struct)
+
+namespace {
+
+struct NoProgress {
+  void operator()(size_t, bool = false) {}
+};
+
+template <typename F>
+PET2D::Strip::Scanner<F, short> make_scanner(const oracle_geometry* g) {
+  // Same argument order as PET2D_STRIP_SCANNER_CL (options.h:42-50)
+  return PET2D::Strip::Scanner<F, short>(
+      (F)g->radius, (F)g->scintillator_length, (short)g->n_y_pixels,
+      (short)g->n_z_pixels, (F)g->pixel_height, (F)g->pixel_width,
+      (F)g->sigma_z, (F)g->sigma_dl, (F)g->center_y, (F)g->center_z);
+}
+
+template <typename F, typename E>
+int run_mlem(const oracle_geometry* g, const E* events, int64_t n, F* rho,
+             int n_iterations, int n_threads) {
+#if _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+  using Kernel = PET2D::Strip::GaussianKernel<F>;
+  PET2D::Strip::Reconstruction<F, Kernel> rec(make_scanner<F>(g));
+  rec.responses.reserve(n);
+  for (int64_t e = 0; e < n; ++e)
+    rec.responses.emplace_back((F)events[3 * e], (F)events[3 * e + 1],
+                               (F)events[3 * e + 2]);
+  const int64_t np = (int64_t)g->n_z_pixels * g->n_y_pixels;
+  for (int64_t i = 0; i < np; ++i) rec.rho[i] = rho[i];
+  NoProgress progress;
+  rec(progress, n_iterations, 0);
+  for (int64_t i = 0; i < np; ++i) rho[i] = rec.rho[i];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ref_mlem_f32(const oracle_geometry* g, const float* events, int64_t n,
+                 float* rho, int n_iterations, int n_threads) {
+  return run_mlem<float>(g, events, n, rho, n_iterations, n_threads);
+}
+
+int ref_mlem_f64(const oracle_geometry* g, const double* events, int64_t n,
+                 double* rho, int n_iterations, int n_threads) {
+  return run_mlem<double>(g, events, n, rho, n_iterations, n_threads);
+}
+
+// Reconstruction ctor's sensitivity map (reconstruction.h:67-73)
+int ref_sensitivity_map_f32(const oracle_geometry* g, float* out) {
+  using Kernel = PET2D::Strip::GaussianKernel<float>;
+  PET2D::Strip::Reconstruction<float, Kernel> rec(make_scanner<float>(g));
+  const int64_t np = (int64_t)g->n_z_pixels * g->n_y_pixels;
+  for (int64_t i = 0; i < np; ++i) out[i] = rec.sensitivity[i];
+  return 0;
+}
+
+// Legacy --gpu sensitivity (cuda/reconstruction.cu:115-126 via
+// Scanner::pixel_sensitivity, scanner.h:174-185), evaluated on the host.
+int ref_pixel_sensitivity_map_f32(const oracle_geometry* g, float* out) {
+  auto s = make_scanner<float>(g);
+  for (int y = 0; y < g->n_y_pixels; ++y)
+    for (int x = 0; x < g->n_z_pixels; ++x)
+      out[(int64_t)y * g->n_z_pixels + x] =
+          s.pixel_sensitivity(PET2D::Pixel<short>(x, y));
+  return 0;
+}
+
+float ref_gaussian_kernel_f32(float sz, float sdl, float y, float tan,
+                              float sec, float R, float dz, float dy) {
+  PET2D::Strip::GaussianKernel<float> k(sz, sdl);
+  return k(y, tan, sec, R, PET2D::Vector<float>(dz, dy));
+}
+
+double ref_gaussian_kernel_f64(double sz, double sdl, double y, double tan,
+                               double sec, double R, double dz, double dy) {
+  PET2D::Strip::GaussianKernel<double> k(sz, sdl);
+  return k(y, tan, sec, R, PET2D::Vector<double>(dz, dy));
+}
+
+float ref_analytic_kernel_f32(float sz, float sdl, float y, float tan,
+                              float sec, float R, float dz, float dy) {
+  PET2D::Strip::AnalyticKernel<float> k(sz, sdl);
+  return k(y, tan, sec, R, PET2D::Vector<float>(dz, dy));
+}
+
+double ref_analytic_kernel_f64(double sz, double sdl, double y, double tan,
+                               double sec, double R, double dz, double dy) {
+  PET2D::Strip::AnalyticKernel<double> k(sz, sdl);
+  return k(y, tan, sec, R, PET2D::Vector<double>(dz, dy));
+}
+
+// Response::calculate_tan_y_z + GaussianKernel::ellipse_bb (float)
+void ref_event_params_f32(const oracle_geometry* g, float z_u, float z_d,
+                          float dl, float* out) {
+  auto s = make_scanner<float>(g);
+  PET2D::Strip::Response<float> r(z_u, z_d, dl);
+  float tan, y, z;
+  r.calculate_tan_y_z(s.radius, tan, y, z);
+  PET2D::Strip::GaussianKernel<float> k(s.sigma_z, s.sigma_dl);
+  float sec, A, B, C, bb_y, bb_z;
+  k.ellipse_bb(tan, sec, A, B, C, bb_y, bb_z);
+  float v[9] = { tan, y, z, sec, A, B, C, bb_y, bb_z };
+  std::memcpy(out, v, sizeof(v));
+}
+
+void ref_pixel_at_f32(const oracle_geometry* g, float z, float y, int* px,
+                      int* py) {
+  auto s = make_scanner<float>(g);
+  auto p = s.pixel_at(PET2D::Point<float>(z, y));
+  *px = p.x;
+  *py = p.y;
+}
+
+float ref_sensitivity_f32(const oracle_geometry* g, float z, float y) {
+  auto s = make_scanner<float>(g);
+  return s.sensitivity(PET2D::Point<float>(z, y));
+}
+
+// Fast text event loader (reconstruction.h:170-195, text_parser.h:140-175):
+// returns the number of events; writes up to `cap` triples.
+int64_t ref_load_txt_events(const char* fn, int is_3d, float* out,
+                            int64_t cap) {
+  oracle_geometry g = { 1, 1, 1, 1, 1, 1, 1, 1, 0, 0 };
+  using Kernel = PET2D::Strip::GaussianKernel<float>;
+  PET2D::Strip::Reconstruction<float, Kernel> rec(make_scanner<float>(&g));
+  try {
+    rec.fast_load_txt_events(fn, is_3d != 0);
+  } catch (...) {
+    return -1;
+  }
+  int64_t n = (int64_t)rec.responses.size();
+  for (int64_t e = 0; e < n && e < cap; ++e) {
+    out[3 * e] = rec.responses[e].z_u;
+    out[3 * e + 1] = rec.responses[e].z_d;
+    out[3 * e + 2] = rec.responses[e].dl;
+  }
+  return n;
+}
+
+}  // extern "C"

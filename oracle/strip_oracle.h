@@ -1,0 +1,130 @@
+/*
+ * CPU oracle for the J-PET 2D strip list-mode TOF MLEM path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is called by
+ * tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg, and
+ * never by the product path.  It restates, in plain C, the reference's CPU
+ * (OpenMP) reconstruction and the helpers it uses; every function cites the
+ * reference file:line it follows (paths relative to the reference root).
+ *
+ * Each function exists in an FP32 (`_f32`) and an FP64 (`_f64`) flavour.  The
+ * FP32 flavour reproduces the reference's float arithmetic, including its
+ * double-precision constant folding where the reference mixes `M_PI` (double)
+ * into float expressions.
+ */
+#ifndef STRIP_ORACLE_H
+#define STRIP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scanner parameters as the CLI hands them to `Scanner(...)`
+ * (src/2d/strip/options.h:42-50, scanner.h:39-67).  Values are converted to
+ * the flavour's float type exactly as the reference's constructor arguments
+ * are. */
+typedef struct {
+  double radius;              /* R: half distance between strips */
+  double scintillator_length; /* L */
+  int n_z_pixels;
+  int n_y_pixels;
+  double pixel_width;  /* along z */
+  double pixel_height; /* along y */
+  double sigma_z;
+  double sigma_dl;
+  double center_y;
+  double center_z;
+} oracle_geometry;
+
+/* Reconstruction semantics.  {0,0,0,1} is the reference CPU path
+ * (reconstruction.h:220-298); {1,1,1,0} is the legacy CUDA warp path
+ * (cuda/reconstruction/warp_granularity.cuh:42-155). */
+typedef struct {
+  int kernel;          /* 0 = GaussianKernel, 1 = AnalyticKernel */
+  int sensitivity;     /* 0 = none, 1 = in denominator, 2 = divides numerator */
+  int sensitivity_map; /* 0 = sensitivity(centre), 1 = 5-point pixel_sensitivity */
+  int guard_zero;      /* 1 = inv = D>0 ? 1/D : 0 ; 0 = inv = 1/D */
+} oracle_mode;
+
+/* ---- per-event geometry (response.h:22-37, gaussian_kernel.h:108-128) ---- */
+void oracle_event_params_f32(const oracle_geometry* g, float z_u, float z_d,
+                             float dl, float* out /* tan,y,z,sec,A,B,C,bb_y,bb_z */,
+                             int* bbox /* x0,y0,x1,y1 (clamped, x1/y1 exclusive) */);
+void oracle_event_params_f64(const oracle_geometry* g, double z_u, double z_d,
+                             double dl, double* out, int* bbox);
+
+/* ---- kernels (gaussian_kernel.h:51-79, analytic_kernel.h:28-62) ---- */
+float oracle_gaussian_kernel_f32(float sigma_z, float sigma_dl, float y,
+                                 float tan, float sec, float R, float dz,
+                                 float dy);
+double oracle_gaussian_kernel_f64(double sigma_z, double sigma_dl, double y,
+                                  double tan, double sec, double R, double dz,
+                                  double dy);
+float oracle_analytic_kernel_f32(float sigma_z, float sigma_dl, float y,
+                                 float tan, float sec, float R, float dz,
+                                 float dy);
+double oracle_analytic_kernel_f64(double sigma_z, double sigma_dl, double y,
+                                  double tan, double sec, double R, double dz,
+                                  double dy);
+/* bb_y/bb_z helpers (gaussian_kernel.h:159-160) */
+double oracle_bb_y_f64(double A, double C, double B_2);
+double oracle_bb_z_f64(double A, double C, double B_2);
+
+/* ---- scanner (scanner.h:136-185) ---- */
+void oracle_pixel_at_f32(const oracle_geometry* g, float z, float y, int* px,
+                         int* py);
+void oracle_pixel_at_f64(const oracle_geometry* g, double z, double y, int* px,
+                         int* py);
+void oracle_pixel_center_f32(const oracle_geometry* g, int px, int py,
+                             float* z, float* y);
+void oracle_pixel_center_f64(const oracle_geometry* g, int px, int py,
+                             double* z, double* y);
+float oracle_sensitivity_f32(const oracle_geometry* g, float z, float y);
+double oracle_sensitivity_f64(const oracle_geometry* g, double z, double y);
+/* whole map, row-major y*n_z+z; kind 0 = centre, 1 = 5-point */
+void oracle_sensitivity_map_f32(const oracle_geometry* g, int kind, float* out);
+void oracle_sensitivity_map_f64(const oracle_geometry* g, int kind,
+                                double* out);
+
+/* ---- projection-space conversions (scanner.h:107-127, event.h:25-42) ---- */
+void oracle_to_projection_f64(double R, double y, double z, double tan,
+                              double* z_u, double* z_d, double* dl);
+
+/* ---- MLEM (reconstruction.h:110-154 + 220-298) ----
+ * events: AoS (z_u, z_d, dl) triples.  rho: n_y*n_z row-major, updated in
+ * place after n_iterations.  n_threads <= 0 uses OpenMP's default; 1 is a
+ * deterministic serial run.  Returns 0 on success. */
+int oracle_mlem_f32(const oracle_geometry* g, const oracle_mode* m,
+                    const float* events, int64_t n_events, float* rho,
+                    int n_iterations, int n_threads);
+int oracle_mlem_f64(const oracle_geometry* g, const oracle_mode* m,
+                    const double* events, int64_t n_events, double* rho,
+                    int n_iterations, int n_threads);
+
+/* Forward projection only: D_e = sum_p K_e(p) rho(p) [* s(p)] per event. */
+int oracle_forward_f32(const oracle_geometry* g, const oracle_mode* m,
+                       const float* events, int64_t n_events, const float* rho,
+                       float* denominators, int n_threads);
+int oracle_forward_f64(const oracle_geometry* g, const oracle_mode* m,
+                       const double* events, int64_t n_events,
+                       const double* rho, double* denominators, int n_threads);
+
+/* Per-event support statistics in FP32 semantics: bbox pixel count B_e and
+ * in-ellipse pixel count P_e (the kernel-evaluation count). */
+int oracle_support_counts_f32(const oracle_geometry* g, const float* events,
+                              int64_t n_events, int64_t* bbox_pixels,
+                              int64_t* ellipse_pixels, int n_threads);
+
+/* Poisson list-mode log-likelihood  sum_e log(D_e) - sum_p s(p) rho(p)
+ * in FP64 (the adjoint's objective). */
+double oracle_loglik_f64(const oracle_geometry* g, const oracle_mode* m,
+                         const double* events, int64_t n_events,
+                         const double* rho, const double* sens_weights,
+                         int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
