@@ -135,6 +135,30 @@ int strip_mlem_set_launch(strip_mlem_ctx* ctx, int n_blocks,
 int strip_mlem_set_algorithm(strip_mlem_ctx* ctx, int algorithm);
 int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx);
 
+/* Tile shapes of the row-band algorithm (the knobs an autotuner turns in
+ * place of -B/-W): forward_rows / backward_rows = image rows per band of the
+ * forward-projection and back-projection kernels (32 or 64),
+ * bucket_columns = x0 bucket width bounding a task's column window (16, 32
+ * or 64), forward_pairs / backward_pairs = (event, band) pairs per forward
+ * workgroup / per back-projection wavefront.  Takes effect at the next
+ * iteration. */
+int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
+                               int backward_rows, int bucket_columns,
+                               int forward_pairs, int backward_pairs);
+/* How the row-band kernels evaluate exp(-Q/2) along an image row: 0 = one
+ * hardware exp2 per pixel; 1 = the recurrence e[k+1] = e[k] r[k],
+ * r[k+1] = r[k] G (exact for exp2 of a quadratic in the column), restarted
+ * from exp2 every 8 (forward) / 4 (back-projection) pixels.  Both stay within
+ * the FP32 parity tolerance of the oracle. */
+int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode);
+
+/* The plan in use: band rows of both kernels, task window width (columns)
+ * and the number of (event, band) pairs of both kernels. */
+int strip_mlem_get_band_tiling(strip_mlem_ctx* ctx, int* forward_rows,
+                               int* backward_rows, int* window_columns,
+                               int64_t* forward_pairs,
+                               int64_t* backward_pairs);
+
 /* Exact number of kernel evaluations of one iteration, sum_e P_e, where P_e
  * counts the pixels of event e's clamped bounding box that pass the 3-sigma
  * ellipse test (gaussian_kernel.h:131-137).  Computed on the device. */

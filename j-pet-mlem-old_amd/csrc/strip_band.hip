@@ -14,10 +14,7 @@
 namespace strip {
 namespace {
 
-constexpr int kFwdThreads = 1024;      // one workgroup holds a rho band
-constexpr int kBwdWaves = 4;           // wavefronts per backward workgroup
-constexpr int kFwdPairsPerTask = 8192; // pairs per forward workgroup
-constexpr int kBwdPairsPerTask = 1024; // pairs per backward wavefront sweep
+constexpr int kFwdWaves = 8;  // wavefronts per forward workgroup
 
 #define BAND_TRY(expr)                                              \
   do {                                                              \
@@ -58,10 +55,11 @@ struct RowSpan {
   float pre;   // c / sqrt(a . S^-1 . a) of this row
 };
 
-__device__ inline RowSpan row_span(const DeviceGeometry& g, float inv_w,
-                                   float inv_2c, float cz, float cy, float A,
-                                   float B, float C, float P, float Q0,
-                                   int x0, int x1, int y0, int y1, int iy) {
+__device__ inline RowSpan row_span(const DeviceGeometry& g, float cz,
+                                   float cy, float A, float B, float P,
+                                   float Q0, int x0, int x1, int y0, int y1,
+                                   int iy) {
+  const float C = g.C, inv_2c = g.inv_2c, inv_w = g.inv_pixel_width;
   RowSpan s;
   s.lo = 0;
   s.n = 0;
@@ -107,7 +105,7 @@ __device__ inline RowSpan row_span(const DeviceGeometry& g, float inv_w,
 }
 
 struct EventView {
-  float cz, cy, A, B, C, P, Q0;
+  float cz, cy, A, B, P, Q0;
   int x0, x1, y0, y1;
 };
 
@@ -121,7 +119,6 @@ __device__ inline EventView load_event(const EventSoA& ev, int e) {
   v.cy = a.y;
   v.A = b.x;
   v.B = b.y;
-  v.C = b.z;
   v.P = c.x;
   v.Q0 = c.y;
   v.x0 = box.x & 0xffff;
@@ -173,12 +170,14 @@ __global__ __launch_bounds__(256) void box_width_kernel(
   width[e] = max((int)(b.x >> 16) - (int)(b.x & 0xffff), 0);
 }
 
-__global__ __launch_bounds__(256) void band_bounds_kernel(
-    const uint32_t* __restrict__ keys, int64_t n, int* __restrict__ first) {
+// first[k] = index of the first sorted pair whose (band, x0 bucket) key is k
+__global__ __launch_bounds__(256) void group_bounds_kernel(
+    const uint32_t* __restrict__ keys, int64_t n, int shift,
+    int* __restrict__ first) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int b = keys[i] >> 16;
-  if (i == 0 || (int)(keys[i - 1] >> 16) != b) first[b] = (int)i;
+  const uint32_t k = keys[i] >> shift;
+  if (i == 0 || (keys[i - 1] >> shift) != k) first[k] = (int)i;
 }
 
 // ---------------------------------------------------------------------------
@@ -243,68 +242,6 @@ __global__ __launch_bounds__(256) void band_finalize_kernel(
 // iteration kernels
 // ---------------------------------------------------------------------------
 
-// Forward projection: workgroup = one rho band (column-major in LDS), waves
-// take the task's pairs round-robin; lane = q*H + r handles row r of the band
-// and every (64/H)-th column of the row's span starting at offset q.
-// MODE 0: partial D per pair; MODE 1: count the span pixels instead.
-template <int H, int MODE>
-__global__ __launch_bounds__(kFwdThreads) void band_forward_kernel(
-    DeviceGeometry g, EventSoA ev, const int* __restrict__ pair_event,
-    const int4* __restrict__ tasks, const float* __restrict__ rhoT,
-    float* __restrict__ partial, int max_bands,
-    unsigned long long* __restrict__ count) {
-  constexpr int LR = 64 / H;
-  extern __shared__ float band[];  // n_z * H, word = col * H + row
-  const int4 task = tasks[blockIdx.x];
-  const int b = task.x;
-  const int y_base = b * H;
-  if (MODE == 0) {
-    const int n_words = g.n_z * H;
-    for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
-      const int col = i / H, r = i - col * H;
-      const int iy = y_base + r;
-      band[i] = (iy < g.n_y) ? rhoT[(int64_t)col * g.n_y + iy] : 0.0f;
-    }
-    __syncthreads();
-  }
-  const int lane = threadIdx.x & 63;
-  const int r = lane & (H - 1);
-  const int q = lane / H;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n_waves = blockDim.x >> 6;
-  const float inv_w = 1.0f / g.pixel_width;
-  const float step = (float)LR * g.pixel_width;
-  unsigned long long counted = 0;
-  for (int p = task.y + wave; p < task.z; p += n_waves) {
-    const int e = __builtin_amdgcn_readfirstlane(pair_event[p]);
-    const EventView v = load_event(ev, e);
-    const float inv_2c = 0.5f / v.C;
-    const RowSpan s = row_span(g, inv_w, inv_2c, v.cz, v.cy, v.A, v.B, v.C,
-                               v.P, v.Q0, v.x0, v.x1, v.y0, v.y1, y_base + r);
-    const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
-    if (MODE == 1) {
-      counted += (unsigned long long)wave_sum_int(q == 0 ? s.n : 0);
-      continue;
-    }
-    const int col0 = s.lo + q;
-    float dz = __builtin_fmaf((float)col0, g.pixel_width,
-                              g.tl_z_half_w - v.cz);
-    const float qa = s.rowA * kExp2Scale;
-    const float qb = s.bdy * kExp2Scale;
-    const float qc = v.C * kExp2Scale;
-    const float* src = band + col0 * H + r;
-    float acc = 0.0f;
-    for (int k = 0; k < nq; ++k) {
-      const float qq = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
-      acc = __builtin_fmaf(__builtin_amdgcn_exp2f(qq), src[k * LR * H], acc);
-      dz += step;
-    }
-    const float d = wave_sum(acc * s.pre);
-    if (lane == 0) partial[(int64_t)e * max_bands + (b - v.y0 / H)] = d;
-  }
-  if (MODE == 1 && lane == 0 && counted) atomicAdd(count, counted);
-}
-
 // D_e = sum of the event's band partials; inv_e = 1/D_e (guarded or not).
 __global__ __launch_bounds__(256) void band_inverse_kernel(
     const uint2* __restrict__ box, int64_t n, int H, int max_bands,
@@ -324,92 +261,295 @@ __global__ __launch_bounds__(256) void band_inverse_kernel(
   if (inv) inv[e] = guard ? ((d > 0.0f) ? 1.0f / d : 0.0f) : 1.0f / d;
 }
 
-// Adds window columns [c0, c1) to the column-major back-projection image and
-// clears them.  lane = q*H + r handles row r of columns c0+q, c0+q+LR, ...
-template <int H>
-__device__ inline void flush_columns(float* win, int W, int c0, int c1,
-                                     int y_base, const DeviceGeometry& g,
-                                     float* __restrict__ bpT, int lane) {
-  constexpr int LR = 64 / H;
-  const int r = lane & (H - 1), q = lane / H;
-  const int iy = y_base + r;
-  for (int c = c0 + q; c < c1; c += LR) {
-    float* w = win + (c & (W - 1)) * H + r;
-    const float v = *w;
-    *w = 0.0f;
-    if (v != 0.0f && iy < g.n_y && c < g.n_z)
-      unsafeAtomicAdd(&bpT[(int64_t)c * g.n_y + iy], v);
-  }
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                         0xF, 0xF, false));
 }
 
-// Back-projection: each wavefront sweeps one task (pairs of one band sorted by
-// x0) through its private LDS window of W columns; plain read-modify-write.
-template <int H>
-__global__ __launch_bounds__(64 * kBwdWaves) void band_backward_kernel(
+__device__ inline float lane_value(float v, int lane) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// Sum over the wavefront with DPP inside each 16-lane row (quad swaps, then
+// row rotations by 4 and 8), then the four row totals read as scalars.
+// Returns the total (wave-uniform).
+__device__ inline float wave_total(float v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x128>(v);  // row_ror:8
+  return (lane_value(v, 0) + lane_value(v, 16)) +
+         (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// Records of up to 64 consecutive pairs, one pair per lane, gathered with a
+// handful of vector loads so the random event reads of a whole batch are in
+// flight together; each pair's fields are then handed to the wave with
+// v_readlane (wave-uniform scalars).
+struct PairBatch {
+  int e;
+  float cz, cy, A, B, P, Q0, inv;
+  uint32_t bx, by;
+};
+
+template <bool WITH_INV>
+__device__ inline PairBatch gather_batch(const EventSoA& ev,
+                                         const int* __restrict__ pair_event,
+                                         const float* __restrict__ inv,
+                                         int first, int end, int lane) {
+  PairBatch pb;
+  const int p = first + lane;
+  pb.e = (p < end) ? pair_event[p] : 0;
+  const float4 a = ev.a[pb.e];
+  const float4 b = ev.b[pb.e];
+  const float4 c = ev.c[pb.e];
+  const uint2 box = ev.box[pb.e];
+  pb.cz = a.x;
+  pb.cy = a.y;
+  pb.A = b.x;
+  pb.B = b.y;
+  pb.P = c.x;
+  pb.Q0 = c.y;
+  pb.bx = box.x;
+  pb.by = box.y;
+  pb.inv = WITH_INV ? inv[pb.e] : 0.0f;
+  return pb;
+}
+
+__device__ inline int lane_int(int v, int j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+__device__ inline float lane_float(float v, int j) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+
+struct PairView {
+  int e;
+  float cz, cy, A, B, P, Q0, inv;
+  int x0, x1, y0, y1;
+};
+
+__device__ inline PairView pick(const PairBatch& pb, int j) {
+  PairView v;
+  v.e = lane_int(pb.e, j);
+  v.cz = lane_float(pb.cz, j);
+  v.cy = lane_float(pb.cy, j);
+  v.A = lane_float(pb.A, j);
+  v.B = lane_float(pb.B, j);
+  v.P = lane_float(pb.P, j);
+  v.Q0 = lane_float(pb.Q0, j);
+  v.inv = lane_float(pb.inv, j);
+  const uint32_t bx = (uint32_t)lane_int((int)pb.bx, j);
+  const uint32_t by = (uint32_t)lane_int((int)pb.by, j);
+  v.x0 = bx & 0xffff;
+  v.x1 = bx >> 16;
+  v.y0 = by & 0xffff;
+  v.y1 = by >> 16;
+  return v;
+}
+
+// Forward projection.  Workgroup = one task (pairs of one band and one x0
+// bucket); its rho window of Wt columns from c0 lives in LDS column-major
+// (word = (col - c0) * H + row; lanes of a column touch distinct rows, hence
+// distinct banks).  Wave w takes a contiguous share of the task's pairs;
+// lane = q*H + r covers row r of the band and every (64/H)-th column of the
+// row's span from offset q.
+// MODE 0: partial D per pair; MODE 1: count the span pixels instead.
+template <int H, int MODE, bool REC>
+__global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
+    DeviceGeometry g, EventSoA ev, const int* __restrict__ pair_event,
+    const int4* __restrict__ tasks, const float* __restrict__ rhoT, int Wt,
+    float* __restrict__ partial, int max_bands,
+    unsigned long long* __restrict__ count) {
+  extern __shared__ float win[];
+  const int4 task = tasks[blockIdx.x];
+  const int b = task.x;
+  const int c0 = task.w;
+  const int y_base = b * H;
+  if (MODE == 0) {
+    const int n_words = Wt * H;
+    for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
+      const int col = c0 + i / H, iy = y_base + (i & (H - 1));
+      win[i] = (col < g.n_z && iy < g.n_y)
+                   ? rhoT[(int64_t)col * g.n_y + iy]
+                   : 0.0f;
+    }
+    __syncthreads();
+  }
+  constexpr int LR = 64 / H;  // lanes per image row
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (H - 1);
+  const int q = lane / H;
+  const float step = (float)LR * g.pixel_width;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int total = task.z - task.y;
+  const int share = (total + kFwdWaves - 1) / kFwdWaves;
+  const int begin = task.y + wave * share;
+  const int end = min(task.z, begin + share);
+  const float qc = g.C * kExp2Scale;
+  const float two_qc_step = 2.0f * qc * step;
+  const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
+  unsigned long long counted = 0;
+  PairBatch next = gather_batch<false>(ev, pair_event, nullptr, begin, end,
+                                       lane);
+  for (int base = begin; base < end; base += 64) {
+    const PairBatch cur = next;
+    if (base + 64 < end)
+      next = gather_batch<false>(ev, pair_event, nullptr, base + 64, end, lane);
+    const int n_here = min(64, end - base);
+    for (int j = 0; j < n_here; ++j) {
+      const PairView v = pick(cur, j);
+      const RowSpan s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0,
+                                 v.x1, v.y0, v.y1, y_base + r);
+      const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
+      if (MODE == 1) {
+        counted += (unsigned long long)wave_sum_int(q == 0 ? s.n : 0);
+        continue;
+      }
+      const int col0 = s.lo + q;
+      float dz = __builtin_fmaf((float)col0, g.pixel_width,
+                                g.tl_z_half_w - v.cz);
+      const float qa = s.rowA * kExp2Scale;
+      const float qb = s.bdy * kExp2Scale;
+      const float* src = win + (col0 - c0) * H + r;
+      float acc = 0.0f;
+      int k = 0;
+      if (REC) {
+        // exp2 of a quadratic along the row: e[k+1] = e[k] r[k],
+        // r[k+1] = r[k] G, restarted from exp2 every 8 columns
+        const float db = __builtin_fmaf(qc * step, step, qb * step);
+        for (; k + 8 <= nq; k += 8) {
+          float e = __builtin_amdgcn_exp2f(
+              __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
+          float rr = __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db));
+#pragma unroll
+          for (int kk = 0; kk < 8; ++kk) {
+            acc = __builtin_fmaf(e, src[(k + kk) * LR * H], acc);
+            e *= rr;
+            rr *= G;
+          }
+          dz += 8.0f * step;
+        }
+      }
+      for (; k < nq; ++k) {
+        const float qq = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
+        acc = __builtin_fmaf(__builtin_amdgcn_exp2f(qq), src[k * LR * H],
+                             acc);
+        dz += step;
+      }
+      const float d = wave_total(acc * s.pre);
+      if (lane == 0) partial[(int64_t)v.e * max_bands + (b - v.y0 / H)] = d;
+    }
+  }
+  if (MODE == 1 && lane == 0 && counted) atomicAdd(count, counted);
+}
+
+// Back-projection.  Each wavefront takes one task and owns a private window
+// of Wt columns x H rows in LDS (word = (col - c0) * H + row): every lane
+// writes only its own (row, column) words, and no other wave sees the
+// window, so plain read-modify-write suffices.  Four columns are read before
+// any is written (distinct words).  The window is then added to the
+// column-major image.
+template <int H, bool REC>
+__global__ __launch_bounds__(256) void band_backward_kernel(
     DeviceGeometry g, EventSoA ev, const int* __restrict__ pair_event,
     const int4* __restrict__ tasks, int n_tasks, const float* __restrict__ inv,
-    int W, float* __restrict__ bpT) {
-  constexpr int LR = 64 / H;
+    int Wt, float* __restrict__ bpT) {
   extern __shared__ float windows[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int t = blockIdx.x * kBwdWaves + wave;
+  const int n_waves = blockDim.x >> 6;
+  const int t = blockIdx.x * n_waves + wave;
   if (t >= n_tasks) return;
-  float* win = windows + wave * W * H;
-  for (int i = lane; i < W * H; i += 64) win[i] = 0.0f;
+  float* win = windows + wave * Wt * H;
+  for (int i = lane; i < Wt * H; i += 64) win[i] = 0.0f;
   const int4 task = tasks[t];
   const int b = task.x;
+  const int c0 = task.w;
   const int y_base = b * H;
+  constexpr int LR = 64 / H;  // lanes per image row
+  constexpr int S = LR * H;   // word stride between a lane's columns
   const int r = lane & (H - 1);
   const int q = lane / H;
-  const float inv_w = 1.0f / g.pixel_width;
   const float step = (float)LR * g.pixel_width;
-  int F = -1;  // window holds columns [F, F + W)
-  for (int p = task.y; p < task.z; ++p) {
-    const int e = __builtin_amdgcn_readfirstlane(pair_event[p]);
-    const EventView v = load_event(ev, e);
-    if (F < 0) F = v.x0;
-    if (v.x1 > F + W) {
-      const int nf = v.x1 - W;
-      flush_columns<H>(win, W, F, nf, y_base, g, bpT, lane);
-      F = nf;
-    }
-    const float s_inv = inv[e];
-    if (s_inv == 0.0f) continue;
-    const float inv_2c = 0.5f / v.C;
-    const RowSpan s = row_span(g, inv_w, inv_2c, v.cz, v.cy, v.A, v.B, v.C,
-                               v.P, v.Q0, v.x0, v.x1, v.y0, v.y1, y_base + r);
-    const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
-    const int col0 = s.lo + q;
-    float dz = __builtin_fmaf((float)col0, g.pixel_width,
-                              g.tl_z_half_w - v.cz);
-    const float qa = s.rowA * kExp2Scale;
-    const float qb = s.bdy * kExp2Scale;
-    const float qc = v.C * kExp2Scale;
-    const float scale = s.pre * s_inv;
-    int col = col0;
-    int k = 0;
-    // two columns per step: both reads are issued before the writes (the two
-    // words differ, and no other lane or wave touches them)
-    for (; k + 1 < nq; k += 2) {
-      const float q0 = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
-      const float dz1 = dz + step;
-      const float q1 = __builtin_fmaf(__builtin_fmaf(qc, dz1, qb), dz1, qa);
-      float* w0 = win + (col & (W - 1)) * H + r;
-      float* w1 = win + ((col + LR) & (W - 1)) * H + r;
-      const float a0 = *w0, a1 = *w1;
-      *w0 = __builtin_fmaf(__builtin_amdgcn_exp2f(q0), scale, a0);
-      *w1 = __builtin_fmaf(__builtin_amdgcn_exp2f(q1), scale, a1);
-      dz += 2.0f * step;
-      col += 2 * LR;
-    }
-    if (k < nq) {
-      const float q0 = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
-      float* w0 = win + (col & (W - 1)) * H + r;
-      *w0 = __builtin_fmaf(__builtin_amdgcn_exp2f(q0), scale, *w0);
+  const float qc = g.C * kExp2Scale;
+  const float two_qc_step = 2.0f * qc * step;
+  const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
+  PairBatch next = gather_batch<true>(ev, pair_event, inv, task.y, task.z,
+                                      lane);
+  for (int base = task.y; base < task.z; base += 64) {
+    const PairBatch cur = next;
+    if (base + 64 < task.z)
+      next = gather_batch<true>(ev, pair_event, inv, base + 64, task.z, lane);
+    const int n_here = min(64, task.z - base);
+    for (int j = 0; j < n_here; ++j) {
+      const PairView v = pick(cur, j);
+      if (v.inv == 0.0f) continue;
+      const RowSpan s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0,
+                                 v.x1, v.y0, v.y1, y_base + r);
+      const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
+      const int col0 = s.lo + q;
+      float dz = __builtin_fmaf((float)col0, g.pixel_width,
+                                g.tl_z_half_w - v.cz);
+      const float qa = s.rowA * kExp2Scale;
+      const float qb = s.bdy * kExp2Scale;
+      const float scale = s.pre * v.inv;
+      float* w = win + (col0 - c0) * H + r;
+      int k = 0;
+      if (REC) {
+        // e[k+1] = e[k] r[k], r[k+1] = r[k] G, restarted every 4 columns
+        const float db = __builtin_fmaf(qc * step, step, qb * step);
+        for (; k + 4 <= nq; k += 4) {
+          const float a0 = w[0], a1 = w[S], a2 = w[2 * S], a3 = w[3 * S];
+          const float e0 = __builtin_amdgcn_exp2f(
+              __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
+          const float r0 =
+              __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db));
+          const float e1 = e0 * r0, r1 = r0 * G;
+          const float e2 = e1 * r1, r2 = r1 * G;
+          const float e3 = e2 * r2;
+          w[0] = __builtin_fmaf(e0, scale, a0);
+          w[S] = __builtin_fmaf(e1, scale, a1);
+          w[2 * S] = __builtin_fmaf(e2, scale, a2);
+          w[3 * S] = __builtin_fmaf(e3, scale, a3);
+          dz += 4.0f * step;
+          w += 4 * S;
+        }
+      }
+      for (; k + 4 <= nq; k += 4) {
+        const float a0 = w[0], a1 = w[S], a2 = w[2 * S], a3 = w[3 * S];
+        const float dz1 = dz + step, dz2 = dz1 + step, dz3 = dz2 + step;
+        const float q0 = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
+        const float q1 = __builtin_fmaf(__builtin_fmaf(qc, dz1, qb), dz1, qa);
+        const float q2 = __builtin_fmaf(__builtin_fmaf(qc, dz2, qb), dz2, qa);
+        const float q3 = __builtin_fmaf(__builtin_fmaf(qc, dz3, qb), dz3, qa);
+        w[0] = __builtin_fmaf(__builtin_amdgcn_exp2f(q0), scale, a0);
+        w[S] = __builtin_fmaf(__builtin_amdgcn_exp2f(q1), scale, a1);
+        w[2 * S] = __builtin_fmaf(__builtin_amdgcn_exp2f(q2), scale, a2);
+        w[3 * S] = __builtin_fmaf(__builtin_amdgcn_exp2f(q3), scale, a3);
+        dz = dz3 + step;
+        w += 4 * S;
+      }
+      for (; k < nq; ++k) {
+        const float qq = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
+        *w = __builtin_fmaf(__builtin_amdgcn_exp2f(qq), scale, *w);
+        dz += step;
+        w += S;
+      }
     }
   }
-  if (F >= 0) flush_columns<H>(win, W, F, F + W, y_base, g, bpT, lane);
+  // flush: lane = q*H + r adds row r of columns c0+q, c0+q+64/H, ...
+  const int iy = y_base + r;
+  for (int c = q; c < Wt; c += LR) {
+    const float val = win[c * H + r];
+    const int col = c0 + c;
+    if (val != 0.0f && iy < g.n_y && col < g.n_z)
+      unsafeAtomicAdd(&bpT[(int64_t)col * g.n_y + iy], val);
+  }
 }
 
 template <typename Kern>
@@ -424,20 +564,22 @@ template <int H>
 int launch_forward(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                    int mode, unsigned long long* count, hipStream_t stream,
                    std::string* err) {
-  if (plan.n_fwd_tasks == 0) return 0;
+  const PairList& L = plan.fwd;
+  if (L.n_tasks == 0) return 0;
   if (mode == 0) {
-    auto kern = band_forward_kernel<H, 0>;
+    auto kern = plan.tuning.recurrence ? band_forward_kernel<H, 0, true>
+                                       : band_forward_kernel<H, 0, false>;
     if (set_lds(kern, plan.fwd_lds, err)) return -1;
-    hipLaunchKernelGGL(kern, dim3(plan.n_fwd_tasks), dim3(kFwdThreads),
-                       plan.fwd_lds, stream, g, ev, plan.pair_event,
-                       plan.fwd_tasks, plan.rhoT[plan.curT], plan.partial,
-                       plan.max_bands, count);
+    hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves),
+                       plan.fwd_lds, stream, g, ev, L.pair_event, L.tasks,
+                       plan.rhoT[plan.curT], plan.Wt, plan.partial,
+                       L.max_bands, count);
   } else {
-    auto kern = band_forward_kernel<H, 1>;
-    hipLaunchKernelGGL(kern, dim3(plan.n_fwd_tasks), dim3(kFwdThreads), 0,
-                       stream, g, ev, plan.pair_event, plan.fwd_tasks,
-                       plan.rhoT[plan.curT], plan.partial, plan.max_bands,
-                       count);
+    auto kern = band_forward_kernel<H, 1, false>;
+    hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves), 0,
+                       stream, g, ev, L.pair_event, L.tasks,
+                       plan.rhoT[plan.curT], plan.Wt, plan.partial,
+                       L.max_bands, count);
   }
   BAND_TRY(hipGetLastError());
   return 0;
@@ -446,13 +588,15 @@ int launch_forward(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
 template <int H>
 int launch_backward(BandPlan& plan, const DeviceGeometry& g,
                     const EventSoA& ev, hipStream_t stream, std::string* err) {
-  if (plan.n_bwd_tasks == 0) return 0;
-  auto kern = band_backward_kernel<H>;
+  const PairList& L = plan.bwd;
+  if (L.n_tasks == 0) return 0;
+  auto kern = plan.tuning.recurrence ? band_backward_kernel<H, true>
+                                     : band_backward_kernel<H, false>;
   if (set_lds(kern, plan.bwd_lds, err)) return -1;
-  const int blocks = (plan.n_bwd_tasks + kBwdWaves - 1) / kBwdWaves;
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * kBwdWaves), plan.bwd_lds,
-                     stream, g, ev, plan.pair_event, plan.bwd_tasks,
-                     plan.n_bwd_tasks, plan.inv, plan.W, plan.bpT);
+  const int blocks = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
+  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * plan.bwd_waves),
+                     plan.bwd_lds, stream, g, ev, L.pair_event, L.tasks,
+                     L.n_tasks, plan.inv, plan.Wt, plan.bpT);
   BAND_TRY(hipGetLastError());
   return 0;
 }
@@ -460,8 +604,7 @@ int launch_backward(BandPlan& plan, const DeviceGeometry& g,
 int dispatch_forward(BandPlan& plan, const DeviceGeometry& g,
                      const EventSoA& ev, int mode, unsigned long long* count,
                      hipStream_t stream, std::string* err) {
-  switch (plan.H) {
-    case 16: return launch_forward<16>(plan, g, ev, mode, count, stream, err);
+  switch (plan.fwd.H) {
     case 32: return launch_forward<32>(plan, g, ev, mode, count, stream, err);
     case 64: return launch_forward<64>(plan, g, ev, mode, count, stream, err);
   }
@@ -472,8 +615,7 @@ int dispatch_forward(BandPlan& plan, const DeviceGeometry& g,
 int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
                       const EventSoA& ev, hipStream_t stream,
                       std::string* err) {
-  switch (plan.H) {
-    case 16: return launch_backward<16>(plan, g, ev, stream, err);
+  switch (plan.bwd.H) {
     case 32: return launch_backward<32>(plan, g, ev, stream, err);
     case 64: return launch_backward<64>(plan, g, ev, stream, err);
   }
@@ -487,139 +629,80 @@ void free_ptr(T*& p) {
   p = nullptr;
 }
 
-}  // namespace
-
-int band_rows_for(const DeviceGeometry& g) {
-  const size_t limit = 128 * 1024;
-  for (int H : {64, 32, 16})
-    if ((size_t)g.n_z * H * sizeof(float) <= limit) return H;
-  return 0;
+void free_list(PairList& L) {
+  if (L.owns_pairs) free_ptr(L.pair_event);
+  L.pair_event = nullptr;
+  free_ptr(L.tasks);
+  L = PairList{};
 }
 
-void band_release(BandPlan& plan) {
-  free_ptr(plan.pair_event);
-  free_ptr(plan.fwd_tasks);
-  free_ptr(plan.bwd_tasks);
-  free_ptr(plan.partial);
-  free_ptr(plan.inv);
-  free_ptr(plan.rhoT[0]);
-  free_ptr(plan.rhoT[1]);
-  free_ptr(plan.bpT);
-  free_ptr(plan.sensT);
-  plan = BandPlan{};
-}
+constexpr size_t kLdsLimit = 160 * 1024;
 
-int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
-                 int64_t n_events, hipStream_t stream, std::string* err) {
-  const int H = band_rows_for(g);
-  if (!H) {
-    if (err) *err = "image too wide for the row-band algorithm";
-    return -1;
-  }
-  if (n_events > 0x7fffffff) {
-    if (err) *err = "too many events for one band plan";
-    return -1;
-  }
-  const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
-  // keep the images; rebuild the lists
-  free_ptr(plan.pair_event);
-  free_ptr(plan.fwd_tasks);
-  free_ptr(plan.bwd_tasks);
-  free_ptr(plan.partial);
-  free_ptr(plan.inv);
-  plan.ready = false;
-  plan.H = H;
-  plan.n_bands = (g.n_y + H - 1) / H;
-  plan.n_events = n_events;
-  if (!plan.rhoT[0]) {
-    BAND_TRY(hipMalloc(&plan.rhoT[0], sizeof(float) * n_pixels));
-    BAND_TRY(hipMalloc(&plan.rhoT[1], sizeof(float) * n_pixels));
-    BAND_TRY(hipMalloc(&plan.bpT, sizeof(float) * n_pixels));
-    BAND_TRY(hipMemsetAsync(plan.bpT, 0, sizeof(float) * n_pixels, stream));
-  }
-  const int64_t n = n_events;
+// Sorted (event, band) pairs of band height H for the resident events, and
+// the index of the first pair of every (band, x0 >> shift) group.
+int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
+                const EventSoA& ev, int64_t n, hipStream_t stream,
+                std::vector<int>& first, std::string* err) {
+  L.H = H;
+  L.n_bands = (g.n_y + H - 1) / H;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
   int* counts = nullptr;
   int* offsets = nullptr;
+  int* d_red = nullptr;
+  void* tmp = nullptr;
   BAND_TRY(hipMalloc(&counts, sizeof(int) * std::max<int64_t>(n, 1)));
   BAND_TRY(hipMalloc(&offsets, sizeof(int) * std::max<int64_t>(n, 1)));
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  BAND_TRY(hipMalloc(&d_red, sizeof(int)));
+  int last_count = 0, last_offset = 0, max_count = 0;
   if (n > 0) {
     hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
                        ev.box, n, H, counts);
     BAND_TRY(hipGetLastError());
-  }
-  // scan and maximum
-  int* d_max = nullptr;
-  BAND_TRY(hipMalloc(&d_max, sizeof(int)));
-  size_t tmp_bytes = 0, t1 = 0, t2 = 0;
-  if (n > 0) {
+    size_t t1 = 0, t2 = 0;
     BAND_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, counts, offsets,
                                                (int)n, stream));
-    BAND_TRY(hipcub::DeviceReduce::Max(nullptr, t2, counts, d_max, (int)n,
+    BAND_TRY(hipcub::DeviceReduce::Max(nullptr, t2, counts, d_red, (int)n,
                                        stream));
-  }
-  tmp_bytes = std::max(t1, t2);
-  void* tmp = nullptr;
-  if (tmp_bytes) BAND_TRY(hipMalloc(&tmp, tmp_bytes));
-  int last_count = 0, last_offset = 0, max_count = 0;
-  if (n > 0) {
-    BAND_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, t1, counts, offsets, (int)n,
-                                               stream));
-    BAND_TRY(hipcub::DeviceReduce::Max(tmp, t2, counts, d_max, (int)n,
+    BAND_TRY(hipMalloc(&tmp, std::max(t1, t2)));
+    BAND_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, t1, counts, offsets,
+                                               (int)n, stream));
+    BAND_TRY(hipcub::DeviceReduce::Max(tmp, t2, counts, d_red, (int)n,
                                        stream));
     BAND_TRY(hipMemcpyAsync(&last_count, counts + n - 1, sizeof(int),
                             hipMemcpyDeviceToHost, stream));
     BAND_TRY(hipMemcpyAsync(&last_offset, offsets + n - 1, sizeof(int),
                             hipMemcpyDeviceToHost, stream));
-    BAND_TRY(hipMemcpyAsync(&max_count, d_max, sizeof(int),
-                            hipMemcpyDeviceToHost, stream));
-    BAND_TRY(hipStreamSynchronize(stream));
-  }
-  if (tmp) BAND_TRY(hipFree(tmp));
-  tmp = nullptr;
-  BAND_TRY(hipFree(d_max));
-  const int64_t n_pairs = (int64_t)last_offset + last_count;
-  plan.n_pairs = n_pairs;
-  plan.max_bands = std::max(max_count, 1);
-  // window width: power of two >= widest bbox (x1 - x0)
-  int max_width = 1;
-  if (n > 0) {
-    hipLaunchKernelGGL(box_width_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, n, counts);
-    BAND_TRY(hipGetLastError());
-    int* d_w = nullptr;
-    BAND_TRY(hipMalloc(&d_w, sizeof(int)));
-    size_t wb = 0;
-    BAND_TRY(hipcub::DeviceReduce::Max(nullptr, wb, counts, d_w, (int)n,
-                                       stream));
-    BAND_TRY(hipMalloc(&tmp, wb));
-    BAND_TRY(hipcub::DeviceReduce::Max(tmp, wb, counts, d_w, (int)n, stream));
-    BAND_TRY(hipMemcpyAsync(&max_width, d_w, sizeof(int),
+    BAND_TRY(hipMemcpyAsync(&max_count, d_red, sizeof(int),
                             hipMemcpyDeviceToHost, stream));
     BAND_TRY(hipStreamSynchronize(stream));
     BAND_TRY(hipFree(tmp));
     tmp = nullptr;
-    BAND_TRY(hipFree(d_w));
-    // the width pass overwrote the per-event pair counts: recompute them
-    hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, n, H, counts);
-    BAND_TRY(hipGetLastError());
   }
-  uint32_t* keys = nullptr;
-  uint32_t* keys_alt = nullptr;
-  int* vals_alt = nullptr;
+  BAND_TRY(hipFree(d_red));
+  const int64_t n_pairs = (int64_t)last_offset + last_count;
+  if (n_pairs > 0x7fffffff) {
+    if (err) *err = "too many (event, band) pairs";
+    return -1;
+  }
+  L.n_pairs = n_pairs;
+  L.max_bands = std::max(max_count, 1);
+  const int n_groups = L.n_bands << (16 - shift);
+  first.assign(n_groups + 1, -1);
   if (n_pairs > 0) {
+    uint32_t* keys = nullptr;
+    uint32_t* keys_alt = nullptr;
+    int* vals_alt = nullptr;
     BAND_TRY(hipMalloc(&keys, sizeof(uint32_t) * n_pairs));
     BAND_TRY(hipMalloc(&keys_alt, sizeof(uint32_t) * n_pairs));
-    BAND_TRY(hipMalloc(&plan.pair_event, sizeof(int) * n_pairs));
+    BAND_TRY(hipMalloc(&L.pair_event, sizeof(int) * n_pairs));
     BAND_TRY(hipMalloc(&vals_alt, sizeof(int) * n_pairs));
     hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, n, H, counts, offsets, keys, plan.pair_event);
+                       ev.box, n, H, counts, offsets, keys, L.pair_event);
     BAND_TRY(hipGetLastError());
     int end_bit = 16;
-    while ((1 << (end_bit - 16)) < plan.n_bands + 1) ++end_bit;
+    while ((1 << (end_bit - 16)) < L.n_bands + 1) ++end_bit;
     hipcub::DoubleBuffer<uint32_t> kb(keys, keys_alt);
-    hipcub::DoubleBuffer<int> vb(plan.pair_event, vals_alt);
+    hipcub::DoubleBuffer<int> vb(L.pair_event, vals_alt);
     size_t sort_bytes = 0;
     BAND_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, kb, vb,
                                                 (int)n_pairs, 0, end_bit,
@@ -632,76 +715,165 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     BAND_TRY(hipFree(tmp));
     tmp = nullptr;
     if (kb.Current() != keys) std::swap(keys, keys_alt);
-    if (vb.Current() != plan.pair_event) std::swap(plan.pair_event, vals_alt);
+    if (vb.Current() != L.pair_event) std::swap(L.pair_event, vals_alt);
     BAND_TRY(hipFree(keys_alt));
     BAND_TRY(hipFree(vals_alt));
-  }
-  // band boundaries
-  std::vector<int> first(plan.n_bands + 1, -1);
-  if (n_pairs > 0) {
     int* d_first = nullptr;
-    BAND_TRY(hipMalloc(&d_first, sizeof(int) * (plan.n_bands + 1)));
-    BAND_TRY(hipMemsetAsync(d_first, 0xff, sizeof(int) * (plan.n_bands + 1),
+    BAND_TRY(hipMalloc(&d_first, sizeof(int) * (n_groups + 1)));
+    BAND_TRY(hipMemsetAsync(d_first, 0xff, sizeof(int) * (n_groups + 1),
                             stream));
     const unsigned pb = (unsigned)((n_pairs + 255) / 256);
-    hipLaunchKernelGGL(band_bounds_kernel, dim3(pb), dim3(256), 0, stream,
-                       keys, n_pairs, d_first);
+    hipLaunchKernelGGL(group_bounds_kernel, dim3(pb), dim3(256), 0, stream,
+                       keys, n_pairs, shift, d_first);
     BAND_TRY(hipGetLastError());
     BAND_TRY(hipMemcpyAsync(first.data(), d_first,
-                            sizeof(int) * (plan.n_bands + 1),
+                            sizeof(int) * (n_groups + 1),
                             hipMemcpyDeviceToHost, stream));
     BAND_TRY(hipStreamSynchronize(stream));
     BAND_TRY(hipFree(d_first));
+    BAND_TRY(hipFree(keys));
   }
-  if (keys) BAND_TRY(hipFree(keys));
   BAND_TRY(hipFree(counts));
   BAND_TRY(hipFree(offsets));
-  // band ranges
-  std::vector<int> begin(plan.n_bands), end(plan.n_bands);
-  {
-    int next = (int)n_pairs;
-    for (int b = plan.n_bands - 1; b >= 0; --b) {
-      if (first[b] >= 0) {
-        begin[b] = first[b];
-        end[b] = next;
-        next = first[b];
-      } else {
-        begin[b] = end[b] = next;
-      }
-    }
+  return 0;
+}
+
+// Tasks of at most `chunk` consecutive pairs inside one group.
+int build_tasks(PairList& L, int shift, const std::vector<int>& first,
+                int chunk, hipStream_t stream, std::string* err) {
+  const int n_groups = (int)first.size() - 1;
+  std::vector<int4> tasks;
+  std::vector<int> end(n_groups, 0);
+  int next = (int)L.n_pairs;
+  for (int k = n_groups - 1; k >= 0; --k) {
+    end[k] = next;
+    if (first[k] >= 0) next = first[k];
   }
-  std::vector<int4> fwd, bwd;
-  for (int b = 0; b < plan.n_bands; ++b) {
-    for (int p = begin[b]; p < end[b]; p += kFwdPairsPerTask)
-      fwd.push_back(make_int4(b, p, std::min(end[b], p + kFwdPairsPerTask), 0));
-    for (int p = begin[b]; p < end[b]; p += kBwdPairsPerTask)
-      bwd.push_back(make_int4(b, p, std::min(end[b], p + kBwdPairsPerTask), 0));
+  chunk = std::max(chunk, 1);
+  for (int k = 0; k < n_groups; ++k) {
+    if (first[k] < 0) continue;
+    const int band = k >> (16 - shift);
+    const int c0 = (k & ((1 << (16 - shift)) - 1)) << shift;
+    for (int p = first[k]; p < end[k]; p += chunk)
+      tasks.push_back(make_int4(band, p, std::min(end[k], p + chunk), c0));
   }
-  plan.n_fwd_tasks = (int)fwd.size();
-  plan.n_bwd_tasks = (int)bwd.size();
-  if (!fwd.empty()) {
-    BAND_TRY(hipMalloc(&plan.fwd_tasks, sizeof(int4) * fwd.size()));
-    BAND_TRY(hipMemcpyAsync(plan.fwd_tasks, fwd.data(),
-                            sizeof(int4) * fwd.size(), hipMemcpyHostToDevice,
-                            stream));
+  L.n_tasks = (int)tasks.size();
+  if (!tasks.empty()) {
+    BAND_TRY(hipMalloc(&L.tasks, sizeof(int4) * tasks.size()));
+    BAND_TRY(hipMemcpyAsync(L.tasks, tasks.data(), sizeof(int4) * tasks.size(),
+                            hipMemcpyHostToDevice, stream));
+    BAND_TRY(hipStreamSynchronize(stream));
   }
-  if (!bwd.empty()) {
-    BAND_TRY(hipMalloc(&plan.bwd_tasks, sizeof(int4) * bwd.size()));
-    BAND_TRY(hipMemcpyAsync(plan.bwd_tasks, bwd.data(),
-                            sizeof(int4) * bwd.size(), hipMemcpyHostToDevice,
-                            stream));
+  return 0;
+}
+
+}  // namespace
+
+bool band_supported_grid(const DeviceGeometry& g) {
+  // group keys hold the band index above 16 bits of column
+  return g.n_z <= 32767 && g.n_y <= 32767;
+}
+
+void band_release(BandPlan& plan) {
+  free_list(plan.bwd);
+  free_list(plan.fwd);
+  free_ptr(plan.partial);
+  free_ptr(plan.inv);
+  free_ptr(plan.rhoT[0]);
+  free_ptr(plan.rhoT[1]);
+  free_ptr(plan.bpT);
+  free_ptr(plan.sensT);
+  BandTuning keep = plan.tuning;
+  plan = BandPlan{};
+  plan.tuning = keep;
+}
+
+int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
+                 int64_t n_events, hipStream_t stream, std::string* err) {
+  if (n_events > 0x7fffffff) {
+    if (err) *err = "too many events for one band plan";
+    return -1;
   }
-  int W = 32;
-  while (W < max_width) W *= 2;
-  plan.W = W;
-  plan.fwd_lds = (size_t)g.n_z * H * sizeof(float);
-  plan.bwd_lds = (size_t)kBwdWaves * W * H * sizeof(float);
-  if (plan.bwd_lds > 160 * 1024) {
+  const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
+  free_list(plan.bwd);
+  free_list(plan.fwd);
+  free_ptr(plan.partial);
+  free_ptr(plan.inv);
+  plan.ready = false;
+  plan.n_events = n_events;
+  if (!plan.rhoT[0]) {
+    BAND_TRY(hipMalloc(&plan.rhoT[0], sizeof(float) * n_pixels));
+    BAND_TRY(hipMalloc(&plan.rhoT[1], sizeof(float) * n_pixels));
+    BAND_TRY(hipMalloc(&plan.bpT, sizeof(float) * n_pixels));
+    BAND_TRY(hipMemsetAsync(plan.bpT, 0, sizeof(float) * n_pixels, stream));
+  }
+  const int64_t n = n_events;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  // widest clamped bbox
+  int max_width = 1;
+  if (n > 0) {
+    int* widths = nullptr;
+    int* d_red = nullptr;
+    void* tmp = nullptr;
+    BAND_TRY(hipMalloc(&widths, sizeof(int) * n));
+    BAND_TRY(hipMalloc(&d_red, sizeof(int)));
+    hipLaunchKernelGGL(box_width_kernel, dim3(blocks), dim3(256), 0, stream,
+                       ev.box, n, widths);
+    BAND_TRY(hipGetLastError());
+    size_t wb = 0;
+    BAND_TRY(hipcub::DeviceReduce::Max(nullptr, wb, widths, d_red, (int)n,
+                                       stream));
+    BAND_TRY(hipMalloc(&tmp, wb));
+    BAND_TRY(hipcub::DeviceReduce::Max(tmp, wb, widths, d_red, (int)n,
+                                       stream));
+    BAND_TRY(hipMemcpyAsync(&max_width, d_red, sizeof(int),
+                            hipMemcpyDeviceToHost, stream));
+    BAND_TRY(hipStreamSynchronize(stream));
+    BAND_TRY(hipFree(tmp));
+    BAND_TRY(hipFree(d_red));
+    BAND_TRY(hipFree(widths));
+    max_width = std::max(max_width, 1);
+  }
+  plan.max_width = max_width;
+  const BandTuning& tu = plan.tuning;
+  const int bucket = (tu.bucket == 16 || tu.bucket == 64) ? tu.bucket : 32;
+  const int shift = bucket == 16 ? 4 : (bucket == 64 ? 6 : 5);
+  plan.Wt = bucket + max_width;
+  const int Hf = tu.fwd_rows == 32 ? 32 : 64;
+  int Hb = tu.bwd_rows == 64 ? 64 : 32;
+  plan.fwd_lds = (size_t)plan.Wt * Hf * sizeof(float);
+  if (plan.fwd_lds > kLdsLimit) {
+    if (err) *err = "task window does not fit in LDS";
+    return -1;
+  }
+  if ((size_t)plan.Wt * Hb * sizeof(float) > kLdsLimit) Hb = 32;
+  const size_t window_bytes = (size_t)plan.Wt * Hb * sizeof(float);
+  plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
+  if (plan.bwd_waves < 1) {
     if (err) *err = "back-projection window does not fit in LDS";
     return -1;
   }
+  plan.bwd_lds = window_bytes * plan.bwd_waves;
+  std::vector<int> first;
+  if (build_pairs(plan.fwd, Hf, shift, g, ev, n, stream, first, err))
+    return -1;
+  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, stream, err))
+    return -1;
+  if (Hb == Hf) {
+    plan.bwd.H = Hf;
+    plan.bwd.n_bands = plan.fwd.n_bands;
+    plan.bwd.max_bands = plan.fwd.max_bands;
+    plan.bwd.n_pairs = plan.fwd.n_pairs;
+    plan.bwd.pair_event = plan.fwd.pair_event;
+    plan.bwd.owns_pairs = false;
+  } else if (build_pairs(plan.bwd, Hb, shift, g, ev, n, stream, first, err)) {
+    return -1;
+  }
+  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, stream, err))
+    return -1;
   if (n > 0) {
-    BAND_TRY(hipMalloc(&plan.partial, sizeof(float) * n * plan.max_bands));
+    BAND_TRY(hipMalloc(&plan.partial,
+                       sizeof(float) * n * plan.fwd.max_bands));
     BAND_TRY(hipMalloc(&plan.inv, sizeof(float) * n));
   }
   BAND_TRY(hipStreamSynchronize(stream));
@@ -759,7 +931,7 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
   if (plan.n_events > 0) {
     const unsigned blocks = (unsigned)((plan.n_events + 255) / 256);
     hipLaunchKernelGGL(band_inverse_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, plan.n_events, plan.H, plan.max_bands,
+                       ev.box, plan.n_events, plan.fwd.H, plan.fwd.max_bands,
                        plan.partial, guard_zero, plan.inv, nullptr);
     BAND_TRY(hipGetLastError());
   }
@@ -781,7 +953,7 @@ int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
   if (plan.n_events > 0) {
     const unsigned blocks = (unsigned)((plan.n_events + 255) / 256);
     hipLaunchKernelGGL(band_inverse_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, plan.n_events, plan.H, plan.max_bands,
+                       ev.box, plan.n_events, plan.fwd.H, plan.fwd.max_bands,
                        plan.partial, 1, nullptr, denominators);
     BAND_TRY(hipGetLastError());
   }

@@ -1,20 +1,21 @@
 // Row-band algorithm of the strip MLEM iteration (host interface).
 //
 // Each event is split into (event, band) pairs, a band being H consecutive
-// image rows over the full image width.  Pairs are radix-sorted once per
-// event upload by (band, bbox x0).  One iteration then runs
-//   band_forward   partial D per pair; the rho band lives in LDS
-//                  column-major (word = column*H + row), one image row per
-//                  lane, so lanes never share an LDS bank;
+// image rows (H = 64 or 32).  Pairs are radix-sorted once per event upload by
+// (band, bbox x0) and cut into tasks that never leave one (band, x0 bucket)
+// group, so every task touches at most Wt = bucket + max bbox width columns.
+// One iteration then runs
+//   band_forward   a workgroup stages the task's rho window (Wt columns x H
+//                  rows, column-major: word = column*H + row) in LDS; one
+//                  image row per lane (64/H lanes per row), so lanes never
+//                  share an LDS bank; partial D per pair;
 //   band_inverse   D_e = sum of the event's partials, inv_e = 1/D_e (guarded);
-//   band_backward  each wavefront sweeps x0-sorted pairs of one band through
-//                  a private sliding window of W columns x H rows in LDS,
-//                  accumulating K*inv_e with plain read-modify-write (no LDS
-//                  atomics: lanes own distinct rows, waves own their
-//                  window); columns leaving the window are added to the
-//                  column-major back-projection image with coalesced
-//                  atomics;
-//   band_finalize  rho'(p) = rho(p) * bp(p) in column-major layout.
+//   band_backward  each wavefront owns a private Wt x H window in LDS and
+//                  accumulates K * inv_e of its task's pairs with plain
+//                  read-modify-write (lanes own distinct rows, waves own their
+//                  window: no LDS atomics), then adds the window to the
+//                  column-major back-projection image with coalesced atomics;
+//   band_finalize  rho'(p) = rho(p) * bp(p) [/ s(p)], column-major.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -25,34 +26,53 @@
 
 namespace strip {
 
+struct BandTuning {
+  int fwd_rows = 64;     // forward band height: 64 or 32
+  int bwd_rows = 32;     // back-projection band height: 64 or 32
+  int bucket = 32;       // x0 bucket width in columns: 16, 32 or 64
+  int fwd_pairs = 4096;  // pairs per forward workgroup task
+  int bwd_pairs = 1024;  // pairs per back-projection wavefront task
+  // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
+  // two-term recurrence e[k+1] = e[k] r[k], r[k+1] = r[k] G of exp2 of a
+  // quadratic, restarted from exp2 every 8 (forward) / 4 (back-projection)
+  // pixels
+  int recurrence = 1;
+};
+
+// (event, band) pairs of one band height, sorted by (band, x0), and the
+// tasks cut from them.
+struct PairList {
+  int H = 0;                  // rows per band
+  int n_bands = 0;
+  int max_bands = 0;          // most bands one event touches
+  int64_t n_pairs = 0;
+  int* pair_event = nullptr;  // pair -> event
+  int4* tasks = nullptr;      // (band, begin, end, first column)
+  int n_tasks = 0;
+  bool owns_pairs = true;
+};
+
 struct BandPlan {
   bool ready = false;
-  int H = 0;          // rows per band (16, 32 or 64); lanes per row = 64/H
-  int W = 0;          // backward window width in columns (power of two)
-  int n_bands = 0;
-  int max_bands = 0;  // max bands any event touches (partial slots/event)
-  int64_t n_pairs = 0;
+  BandTuning tuning;
+  PairList fwd, bwd;
+  int Wt = 0;         // task window width in columns
+  int bwd_waves = 4;  // wavefronts per back-projection workgroup
+  int max_width = 0;  // widest clamped bbox (x1 - x0)
   int64_t n_events = 0;
-  int* pair_event = nullptr;  // pair -> event, sorted by (band, x0)
-  int4* fwd_tasks = nullptr;  // (band, begin, end, 0) per workgroup
-  int n_fwd_tasks = 0;
-  int4* bwd_tasks = nullptr;  // (band, begin, end, 0) per wavefront
-  int n_bwd_tasks = 0;
-  float* partial = nullptr;   // n_events * max_bands
+  float* partial = nullptr;   // n_events * fwd.max_bands
   float* inv = nullptr;       // n_events
   float* rhoT[2] = {nullptr, nullptr};  // column-major images (n_z * n_y)
   int curT = 0;
   float* bpT = nullptr;
   float* sensT = nullptr;
   size_t fwd_lds = 0, bwd_lds = 0;
-  int64_t pair_cap = 0, event_cap = 0;
 };
 
-// H for this grid, or 0 when the band algorithm cannot hold a band in LDS.
-int band_rows_for(const DeviceGeometry& g);
+// True when the row-band algorithm can run on this grid.
+bool band_supported_grid(const DeviceGeometry& g);
 
-// Builds the pair lists for the resident events.  Returns 0 or an error
-// message in *err.
+// Builds the pair lists and tasks for the resident events.
 int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                  int64_t n_events, hipStream_t stream, std::string* err);
 

@@ -20,6 +20,10 @@ struct DeviceGeometry {
   float tl_y, tl_z, tl_y_half_h, tl_z_half_w;
   float inv_pow_sigma_z, inv_pow_sigma_dl, norm_c;
   int n_z, n_y;
+  // derived constants of the span walk (not part of the reference state)
+  float inv_pixel_width;  // 1 / pixel_width
+  float C;                // 2 / sigma_z^2, the dz^2 coefficient of Q
+  float inv_2c;           // 1 / (2 C)
 };
 
 // Per-event parameters, precomputed once per event upload.

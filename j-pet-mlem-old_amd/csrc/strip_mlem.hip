@@ -290,6 +290,9 @@ DeviceGeometry make_device_geometry(const strip_mlem_geometry& geo) {
                                        g.inv_pow_sigma_dl));
   g.n_z = nz;
   g.n_y = ny;
+  g.inv_pixel_width = 1.0f / w;
+  g.C = 2.0f * g.inv_pow_sigma_z;  // as ellipse_bb computes C
+  g.inv_2c = 0.5f / g.C;
   return g;
 }
 
@@ -466,7 +469,7 @@ int launch_event_pass(strip_mlem_ctx* ctx, const float* rho) {
 
 bool band_supported(const strip_mlem_ctx* ctx) {
   return ctx->mode.kernel == 0 && ctx->mode.sensitivity != 1 &&
-         strip::band_rows_for(ctx->g) != 0;
+         strip::band_supported_grid(ctx->g);
 }
 
 int effective_algorithm(const strip_mlem_ctx* ctx) {
@@ -795,6 +798,50 @@ int strip_mlem_set_algorithm(strip_mlem_ctx* ctx, int algorithm) {
 
 int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx) {
   return ctx ? effective_algorithm(ctx) : -1;
+}
+
+int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
+                               int backward_rows, int bucket_columns,
+                               int forward_pairs, int backward_pairs) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if ((forward_rows != 32 && forward_rows != 64) ||
+      (backward_rows != 32 && backward_rows != 64) ||
+      (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
+      forward_pairs < 1 || backward_pairs < 1)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "band rows 32/64, bucket 16/32/64, pairs >= 1");
+  ctx->band.tuning.fwd_rows = forward_rows;
+  ctx->band.tuning.bwd_rows = backward_rows;
+  ctx->band.tuning.bucket = bucket_columns;
+  ctx->band.tuning.fwd_pairs = forward_pairs;
+  ctx->band.tuning.bwd_pairs = backward_pairs;
+  ctx->band_dirty = true;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (mode != 0 && mode != 1)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "exp mode must be 0 or 1");
+  ctx->band.tuning.recurrence = mode;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_band_tiling(strip_mlem_ctx* ctx, int* forward_rows,
+                               int* backward_rows, int* window_columns,
+                               int64_t* forward_pairs,
+                               int64_t* backward_pairs) {
+  CTX_CHECK(ctx);
+  if (effective_algorithm(ctx) != 2)
+    return strip::fail(STRIP_MLEM_ERR_STATE, "row-band algorithm not in use");
+  int rc = ensure_band(ctx);
+  if (rc) return rc;
+  if (forward_rows) *forward_rows = ctx->band.fwd.H;
+  if (backward_rows) *backward_rows = ctx->band.bwd.H;
+  if (window_columns) *window_columns = ctx->band.Wt;
+  if (forward_pairs) *forward_pairs = ctx->band.fwd.n_pairs;
+  if (backward_pairs) *backward_pairs = ctx->band.bwd.n_pairs;
+  return STRIP_MLEM_OK;
 }
 
 int strip_mlem_kernel_evaluations(strip_mlem_ctx* ctx, int64_t* count) {

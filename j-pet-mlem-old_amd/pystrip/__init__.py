@@ -119,6 +119,12 @@ def load_library():
         "strip_mlem_set_launch": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "strip_mlem_set_algorithm": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_get_algorithm": (ctypes.c_int, [P]),
+        "strip_mlem_set_band_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 5),
+        "strip_mlem_set_exp_mode": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_get_band_tiling": (ctypes.c_int, [
+            P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
+            ctypes.POINTER(i64)]),
         "strip_mlem_kernel_evaluations": (ctypes.c_int,
                                           [P, ctypes.POINTER(i64)]),
         "strip_mlem_forward": (ctypes.c_int, [P, fptr]),
@@ -159,6 +165,8 @@ def exported_symbols():
         "strip_mlem_iterate", "strip_mlem_iterate_async",
         "strip_mlem_synchronize", "strip_mlem_set_launch",
         "strip_mlem_set_algorithm", "strip_mlem_get_algorithm",
+        "strip_mlem_set_band_tiling", "strip_mlem_get_band_tiling",
+        "strip_mlem_set_exp_mode",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
@@ -268,6 +276,27 @@ class Reconstruction:
 
     def set_algorithm(self, algorithm: int):
         _check(self._lib.strip_mlem_set_algorithm(self._ctx, algorithm))
+
+    def set_band_tiling(self, forward_rows=64, backward_rows=32,
+                        bucket_columns=32, forward_pairs=4096,
+                        backward_pairs=1024):
+        _check(self._lib.strip_mlem_set_band_tiling(
+            self._ctx, forward_rows, backward_rows, bucket_columns,
+            forward_pairs, backward_pairs))
+
+    def set_exp_mode(self, mode: int):
+        """0 = exp2 per pixel, 1 = restarted exp recurrence along rows."""
+        _check(self._lib.strip_mlem_set_exp_mode(self._ctx, int(mode)))
+
+    def band_tiling(self):
+        """(forward rows, back-projection rows, window columns,
+        forward pairs, back-projection pairs)."""
+        hf, hb, w = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nf, nb = ctypes.c_int64(), ctypes.c_int64()
+        _check(self._lib.strip_mlem_get_band_tiling(
+            self._ctx, ctypes.byref(hf), ctypes.byref(hb), ctypes.byref(w),
+            ctypes.byref(nf), ctypes.byref(nb)))
+        return hf.value, hb.value, w.value, nf.value, nb.value
 
     @property
     def algorithm(self) -> int:

@@ -214,3 +214,24 @@ def test_algorithm_selection(pystrip_mod, phantom_mod):
     assert legacy.algorithm == 1
     with pytest.raises(pystrip_mod.StripMLEMError):
         legacy.set_algorithm(2)
+
+
+@pytest.mark.parametrize("name", [CFG1, CFG2])
+def test_recurrence_mode_parity(pystrip_mod, phantom_mod, oracle_mod, name):
+    # the restarted exp recurrence along rows (exp mode 1) against the oracle
+    print_case("exp recurrence mode", name)
+    w, ev, rec, og = _setup(pystrip_mod, phantom_mod, oracle_mod, name,
+                            50_000, algorithm=2)
+    rec.set_exp_mode(1)
+    rng = np.random.default_rng(5)
+    rho = rng.uniform(0.5, 2.0, size=rec.shape).astype(np.float32)
+    rec.rho = rho
+    d_gpu = rec.forward().astype(np.float64)
+    d_ref = oracle_mod.forward(og, ev, rho).astype(np.float64)
+    nz = d_ref > 0
+    assert np.max(np.abs(d_gpu[nz] - d_ref[nz]) / d_ref[nz]) < 2e-5
+    rec.rho = np.ones(rec.shape, np.float32)
+    rec.iterate(5)
+    e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=5))
+    print("rel-L2", e)
+    assert e <= 1e-4
