@@ -4,11 +4,14 @@
 //   - global float atomic add rate (L2 atomics), scattered and coalesced
 //   - LDS float atomic add rate
 // Every loop has a fixed trip count and every index stays inside its buffer.
-// Usage: strip_peak_probe  -> one JSON object on stdout.
+// Usage: strip_peak_probe           -> JSON objects on stdout
+//        strip_peak_probe --copy M  -> copies M MiB once with a plain kernel
+//                                     (calibrates HBM byte counters)
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #define CHECK(x)                                                        \
@@ -138,6 +141,32 @@ __global__ __launch_bounds__(256) void lds_pattern_probe(float* out, int mode) {
   out[blockIdx.x * blockDim.x + threadIdx.x] = acc;
 }
 
+__global__ __launch_bounds__(256) void copy_kernel(float4* __restrict__ dst,
+                                                   const float4* __restrict__ src,
+                                                   size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int copy_mode(size_t mib) {
+  const size_t bytes = mib << 20;
+  const size_t n = bytes / sizeof(float4);
+  float4 *a = nullptr, *b = nullptr;
+  CHECK(hipMalloc(&a, bytes));
+  CHECK(hipMalloc(&b, bytes));
+  CHECK(hipMemset(a, 1, bytes));
+  CHECK(hipDeviceSynchronize());
+  copy_kernel<<<4096, 256>>>(b, a, n);
+  CHECK(hipGetLastError());
+  CHECK(hipDeviceSynchronize());
+  std::printf("{\"copy_bytes\": %zu, \"read_kib\": %zu, \"write_kib\": %zu}\n",
+              bytes, bytes >> 10, bytes >> 10);
+  CHECK(hipFree(a));
+  CHECK(hipFree(b));
+  return 0;
+}
+
 template <typename F>
 float time_ms(F launch, int reps) {
   hipEvent_t a, b;
@@ -156,7 +185,9 @@ float time_ms(F launch, int reps) {
   return ms / reps;
 }
 
-int main() {
+int main(int argc, char** argv) {
+  if (argc == 3 && std::string(argv[1]) == "--copy")
+    return copy_mode((size_t)std::atol(argv[2]));
   hipDeviceProp_t prop;
   CHECK(hipGetDeviceProperties(&prop, 0));
   const int cus = prop.multiProcessorCount;

@@ -104,28 +104,12 @@ __device__ inline RowSpan row_span(const DeviceGeometry& g, float cz,
   return s;
 }
 
-struct EventView {
-  float cz, cy, A, B, P, Q0;
-  int x0, x1, y0, y1;
-};
-
-__device__ inline EventView load_event(const EventSoA& ev, int e) {
-  const float4 a = ev.a[e];
-  const float4 b = ev.b[e];
-  const float4 c = ev.c[e];
-  const uint2 box = ev.box[e];
-  EventView v;
-  v.cz = a.x;
-  v.cy = a.y;
-  v.A = b.x;
-  v.B = b.y;
-  v.P = c.x;
-  v.Q0 = c.y;
-  v.x0 = box.x & 0xffff;
-  v.x1 = box.x >> 16;
-  v.y0 = box.y & 0xffff;
-  v.y1 = box.y >> 16;
-  return v;
+__device__ inline void record_box(const BandRecord& rec, int& x0, int& x1,
+                                  int& y0, int& y1) {
+  x0 = rec.bx & 0xffff;
+  x1 = rec.bx >> 16;
+  y0 = rec.by & 0xffff;
+  y1 = rec.by >> 16;
 }
 
 // ---------------------------------------------------------------------------
@@ -133,28 +117,27 @@ __device__ inline EventView load_event(const EventSoA& ev, int e) {
 // ---------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void count_pairs_kernel(
-    const uint2* __restrict__ box, int64_t n, int H, int* __restrict__ counts) {
+    const BandRecord* __restrict__ rec, int64_t n, int H,
+    int* __restrict__ counts) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  const uint2 b = box[e];
-  const int x0 = b.x & 0xffff, x1 = b.x >> 16;
-  const int y0 = b.y & 0xffff, y1 = b.y >> 16;
+  int x0, x1, y0, y1;
+  record_box(rec[e], x0, x1, y0, y1);
   int c = 0;
   if (x1 > x0 && y1 > y0) c = (y1 - 1) / H - y0 / H + 1;
   counts[e] = c;
 }
 
 __global__ __launch_bounds__(256) void emit_pairs_kernel(
-    const uint2* __restrict__ box, int64_t n, int H,
+    const BandRecord* __restrict__ rec, int64_t n, int H,
     const int* __restrict__ counts, const int* __restrict__ offsets,
     uint32_t* __restrict__ keys, int* __restrict__ values) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int c = counts[e];
   if (!c) return;
-  const uint2 b = box[e];
-  const uint32_t x0 = b.x & 0xffff;
-  const int y0 = b.y & 0xffff;
+  const uint32_t x0 = rec[e].bx & 0xffff;
+  const int y0 = rec[e].by & 0xffff;
   const int base = offsets[e];
   for (int k = 0; k < c; ++k) {
     keys[base + k] = ((uint32_t)(y0 / H + k) << 16) | x0;
@@ -168,6 +151,49 @@ __global__ __launch_bounds__(256) void box_width_kernel(
   if (e >= n) return;
   const uint2 b = box[e];
   width[e] = max((int)(b.x >> 16) - (int)(b.x & 0xffff), 0);
+}
+
+// spatial order of the events: key = (bbox y0, bbox x0)
+__global__ __launch_bounds__(256) void order_keys_kernel(
+    const uint2* __restrict__ box, int64_t n, uint32_t* __restrict__ keys,
+    int* __restrict__ values) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const uint2 b = box[e];
+  keys[e] = ((b.y & 0xffff) << 16) | (b.x & 0xffff);
+  values[e] = (int)e;
+}
+
+// rec[i] = packed parameters of event order[i]
+__global__ __launch_bounds__(256) void pack_records_kernel(
+    EventSoA ev, const int* __restrict__ order, int64_t n,
+    BandRecord* __restrict__ rec) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = order[i];
+  const float4 a = ev.a[e];
+  const float4 b = ev.b[e];
+  const float4 c = ev.c[e];
+  const uint2 box = ev.box[e];
+  BandRecord r;
+  r.cz = a.x;
+  r.cy = a.y;
+  r.A = b.x;
+  r.B = b.y;
+  r.P = c.x;
+  r.Q0 = c.y;
+  r.bx = box.x;
+  r.by = box.y;
+  rec[i] = r;
+}
+
+// out[order[i]] = in[i]
+__global__ __launch_bounds__(256) void unpermute_kernel(
+    const float* __restrict__ in, const int* __restrict__ order, int64_t n,
+    float* __restrict__ out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[order[i]] = in[i];
 }
 
 // first[k] = index of the first sorted pair whose (band, x0 bucket) key is k
@@ -244,14 +270,13 @@ __global__ __launch_bounds__(256) void band_finalize_kernel(
 
 // D_e = sum of the event's band partials; inv_e = 1/D_e (guarded or not).
 __global__ __launch_bounds__(256) void band_inverse_kernel(
-    const uint2* __restrict__ box, int64_t n, int H, int max_bands,
+    const BandRecord* __restrict__ rec, int64_t n, int H, int max_bands,
     const float* __restrict__ partial, int guard, float* __restrict__ inv,
     float* __restrict__ denominators) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
-  const uint2 b = box[e];
-  const int x0 = b.x & 0xffff, x1 = b.x >> 16;
-  const int y0 = b.y & 0xffff, y1 = b.y >> 16;
+  int x0, x1, y0, y1;
+  record_box(rec[e], x0, x1, y0, y1);
   float d = 0.0f;
   if (x1 > x0 && y1 > y0) {
     const int nb = (y1 - 1) / H - y0 / H + 1;
@@ -296,25 +321,22 @@ struct PairBatch {
 };
 
 template <bool WITH_INV>
-__device__ inline PairBatch gather_batch(const EventSoA& ev,
+__device__ inline PairBatch gather_batch(const BandRecord* __restrict__ rec,
                                          const int* __restrict__ pair_event,
                                          const float* __restrict__ inv,
                                          int first, int end, int lane) {
   PairBatch pb;
   const int p = first + lane;
   pb.e = (p < end) ? pair_event[p] : 0;
-  const float4 a = ev.a[pb.e];
-  const float4 b = ev.b[pb.e];
-  const float4 c = ev.c[pb.e];
-  const uint2 box = ev.box[pb.e];
-  pb.cz = a.x;
-  pb.cy = a.y;
-  pb.A = b.x;
-  pb.B = b.y;
-  pb.P = c.x;
-  pb.Q0 = c.y;
-  pb.bx = box.x;
-  pb.by = box.y;
+  const BandRecord r = rec[pb.e];
+  pb.cz = r.cz;
+  pb.cy = r.cy;
+  pb.A = r.A;
+  pb.B = r.B;
+  pb.P = r.P;
+  pb.Q0 = r.Q0;
+  pb.bx = r.bx;
+  pb.by = r.by;
   pb.inv = WITH_INV ? inv[pb.e] : 0.0f;
   return pb;
 }
@@ -361,8 +383,9 @@ __device__ inline PairView pick(const PairBatch& pb, int j) {
 // MODE 0: partial D per pair; MODE 1: count the span pixels instead.
 template <int H, int MODE, bool REC>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
-    DeviceGeometry g, EventSoA ev, const int* __restrict__ pair_event,
-    const int4* __restrict__ tasks, const float* __restrict__ rhoT, int Wt,
+    DeviceGeometry g, const BandRecord* __restrict__ rec,
+    const int* __restrict__ pair_event, const int4* __restrict__ tasks,
+    const float* __restrict__ rhoT, int Wt,
     float* __restrict__ partial, int max_bands,
     unsigned long long* __restrict__ count) {
   extern __shared__ float win[];
@@ -394,12 +417,12 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   const float two_qc_step = 2.0f * qc * step;
   const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
   unsigned long long counted = 0;
-  PairBatch next = gather_batch<false>(ev, pair_event, nullptr, begin, end,
+  PairBatch next = gather_batch<false>(rec, pair_event, nullptr, begin, end,
                                        lane);
   for (int base = begin; base < end; base += 64) {
     const PairBatch cur = next;
     if (base + 64 < end)
-      next = gather_batch<false>(ev, pair_event, nullptr, base + 64, end, lane);
+      next = gather_batch<false>(rec, pair_event, nullptr, base + 64, end, lane);
     const int n_here = min(64, end - base);
     for (int j = 0; j < n_here; ++j) {
       const PairView v = pick(cur, j);
@@ -456,8 +479,9 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
 // column-major image.
 template <int H, bool REC>
 __global__ __launch_bounds__(256) void band_backward_kernel(
-    DeviceGeometry g, EventSoA ev, const int* __restrict__ pair_event,
-    const int4* __restrict__ tasks, int n_tasks, const float* __restrict__ inv,
+    DeviceGeometry g, const BandRecord* __restrict__ rec,
+    const int* __restrict__ pair_event, const int4* __restrict__ tasks,
+    int n_tasks, const float* __restrict__ inv,
     int Wt, float* __restrict__ bpT) {
   extern __shared__ float windows[];
   const int lane = threadIdx.x & 63;
@@ -479,12 +503,12 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
   const float qc = g.C * kExp2Scale;
   const float two_qc_step = 2.0f * qc * step;
   const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
-  PairBatch next = gather_batch<true>(ev, pair_event, inv, task.y, task.z,
+  PairBatch next = gather_batch<true>(rec, pair_event, inv, task.y, task.z,
                                       lane);
   for (int base = task.y; base < task.z; base += 64) {
     const PairBatch cur = next;
     if (base + 64 < task.z)
-      next = gather_batch<true>(ev, pair_event, inv, base + 64, task.z, lane);
+      next = gather_batch<true>(rec, pair_event, inv, base + 64, task.z, lane);
     const int n_here = min(64, task.z - base);
     for (int j = 0; j < n_here; ++j) {
       const PairView v = pick(cur, j);
@@ -571,13 +595,14 @@ int launch_forward(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                                        : band_forward_kernel<H, 0, false>;
     if (set_lds(kern, plan.fwd_lds, err)) return -1;
     hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves),
-                       plan.fwd_lds, stream, g, ev, L.pair_event, L.tasks,
+                       plan.fwd_lds, stream, g, plan.records, L.pair_event,
+                       L.tasks,
                        plan.rhoT[plan.curT], plan.Wt, plan.partial,
                        L.max_bands, count);
   } else {
     auto kern = band_forward_kernel<H, 1, false>;
     hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves), 0,
-                       stream, g, ev, L.pair_event, L.tasks,
+                       stream, g, plan.records, L.pair_event, L.tasks,
                        plan.rhoT[plan.curT], plan.Wt, plan.partial,
                        L.max_bands, count);
   }
@@ -595,7 +620,8 @@ int launch_backward(BandPlan& plan, const DeviceGeometry& g,
   if (set_lds(kern, plan.bwd_lds, err)) return -1;
   const int blocks = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * plan.bwd_waves),
-                     plan.bwd_lds, stream, g, ev, L.pair_event, L.tasks,
+                     plan.bwd_lds, stream, g, plan.records, L.pair_event,
+                     L.tasks,
                      L.n_tasks, plan.inv, plan.Wt, plan.bpT);
   BAND_TRY(hipGetLastError());
   return 0;
@@ -641,7 +667,7 @@ constexpr size_t kLdsLimit = 160 * 1024;
 // Sorted (event, band) pairs of band height H for the resident events, and
 // the index of the first pair of every (band, x0 >> shift) group.
 int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
-                const EventSoA& ev, int64_t n, hipStream_t stream,
+                const BandRecord* rec, int64_t n, hipStream_t stream,
                 std::vector<int>& first, std::string* err) {
   L.H = H;
   L.n_bands = (g.n_y + H - 1) / H;
@@ -656,7 +682,7 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
   int last_count = 0, last_offset = 0, max_count = 0;
   if (n > 0) {
     hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, n, H, counts);
+                       rec, n, H, counts);
     BAND_TRY(hipGetLastError());
     size_t t1 = 0, t2 = 0;
     BAND_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, counts, offsets,
@@ -697,7 +723,7 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
     BAND_TRY(hipMalloc(&L.pair_event, sizeof(int) * n_pairs));
     BAND_TRY(hipMalloc(&vals_alt, sizeof(int) * n_pairs));
     hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, n, H, counts, offsets, keys, L.pair_event);
+                       rec, n, H, counts, offsets, keys, L.pair_event);
     BAND_TRY(hipGetLastError());
     int end_bit = 16;
     while ((1 << (end_bit - 16)) < L.n_bands + 1) ++end_bit;
@@ -779,6 +805,8 @@ void band_release(BandPlan& plan) {
   free_list(plan.fwd);
   free_ptr(plan.partial);
   free_ptr(plan.inv);
+  free_ptr(plan.records);
+  free_ptr(plan.order);
   free_ptr(plan.rhoT[0]);
   free_ptr(plan.rhoT[1]);
   free_ptr(plan.bpT);
@@ -799,6 +827,8 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   free_list(plan.fwd);
   free_ptr(plan.partial);
   free_ptr(plan.inv);
+  free_ptr(plan.records);
+  free_ptr(plan.order);
   plan.ready = false;
   plan.n_events = n_events;
   if (!plan.rhoT[0]) {
@@ -835,6 +865,38 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     max_width = std::max(max_width, 1);
   }
   plan.max_width = max_width;
+  // spatial renumbering and packed records
+  if (n > 0) {
+    uint32_t* keys = nullptr;
+    uint32_t* keys_alt = nullptr;
+    int* vals_alt = nullptr;
+    void* tmp = nullptr;
+    BAND_TRY(hipMalloc(&keys, sizeof(uint32_t) * n));
+    BAND_TRY(hipMalloc(&keys_alt, sizeof(uint32_t) * n));
+    BAND_TRY(hipMalloc(&plan.order, sizeof(int) * n));
+    BAND_TRY(hipMalloc(&vals_alt, sizeof(int) * n));
+    BAND_TRY(hipMalloc(&plan.records, sizeof(BandRecord) * n));
+    hipLaunchKernelGGL(order_keys_kernel, dim3(blocks), dim3(256), 0, stream,
+                       ev.box, n, keys, plan.order);
+    BAND_TRY(hipGetLastError());
+    hipcub::DoubleBuffer<uint32_t> kb(keys, keys_alt);
+    hipcub::DoubleBuffer<int> vb(plan.order, vals_alt);
+    size_t sort_bytes = 0;
+    BAND_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, kb, vb,
+                                                (int)n, 0, 32, stream));
+    BAND_TRY(hipMalloc(&tmp, sort_bytes));
+    BAND_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, kb, vb,
+                                                (int)n, 0, 32, stream));
+    BAND_TRY(hipStreamSynchronize(stream));
+    BAND_TRY(hipFree(tmp));
+    if (vb.Current() != plan.order) std::swap(plan.order, vals_alt);
+    BAND_TRY(hipFree(keys));
+    BAND_TRY(hipFree(keys_alt));
+    BAND_TRY(hipFree(vals_alt));
+    hipLaunchKernelGGL(pack_records_kernel, dim3(blocks), dim3(256), 0, stream,
+                       ev, plan.order, n, plan.records);
+    BAND_TRY(hipGetLastError());
+  }
   const BandTuning& tu = plan.tuning;
   const int bucket = (tu.bucket == 16 || tu.bucket == 64) ? tu.bucket : 32;
   const int shift = bucket == 16 ? 4 : (bucket == 64 ? 6 : 5);
@@ -855,7 +917,8 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   }
   plan.bwd_lds = window_bytes * plan.bwd_waves;
   std::vector<int> first;
-  if (build_pairs(plan.fwd, Hf, shift, g, ev, n, stream, first, err))
+  if (build_pairs(plan.fwd, Hf, shift, g, plan.records, n, stream, first,
+                  err))
     return -1;
   if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, stream, err))
     return -1;
@@ -866,7 +929,8 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     plan.bwd.n_pairs = plan.fwd.n_pairs;
     plan.bwd.pair_event = plan.fwd.pair_event;
     plan.bwd.owns_pairs = false;
-  } else if (build_pairs(plan.bwd, Hb, shift, g, ev, n, stream, first, err)) {
+  } else if (build_pairs(plan.bwd, Hb, shift, g, plan.records, n, stream,
+                         first, err)) {
     return -1;
   }
   if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, stream, err))
@@ -931,8 +995,9 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
   if (plan.n_events > 0) {
     const unsigned blocks = (unsigned)((plan.n_events + 255) / 256);
     hipLaunchKernelGGL(band_inverse_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, plan.n_events, plan.fwd.H, plan.fwd.max_bands,
-                       plan.partial, guard_zero, plan.inv, nullptr);
+                       plan.records, plan.n_events, plan.fwd.H,
+                       plan.fwd.max_bands, plan.partial, guard_zero, plan.inv,
+                       nullptr);
     BAND_TRY(hipGetLastError());
   }
   if (dispatch_backward(plan, g, ev, stream, err)) return -1;
@@ -952,9 +1017,15 @@ int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
   if (dispatch_forward(plan, g, ev, 0, nullptr, stream, err)) return -1;
   if (plan.n_events > 0) {
     const unsigned blocks = (unsigned)((plan.n_events + 255) / 256);
+    // D in record order into plan.inv (free between iterations), then back
+    // to the caller's event order
     hipLaunchKernelGGL(band_inverse_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, plan.n_events, plan.fwd.H, plan.fwd.max_bands,
-                       plan.partial, 1, nullptr, denominators);
+                       plan.records, plan.n_events, plan.fwd.H,
+                       plan.fwd.max_bands, plan.partial, 1, nullptr,
+                       plan.inv);
+    BAND_TRY(hipGetLastError());
+    hipLaunchKernelGGL(unpermute_kernel, dim3(blocks), dim3(256), 0, stream,
+                       plan.inv, plan.order, plan.n_events, denominators);
     BAND_TRY(hipGetLastError());
   }
   return 0;

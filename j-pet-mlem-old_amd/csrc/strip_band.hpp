@@ -39,6 +39,15 @@ struct BandTuning {
   int recurrence = 1;
 };
 
+// Per-event parameters of the row-band kernels, packed in one 32-byte record
+// so a pair costs one cache-line access: ellipse centre (cz, cy), quadratic
+// form coefficients A, B (C is a constant of the geometry), row prefactor
+// coefficients P', Q0' and the clamped bbox (x0 | x1<<16, y0 | y1<<16).
+struct alignas(32) BandRecord {
+  float cz, cy, A, B, P, Q0;
+  uint32_t bx, by;
+};
+
 // (event, band) pairs of one band height, sorted by (band, x0), and the
 // tasks cut from them.
 struct PairList {
@@ -60,6 +69,10 @@ struct BandPlan {
   int bwd_waves = 4;  // wavefronts per back-projection workgroup
   int max_width = 0;  // widest clamped bbox (x1 - x0)
   int64_t n_events = 0;
+  // events renumbered in (bbox y0, x0) order so neighbouring pairs of a task
+  // share cache lines; order[i] = original index of record i
+  BandRecord* records = nullptr;
+  int* order = nullptr;
   float* partial = nullptr;   // n_events * fwd.max_bands
   float* inv = nullptr;       // n_events
   float* rhoT[2] = {nullptr, nullptr};  // column-major images (n_z * n_y)

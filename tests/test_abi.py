@@ -1,0 +1,75 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/*.h declares; host-side argument checking (no GPU compute)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADERS = [os.path.join(ROOT, "include", h)
+           for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
+
+
+def declared_functions(path):
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    names = set()
+    for m in re.finditer(r"\b([a-z_][a-z0-9_]*)\s*\(", text):
+        name = m.group(1)
+        if name.startswith(("strip_mlem_", "strip_phantom_", "strip_events_")):
+            # skip typedef'd function-pointer names
+            if not re.search(r"\(\s*\*\s*" + name + r"\b", text):
+                names.add(name)
+    return names
+
+
+def library_for(header):
+    base = os.path.basename(header)
+    lib = {"strip_mlem.h": "libstrip_mlem.so",
+           "strip_events.h": "libstrip_mlem.so",
+           "strip_phantom.h": "libstrip_mlem.so"}.get(base)
+    return os.path.join(ROOT, "j-pet-mlem-old_amd", "lib", lib) if lib else None
+
+
+@pytest.mark.parametrize("header", HEADERS, ids=os.path.basename)
+def test_library_exports_declared_symbols(header):
+    lib_path = library_for(header)
+    assert lib_path, f"no library mapped for {header}"
+    if not os.path.exists(lib_path):
+        pytest.skip(f"{lib_path} not built")
+    lib = ctypes.CDLL(lib_path)
+    names = declared_functions(header)
+    assert names, header
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_abi(pystrip_mod):
+    names = declared_functions(os.path.join(ROOT, "include", "strip_mlem.h"))
+    assert set(pystrip_mod.exported_symbols()) == names
+
+
+def test_argument_checks_without_gpu(pystrip_mod):
+    lib = pystrip_mod.load_library()
+    ctx = ctypes.c_void_p()
+    # NULL geometry, bad pixel counts and bad sigmas are rejected before any
+    # device is touched
+    assert lib.strip_mlem_create(None, None, 0, ctypes.byref(ctx)) != 0
+    g = pystrip_mod.Geometry.make(350, 500, 0, 160, 2, 10, 42)
+    assert lib.strip_mlem_create(ctypes.byref(g), None, 0,
+                                 ctypes.byref(ctx)) != 0
+    assert b"pixel" in lib.strip_mlem_last_error()
+    g = pystrip_mod.Geometry.make(350, 500, 256, 160, 2, -1, 42)
+    assert lib.strip_mlem_create(ctypes.byref(g), None, 0,
+                                 ctypes.byref(ctx)) != 0
+    assert b"sigma" in lib.strip_mlem_last_error()
+    m = pystrip_mod.Mode(5, 0, 0, 1)
+    g = pystrip_mod.Geometry.make(350, 500, 256, 160, 2, 10, 42)
+    assert lib.strip_mlem_create(ctypes.byref(g), ctypes.byref(m), 0,
+                                 ctypes.byref(ctx)) != 0
+    assert lib.strip_mlem_set_launch(None, 0, 0) != 0
+    assert lib.strip_mlem_n_events(None) == -1
