@@ -151,6 +151,15 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
  * from exp2 every 8 (forward) / 4 (back-projection) pixels.  Both stay within
  * the FP32 parity tolerance of the oracle. */
 int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode);
+/* Row runs of the row-band algorithm: 1 (default) = the in-ellipse column run
+ * of every bbox row of every event is solved once per upload and stored
+ * (2 bytes per row; used when every clamped bbox is at most 255 columns wide
+ * and the table takes at most half the free device memory), 0 = each
+ * iteration solves the runs again.  Same pixels either way.  Takes effect at
+ * the next upload-time plan (next iteration).  strip_mlem_get_span_table
+ * reports whether the plan in use has the table (1) or not (0). */
+int strip_mlem_set_span_table(strip_mlem_ctx* ctx, int enable);
+int strip_mlem_get_span_table(strip_mlem_ctx* ctx, int* in_use);
 
 /* The plan in use: band rows of both kernels, task window width (columns)
  * and the number of (event, band) pairs of both kernels. */

@@ -112,6 +112,33 @@ __device__ inline void record_box(const BandRecord& rec, int& x0, int& x1,
   y1 = rec.by >> 16;
 }
 
+// Row iy's run from a span-table word, with the row's exact terms computed as
+// in row_span.
+__device__ inline RowSpan stored_span(const DeviceGeometry& g, uint32_t word,
+                                      float cy, float A, float B, float P,
+                                      float Q0, int x0, int iy) {
+  RowSpan s;
+  s.n = (int)(word >> 8);
+  s.lo = x0 + (int)(word & 0xff);
+  const float py = pixel_center_y(g, iy);
+  const float dy = py - cy;
+  row_terms_exact(A, B, dy, s.rowA, s.bdy);
+  const float yp = dy + cy;
+  s.pre = __builtin_amdgcn_rsqf(__builtin_fmaf(P, yp * yp, Q0));
+  return s;
+}
+
+// Span-table word of row iy of record e.  The index is clamped into the
+// record's slot so the load is unconditional and always in bounds; the
+// caller drops the word for rows outside [y0, y1) when it uses it (keeping
+// the load free of conditions lets it stay in flight across pairs).
+__device__ inline uint32_t fetch_span(const uint16_t* __restrict__ spans,
+                                      int stride, int e, uint32_t by, int iy) {
+  const int y0 = (int)(by & 0xffff);
+  const int k = min(max(iy - y0, 0), stride - 1);
+  return spans[(int64_t)e * stride + k];
+}
+
 // ---------------------------------------------------------------------------
 // pair list construction
 // ---------------------------------------------------------------------------
@@ -145,12 +172,15 @@ __global__ __launch_bounds__(256) void emit_pairs_kernel(
   }
 }
 
-__global__ __launch_bounds__(256) void box_width_kernel(
-    const uint2* __restrict__ box, int64_t n, int* __restrict__ width) {
+// extent[e] = clamped bbox width (rows = 0) or height (rows = 1)
+__global__ __launch_bounds__(256) void box_extent_kernel(
+    const uint2* __restrict__ box, int64_t n, int rows,
+    int* __restrict__ extent) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const uint2 b = box[e];
-  width[e] = max((int)(b.x >> 16) - (int)(b.x & 0xffff), 0);
+  const uint32_t v = rows ? b.y : b.x;
+  extent[e] = max((int)(v >> 16) - (int)(v & 0xffff), 0);
 }
 
 // spatial order of the events: key = (bbox y0, bbox x0)
@@ -194,6 +224,26 @@ __global__ __launch_bounds__(256) void unpermute_kernel(
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   out[order[i]] = in[i];
+}
+
+// Span table: one wavefront per record, lanes over the rows of its bbox;
+// spans[i * stride + k] = (lo - x0) | n << 8 of row y0 + k (stride >= bbox
+// height, lo - x0 and n below 256).
+__global__ __launch_bounds__(256) void span_table_kernel(
+    DeviceGeometry g, const BandRecord* __restrict__ rec, int64_t n,
+    int stride, uint16_t* __restrict__ spans) {
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int lane = threadIdx.x & 63;
+  const BandRecord r = rec[i];
+  int x0, x1, y0, y1;
+  record_box(r, x0, x1, y0, y1);
+  uint16_t* out = spans + i * stride;
+  for (int k = lane; k < y1 - y0; k += 64) {
+    const RowSpan s =
+        row_span(g, r.cz, r.cy, r.A, r.B, r.P, r.Q0, x0, x1, y0, y1, y0 + k);
+    out[k] = (uint16_t)(s.n > 0 ? ((s.lo - x0) | (s.n << 8)) : 0);
+  }
 }
 
 // first[k] = index of the first sorted pair whose (band, x0 bucket) key is k
@@ -381,10 +431,11 @@ __device__ inline PairView pick(const PairBatch& pb, int j) {
 // lane = q*H + r covers row r of the band and every (64/H)-th column of the
 // row's span from offset q.
 // MODE 0: partial D per pair; MODE 1: count the span pixels instead.
-template <int H, int MODE, bool REC>
+template <int H, int MODE, bool REC, bool SPANS>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     DeviceGeometry g, const BandRecord* __restrict__ rec,
     const int* __restrict__ pair_event, const int4* __restrict__ tasks,
+    const uint16_t* __restrict__ spans, int span_stride,
     const float* __restrict__ rhoT, int Wt,
     float* __restrict__ partial, int max_bands,
     unsigned long long* __restrict__ count) {
@@ -424,10 +475,28 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     if (base + 64 < end)
       next = gather_batch<false>(rec, pair_event, nullptr, base + 64, end, lane);
     const int n_here = min(64, end - base);
+    // the next pair's span word is in flight while one pair is processed
+    uint32_t sp = 0;
+    if (SPANS)
+      sp = fetch_span(spans, span_stride, lane_int(cur.e, 0),
+                      (uint32_t)lane_int((int)cur.by, 0), y_base + r);
     for (int j = 0; j < n_here; ++j) {
       const PairView v = pick(cur, j);
-      const RowSpan s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0,
-                                 v.x1, v.y0, v.y1, y_base + r);
+      RowSpan s;
+      if (SPANS) {
+        const int iy = y_base + r;
+        const uint32_t word = (iy >= v.y0 && iy < v.y1) ? sp : 0u;
+        // the word is consumed before the next load reuses its register, so
+        // that load stays in flight through this pair
+        __builtin_amdgcn_sched_barrier(0);
+        const int j1 = min(j + 1, n_here - 1);
+        sp = fetch_span(spans, span_stride, lane_int(cur.e, j1),
+                        (uint32_t)lane_int((int)cur.by, j1), y_base + r);
+        s = stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy);
+      } else {
+        s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0, v.x1, v.y0,
+                     v.y1, y_base + r);
+      }
       const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
       if (MODE == 1) {
         counted += (unsigned long long)wave_sum_int(q == 0 ? s.n : 0);
@@ -477,12 +546,12 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
 // window, so plain read-modify-write suffices.  Four columns are read before
 // any is written (distinct words).  The window is then added to the
 // column-major image.
-template <int H, bool REC>
+template <int H, bool REC, bool SPANS>
 __global__ __launch_bounds__(256) void band_backward_kernel(
     DeviceGeometry g, const BandRecord* __restrict__ rec,
     const int* __restrict__ pair_event, const int4* __restrict__ tasks,
-    int n_tasks, const float* __restrict__ inv,
-    int Wt, float* __restrict__ bpT) {
+    int n_tasks, const uint16_t* __restrict__ spans, int span_stride,
+    const float* __restrict__ inv, int Wt, float* __restrict__ bpT) {
   extern __shared__ float windows[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -510,11 +579,29 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
     if (base + 64 < task.z)
       next = gather_batch<true>(rec, pair_event, inv, base + 64, task.z, lane);
     const int n_here = min(64, task.z - base);
+    uint32_t sp = 0;
+    if (SPANS)
+      sp = fetch_span(spans, span_stride, lane_int(cur.e, 0),
+                      (uint32_t)lane_int((int)cur.by, 0), y_base + r);
     for (int j = 0; j < n_here; ++j) {
       const PairView v = pick(cur, j);
-      if (v.inv == 0.0f) continue;
-      const RowSpan s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0,
-                                 v.x1, v.y0, v.y1, y_base + r);
+      const int iy = y_base + r;
+      uint32_t word = 0;
+      if (SPANS) {
+        // an event with inv = 0 adds nothing: empty run; the word is
+        // consumed before the next load reuses its register
+        word = (iy >= v.y0 && iy < v.y1 && v.inv != 0.0f) ? sp : 0u;
+        __builtin_amdgcn_sched_barrier(0);
+        const int j1 = min(j + 1, n_here - 1);
+        sp = fetch_span(spans, span_stride, lane_int(cur.e, j1),
+                        (uint32_t)lane_int((int)cur.by, j1), y_base + r);
+      } else if (v.inv == 0.0f) {
+        continue;
+      }
+      const RowSpan s =
+          SPANS ? stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy)
+                : row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0, v.x1,
+                           v.y0, v.y1, y_base + r);
       const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
       const int col0 = s.lo + q;
       float dz = __builtin_fmaf((float)col0, g.pixel_width,
@@ -590,21 +677,27 @@ int launch_forward(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                    std::string* err) {
   const PairList& L = plan.fwd;
   if (L.n_tasks == 0) return 0;
+  const bool sp = plan.spans != nullptr;
   if (mode == 0) {
-    auto kern = plan.tuning.recurrence ? band_forward_kernel<H, 0, true>
-                                       : band_forward_kernel<H, 0, false>;
+    auto kern =
+        plan.tuning.recurrence
+            ? (sp ? band_forward_kernel<H, 0, true, true>
+                  : band_forward_kernel<H, 0, true, false>)
+            : (sp ? band_forward_kernel<H, 0, false, true>
+                  : band_forward_kernel<H, 0, false, false>);
     if (set_lds(kern, plan.fwd_lds, err)) return -1;
     hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves),
                        plan.fwd_lds, stream, g, plan.records, L.pair_event,
-                       L.tasks,
+                       L.tasks, plan.spans, plan.span_stride,
                        plan.rhoT[plan.curT], plan.Wt, plan.partial,
                        L.max_bands, count);
   } else {
-    auto kern = band_forward_kernel<H, 1, false>;
+    auto kern = sp ? band_forward_kernel<H, 1, false, true>
+                   : band_forward_kernel<H, 1, false, false>;
     hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves), 0,
                        stream, g, plan.records, L.pair_event, L.tasks,
-                       plan.rhoT[plan.curT], plan.Wt, plan.partial,
-                       L.max_bands, count);
+                       plan.spans, plan.span_stride, plan.rhoT[plan.curT],
+                       plan.Wt, plan.partial, L.max_bands, count);
   }
   BAND_TRY(hipGetLastError());
   return 0;
@@ -615,14 +708,18 @@ int launch_backward(BandPlan& plan, const DeviceGeometry& g,
                     const EventSoA& ev, hipStream_t stream, std::string* err) {
   const PairList& L = plan.bwd;
   if (L.n_tasks == 0) return 0;
-  auto kern = plan.tuning.recurrence ? band_backward_kernel<H, true>
-                                     : band_backward_kernel<H, false>;
+  const bool sp = plan.spans != nullptr;
+  auto kern = plan.tuning.recurrence
+                  ? (sp ? band_backward_kernel<H, true, true>
+                        : band_backward_kernel<H, true, false>)
+                  : (sp ? band_backward_kernel<H, false, true>
+                        : band_backward_kernel<H, false, false>);
   if (set_lds(kern, plan.bwd_lds, err)) return -1;
   const int blocks = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * plan.bwd_waves),
                      plan.bwd_lds, stream, g, plan.records, L.pair_event,
-                     L.tasks,
-                     L.n_tasks, plan.inv, plan.Wt, plan.bpT);
+                     L.tasks, L.n_tasks, plan.spans, plan.span_stride,
+                     plan.inv, plan.Wt, plan.bpT);
   BAND_TRY(hipGetLastError());
   return 0;
 }
@@ -807,6 +904,7 @@ void band_release(BandPlan& plan) {
   free_ptr(plan.inv);
   free_ptr(plan.records);
   free_ptr(plan.order);
+  free_ptr(plan.spans);
   free_ptr(plan.rhoT[0]);
   free_ptr(plan.rhoT[1]);
   free_ptr(plan.bpT);
@@ -829,6 +927,7 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   free_ptr(plan.inv);
   free_ptr(plan.records);
   free_ptr(plan.order);
+  free_ptr(plan.spans);
   plan.ready = false;
   plan.n_events = n_events;
   if (!plan.rhoT[0]) {
@@ -839,31 +938,34 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   }
   const int64_t n = n_events;
   const unsigned blocks = (unsigned)((n + 255) / 256);
-  // widest clamped bbox
-  int max_width = 1;
+  // widest and tallest clamped bbox
+  int max_extent[2] = {1, 1};
   if (n > 0) {
-    int* widths = nullptr;
+    int* extent = nullptr;
     int* d_red = nullptr;
     void* tmp = nullptr;
-    BAND_TRY(hipMalloc(&widths, sizeof(int) * n));
+    BAND_TRY(hipMalloc(&extent, sizeof(int) * n));
     BAND_TRY(hipMalloc(&d_red, sizeof(int)));
-    hipLaunchKernelGGL(box_width_kernel, dim3(blocks), dim3(256), 0, stream,
-                       ev.box, n, widths);
-    BAND_TRY(hipGetLastError());
     size_t wb = 0;
-    BAND_TRY(hipcub::DeviceReduce::Max(nullptr, wb, widths, d_red, (int)n,
+    BAND_TRY(hipcub::DeviceReduce::Max(nullptr, wb, extent, d_red, (int)n,
                                        stream));
     BAND_TRY(hipMalloc(&tmp, wb));
-    BAND_TRY(hipcub::DeviceReduce::Max(tmp, wb, widths, d_red, (int)n,
-                                       stream));
-    BAND_TRY(hipMemcpyAsync(&max_width, d_red, sizeof(int),
-                            hipMemcpyDeviceToHost, stream));
-    BAND_TRY(hipStreamSynchronize(stream));
+    for (int rows = 0; rows < 2; ++rows) {
+      hipLaunchKernelGGL(box_extent_kernel, dim3(blocks), dim3(256), 0,
+                         stream, ev.box, n, rows, extent);
+      BAND_TRY(hipGetLastError());
+      BAND_TRY(hipcub::DeviceReduce::Max(tmp, wb, extent, d_red, (int)n,
+                                         stream));
+      BAND_TRY(hipMemcpyAsync(&max_extent[rows], d_red, sizeof(int),
+                              hipMemcpyDeviceToHost, stream));
+      BAND_TRY(hipStreamSynchronize(stream));
+      max_extent[rows] = std::max(max_extent[rows], 1);
+    }
     BAND_TRY(hipFree(tmp));
     BAND_TRY(hipFree(d_red));
-    BAND_TRY(hipFree(widths));
-    max_width = std::max(max_width, 1);
+    BAND_TRY(hipFree(extent));
   }
+  const int max_width = max_extent[0];
   plan.max_width = max_width;
   // spatial renumbering and packed records
   if (n > 0) {
@@ -896,6 +998,22 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     hipLaunchKernelGGL(pack_records_kernel, dim3(blocks), dim3(256), 0, stream,
                        ev, plan.order, n, plan.records);
     BAND_TRY(hipGetLastError());
+  }
+  // span table, when its byte runs fit and it takes at most half the free
+  // device memory
+  plan.span_stride = max_extent[1];
+  if (n > 0 && plan.tuning.spans && max_width <= 255) {
+    const size_t bytes = sizeof(uint16_t) * (size_t)n * plan.span_stride;
+    size_t free_bytes = 0, total_bytes = 0;
+    BAND_TRY(hipMemGetInfo(&free_bytes, &total_bytes));
+    if (bytes <= free_bytes / 2) {
+      BAND_TRY(hipMalloc(&plan.spans, bytes));
+      const unsigned span_blocks = (unsigned)((n + 3) / 4);
+      hipLaunchKernelGGL(span_table_kernel, dim3(span_blocks), dim3(256), 0,
+                         stream, g, plan.records, n, plan.span_stride,
+                         plan.spans);
+      BAND_TRY(hipGetLastError());
+    }
   }
   const BandTuning& tu = plan.tuning;
   const int bucket = (tu.bucket == 16 || tu.bucket == 64) ? tu.bucket : 32;

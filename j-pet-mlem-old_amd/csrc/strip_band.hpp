@@ -16,6 +16,9 @@
 //                  window: no LDS atomics), then adds the window to the
 //                  column-major back-projection image with coalesced atomics;
 //   band_finalize  rho'(p) = rho(p) * bp(p) [/ s(p)], column-major.
+// The in-ellipse pixels of an event in one image row form one run of
+// columns; the runs of every bbox row of every event are found once per
+// upload (span table) and read by both per-iteration kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -37,6 +40,9 @@ struct BandTuning {
   // quadratic, restarted from exp2 every 8 (forward) / 4 (back-projection)
   // pixels
   int recurrence = 1;
+  // 1 = build the span table when it fits, 0 = solve every row's run in the
+  // iteration kernels
+  int spans = 1;
 };
 
 // Per-event parameters of the row-band kernels, packed in one 32-byte record
@@ -73,6 +79,13 @@ struct BandPlan {
   // share cache lines; order[i] = original index of record i
   BandRecord* records = nullptr;
   int* order = nullptr;
+  // exact support of every bbox row of every record, computed once per
+  // upload: spans[i * span_stride + (iy - y0)] = (lo - x0) | n << 8 (first
+  // column and length of row iy's in-ellipse run); null when a bbox is wider
+  // than 255 columns or the table does not fit, then the kernels solve each
+  // row's span themselves
+  uint16_t* spans = nullptr;
+  int span_stride = 0;  // tallest clamped bbox (y1 - y0)
   float* partial = nullptr;   // n_events * fwd.max_bands
   float* inv = nullptr;       // n_events
   float* rhoT[2] = {nullptr, nullptr};  // column-major images (n_z * n_y)

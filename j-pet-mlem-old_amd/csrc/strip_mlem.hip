@@ -827,6 +827,28 @@ int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode) {
   return STRIP_MLEM_OK;
 }
 
+int strip_mlem_set_span_table(strip_mlem_ctx* ctx, int enable) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (enable != 0 && enable != 1)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "enable must be 0 or 1");
+  if (ctx->band.tuning.spans != enable) {
+    ctx->band.tuning.spans = enable;
+    ctx->band_dirty = true;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_span_table(strip_mlem_ctx* ctx, int* in_use) {
+  CTX_CHECK(ctx);
+  if (!in_use) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL output");
+  *in_use = 0;
+  if (effective_algorithm(ctx) != 2) return STRIP_MLEM_OK;
+  int rc = ensure_band(ctx);
+  if (rc) return rc;
+  *in_use = ctx->band.spans ? 1 : 0;
+  return STRIP_MLEM_OK;
+}
+
 int strip_mlem_get_band_tiling(strip_mlem_ctx* ctx, int* forward_rows,
                                int* backward_rows, int* window_columns,
                                int64_t* forward_pairs,

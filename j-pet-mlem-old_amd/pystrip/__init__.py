@@ -121,6 +121,9 @@ def load_library():
         "strip_mlem_get_algorithm": (ctypes.c_int, [P]),
         "strip_mlem_set_band_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 5),
         "strip_mlem_set_exp_mode": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_set_span_table": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_get_span_table": (ctypes.c_int,
+                                      [P, ctypes.POINTER(ctypes.c_int)]),
         "strip_mlem_get_band_tiling": (ctypes.c_int, [
             P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
             ctypes.POINTER(ctypes.c_int), ctypes.POINTER(i64),
@@ -166,7 +169,8 @@ def exported_symbols():
         "strip_mlem_synchronize", "strip_mlem_set_launch",
         "strip_mlem_set_algorithm", "strip_mlem_get_algorithm",
         "strip_mlem_set_band_tiling", "strip_mlem_get_band_tiling",
-        "strip_mlem_set_exp_mode",
+        "strip_mlem_set_exp_mode", "strip_mlem_set_span_table",
+        "strip_mlem_get_span_table",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
@@ -287,6 +291,17 @@ class Reconstruction:
     def set_exp_mode(self, mode: int):
         """0 = exp2 per pixel, 1 = restarted exp recurrence along rows."""
         _check(self._lib.strip_mlem_set_exp_mode(self._ctx, int(mode)))
+
+    def set_span_table(self, enable: bool):
+        """Store every event's per-row in-ellipse runs once per upload (True)
+        or solve them in every iteration (False)."""
+        _check(self._lib.strip_mlem_set_span_table(self._ctx, int(bool(enable))))
+
+    def span_table(self) -> bool:
+        """Whether the plan in use has the span table."""
+        v = ctypes.c_int()
+        _check(self._lib.strip_mlem_get_span_table(self._ctx, ctypes.byref(v)))
+        return bool(v.value)
 
     def band_tiling(self):
         """(forward rows, back-projection rows, window columns,

@@ -18,7 +18,10 @@ CFG1 = "strip-256x160-2mm-1e6"
 CFG2 = "strip-1000x400-1mm-2e7"
 
 
-ALGORITHMS = [1, 2]
+# hot-path variants: bbox wavefront kernel (1), row-band kernels with the
+# per-upload span table (2), row-band kernels solving the row runs in every
+# iteration (-2)
+ALGORITHMS = [1, 2, -2]
 
 
 def _setup(pystrip_mod, phantom_mod, oracle_mod, name, n, seed=0,
@@ -29,8 +32,12 @@ def _setup(pystrip_mod, phantom_mod, oracle_mod, name, n, seed=0,
     og = oracle_mod.geometry(*w.oracle_geometry_tuple())
     rec = pystrip_mod.Reconstruction(g, mode or pystrip_mod.MODE_CPU)
     if algorithm:
-        rec.set_algorithm(algorithm)
+        rec.set_algorithm(abs(algorithm))
+        if algorithm < 0:
+            rec.set_span_table(False)
     rec.set_events(ev)
+    if abs(algorithm) == 2:
+        assert rec.span_table() == (algorithm > 0)
     return w, ev, rec, og
 
 
@@ -53,7 +60,7 @@ def test_support_counts_exact(pystrip_mod, phantom_mod, oracle_mod, name,
     w, ev, rec, og = _setup(pystrip_mod, phantom_mod, oracle_mod, name, 20000,
                             algorithm=algorithm)
     nb, ne = oracle_mod.support_counts(og, ev)
-    assert rec.algorithm == algorithm
+    assert rec.algorithm == abs(algorithm)
     assert rec.kernel_evaluations() == int(ne.sum())
 
 
@@ -149,7 +156,8 @@ def test_empty_and_outside_events(pystrip_mod, phantom_mod, oracle_mod,
     print_case("empty / outside events, algorithm", algorithm)
     w = phantom_mod.WORKLOADS[CFG1]
     rec = pystrip_mod.Reconstruction(w.geometry())
-    rec.set_algorithm(algorithm)
+    rec.set_algorithm(abs(algorithm))
+    rec.set_span_table(algorithm > 0)
     rec.set_events(np.zeros((0, 3), np.float32))
     rec.iterate(2)
     assert np.all(rec.rho == 0)
@@ -157,7 +165,8 @@ def test_empty_and_outside_events(pystrip_mod, phantom_mod, oracle_mod,
     far = np.array([[2000.0, 2000.0, 0.0], [-3000.0, -3000.0, 10.0]],
                    np.float32)
     rec2 = pystrip_mod.Reconstruction(w.geometry())
-    rec2.set_algorithm(algorithm)
+    rec2.set_algorithm(abs(algorithm))
+    rec2.set_span_table(algorithm > 0)
     rec2.set_events(far)
     d = rec2.forward()
     og = oracle_mod.geometry(*w.oracle_geometry_tuple())
