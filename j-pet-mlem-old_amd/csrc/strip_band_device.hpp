@@ -1,0 +1,237 @@
+// Device helpers shared by the row-band kernels (strip_band.hip) and the
+// adjoint kernels (strip_adjoint.hip): exact row runs of the 3-sigma ellipse,
+// the span-table word, DPP wavefront sums and the batched pair gather.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "strip_band.hpp"
+#include "strip_device.hpp"
+
+namespace strip {
+namespace {
+
+constexpr int kFwdWaves = 8;  // wavefronts per forward workgroup
+
+#define BAND_TRY(expr)                                              \
+  do {                                                              \
+    hipError_t _e = (expr);                                         \
+    if (_e != hipSuccess) {                                         \
+      if (err) *err = std::string(#expr) + ": " + hipGetErrorString(_e); \
+      return -1;                                                    \
+    }                                                               \
+  } while (0)
+
+// Exact per-row terms of the 3-sigma test (gaussian_kernel.h:131-137):
+// Q(dz) = ((A dy) dy + (B dy) dz) + (C dz) dz, evaluated like the reference.
+__device__ inline void row_terms_exact(float A, float B, float dy,
+                                       float& rowA, float& bdy) {
+#pragma clang fp contract(off)
+  rowA = (A * dy) * dy;
+  bdy = B * dy;
+}
+
+__device__ inline float row_q_exact(const DeviceGeometry& g, float cz,
+                                    float C, float rowA, float bdy, int col) {
+#pragma clang fp contract(off)
+  float dz = pixel_center_z(g, col) - cz;
+  return (rowA + bdy * dz) + (C * dz * dz);
+}
+
+// The pixels of one image row inside an event's support: the row's columns
+// in the clamped bbox [x0, x1) whose centre passes Q <= 9.  The interval is
+// found from the real roots of Q(dz) = 9; where a root lies within 2e-3
+// pixel of a pixel centre (or the span is narrower than a pixel) the end
+// pixels are re-tested with the exact FP32 form, so the set equals the
+// reference's (reconstruction.h:255-264) pixel for pixel.
+struct RowSpan {
+  int lo;      // first column
+  int n;       // number of columns (0 = empty)
+  float rowA;  // (A dy) dy
+  float bdy;   // B dy
+  float pre;   // c / sqrt(a . S^-1 . a) of this row
+};
+
+__device__ inline RowSpan row_span(const DeviceGeometry& g, float cz,
+                                   float cy, float A, float B, float P,
+                                   float Q0, int x0, int x1, int y0, int y1,
+                                   int iy) {
+  const float C = g.C, inv_2c = g.inv_2c, inv_w = g.inv_pixel_width;
+  RowSpan s;
+  s.lo = 0;
+  s.n = 0;
+  s.rowA = 0.f;
+  s.bdy = 0.f;
+  s.pre = 0.f;
+  if (iy < y0 || iy >= y1) return s;
+  const float py = pixel_center_y(g, iy);
+  const float dy = py - cy;
+  row_terms_exact(A, B, dy, s.rowA, s.bdy);
+  const float yp = dy + cy;
+  s.pre = __builtin_amdgcn_rsqf(__builtin_fmaf(P, yp * yp, Q0));
+  const float disc = s.bdy * s.bdy - 4.0f * C * (s.rowA - 9.0f);
+  if (disc < 0.0f) return s;
+  const float half = __builtin_sqrtf(disc) * inv_2c;
+  const float mid = cz - s.bdy * inv_2c;
+  float fl = (mid - half - g.tl_z_half_w) * inv_w;
+  float fh = (mid + half - g.tl_z_half_w) * inv_w;
+  fl = fminf(fmaxf(fl, (float)x0 - 2.0f), (float)x1 + 1.0f);
+  fh = fminf(fmaxf(fh, (float)x0 - 2.0f), (float)x1 + 1.0f);
+  int lo = (int)ceilf(fl);
+  int hi = (int)floorf(fh);
+  const float eps = 2e-3f;
+  const bool check = (fl - floorf(fl) < eps) || (ceilf(fl) - fl < eps) ||
+                     (fh - floorf(fh) < eps) || (ceilf(fh) - fh < eps) ||
+                     (half * inv_w < 1.0f);
+  lo = max(lo, x0);
+  hi = min(hi, x1 - 1);
+  if (check) {
+    if (lo - 1 >= x0 && row_q_exact(g, cz, C, s.rowA, s.bdy, lo - 1) <= 9.0f)
+      lo -= 1;
+    else if (lo <= hi && row_q_exact(g, cz, C, s.rowA, s.bdy, lo) > 9.0f)
+      lo += 1;
+    if (hi + 1 <= x1 - 1 &&
+        row_q_exact(g, cz, C, s.rowA, s.bdy, hi + 1) <= 9.0f)
+      hi += 1;
+    else if (hi >= lo && row_q_exact(g, cz, C, s.rowA, s.bdy, hi) > 9.0f)
+      hi -= 1;
+  }
+  s.lo = lo;
+  s.n = max(hi - lo + 1, 0);
+  return s;
+}
+
+__device__ inline void record_box(const BandRecord& rec, int& x0, int& x1,
+                                  int& y0, int& y1) {
+  x0 = rec.bx & 0xffff;
+  x1 = rec.bx >> 16;
+  y0 = rec.by & 0xffff;
+  y1 = rec.by >> 16;
+}
+
+// Row iy's run from a span-table word, with the row's exact terms computed as
+// in row_span.
+__device__ inline RowSpan stored_span(const DeviceGeometry& g, uint32_t word,
+                                      float cy, float A, float B, float P,
+                                      float Q0, int x0, int iy) {
+  RowSpan s;
+  s.n = (int)(word >> 8);
+  s.lo = x0 + (int)(word & 0xff);
+  const float py = pixel_center_y(g, iy);
+  const float dy = py - cy;
+  row_terms_exact(A, B, dy, s.rowA, s.bdy);
+  const float yp = dy + cy;
+  s.pre = __builtin_amdgcn_rsqf(__builtin_fmaf(P, yp * yp, Q0));
+  return s;
+}
+
+// Span-table word of row iy of record e.  The index is clamped into the
+// record's slot so the load is unconditional and always in bounds; the
+// caller drops the word for rows outside [y0, y1) when it uses it (keeping
+// the load free of conditions lets it stay in flight across pairs).
+__device__ inline uint32_t fetch_span(const uint16_t* __restrict__ spans,
+                                      int stride, int e, uint32_t by, int iy) {
+  const int y0 = (int)(by & 0xffff);
+  const int k = min(max(iy - y0, 0), stride - 1);
+  return spans[(int64_t)e * stride + k];
+}
+
+template <int CTRL>
+__device__ inline float dpp_move(float v) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL,
+                                         0xF, 0xF, false));
+}
+
+__device__ inline float lane_value(float v, int lane) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
+// Sum over the wavefront with DPP inside each 16-lane row (quad swaps, then
+// row rotations by 4 and 8), then the four row totals read as scalars.
+// Returns the total (wave-uniform).
+__device__ inline float wave_total(float v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x128>(v);  // row_ror:8
+  return (lane_value(v, 0) + lane_value(v, 16)) +
+         (lane_value(v, 32) + lane_value(v, 48));
+}
+
+// Records of up to 64 consecutive pairs, one pair per lane, gathered with a
+// handful of vector loads so the random event reads of a whole batch are in
+// flight together; each pair's fields are then handed to the wave with
+// v_readlane (wave-uniform scalars).
+struct PairBatch {
+  int e;
+  float cz, cy, A, B, P, Q0, inv;
+  uint32_t bx, by;
+};
+
+template <bool WITH_INV>
+__device__ inline PairBatch gather_batch(const BandRecord* __restrict__ rec,
+                                         const int* __restrict__ pair_event,
+                                         const float* __restrict__ inv,
+                                         int first, int end, int lane) {
+  PairBatch pb;
+  const int p = first + lane;
+  pb.e = (p < end) ? pair_event[p] : 0;
+  const BandRecord r = rec[pb.e];
+  pb.cz = r.cz;
+  pb.cy = r.cy;
+  pb.A = r.A;
+  pb.B = r.B;
+  pb.P = r.P;
+  pb.Q0 = r.Q0;
+  pb.bx = r.bx;
+  pb.by = r.by;
+  pb.inv = WITH_INV ? inv[pb.e] : 0.0f;
+  return pb;
+}
+
+__device__ inline int lane_int(int v, int j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+__device__ inline float lane_float(float v, int j) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), j));
+}
+
+struct PairView {
+  int e;
+  float cz, cy, A, B, P, Q0, inv;
+  int x0, x1, y0, y1;
+};
+
+__device__ inline PairView pick(const PairBatch& pb, int j) {
+  PairView v;
+  v.e = lane_int(pb.e, j);
+  v.cz = lane_float(pb.cz, j);
+  v.cy = lane_float(pb.cy, j);
+  v.A = lane_float(pb.A, j);
+  v.B = lane_float(pb.B, j);
+  v.P = lane_float(pb.P, j);
+  v.Q0 = lane_float(pb.Q0, j);
+  v.inv = lane_float(pb.inv, j);
+  const uint32_t bx = (uint32_t)lane_int((int)pb.bx, j);
+  const uint32_t by = (uint32_t)lane_int((int)pb.by, j);
+  v.x0 = bx & 0xffff;
+  v.x1 = bx >> 16;
+  v.y0 = by & 0xffff;
+  v.y1 = by >> 16;
+  return v;
+}
+
+// Forward projection.  Workgroup = one task (pairs of one band and one x0
+// bucket); its rho window of Wt columns from c0 lives in LDS column-major
+// (word = (col - c0) * H + row; lanes of a column touch distinct rows, hence
+// distinct banks).  Wave w takes a contiguous share of the task's pairs;
+// lane = q*H + r covers row r of the band and every (64/H)-th column of the
+// row's span from offset q.
+// MODE 0: partial D per pair; MODE 1: count the span pixels instead.
+}  // namespace
+}  // namespace strip

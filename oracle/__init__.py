@@ -108,6 +108,9 @@ def lib():
             "oracle_forward_f64": (ctypes.c_int, [G, M, dp, i64, dp, dp, ctypes.c_int]),
             "oracle_support_counts_f32": (ctypes.c_int, [G, fp, i64, i64p, i64p, ctypes.c_int]),
             "oracle_loglik_f64": (d, [G, M, dp, i64, dp, dp, ctypes.c_int]),
+            "oracle_loglik_grad_f64": (ctypes.c_int, [G, dp, i64, dp, dp, dp, dp, dp]),
+            "oracle_mlem_loss_f64": (ctypes.c_int, [G, dp, i64, dp, ctypes.c_int, dp, dp,
+                                                     dp, dp, dp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -249,3 +252,49 @@ def rel_l2(a, b):
     b = np.asarray(b, np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def loglik_grad(g, events, rho, sigma=None):
+    """FP64 Poisson log-likelihood L = sum_{D>0} log D_e - sum rho and its
+    gradients (d L / d rho, d L / d (sigma_z, sigma_dl)); the kernel at
+    `sigma` (default: the geometry's), the support at the geometry's sigmas."""
+    ev = np.ascontiguousarray(events, np.float64)
+    r = np.ascontiguousarray(rho, np.float64)
+    sg = np.array(sigma if sigma is not None else (g.sigma_z, g.sigma_dl),
+                  np.float64)
+    value = np.zeros(1)
+    grad_rho = np.empty_like(r)
+    grad_sigma = np.zeros(2)
+    rc = lib().oracle_loglik_grad_f64(ctypes.byref(g), _dp(ev), ev.shape[0],
+                                      _dp(r), _dp(sg), _dp(value),
+                                      _dp(grad_rho), _dp(grad_sigma))
+    if rc:
+        raise RuntimeError("oracle_loglik_grad_f64 failed")
+    return float(value[0]), grad_rho, grad_sigma
+
+
+def mlem_loss(g, events, rho0, iterations, target, sigma=None, d_rho0=None,
+              d_sigma=None):
+    """FP64 unrolled-MLEM loss mean((rho_N - target)^2), its directional
+    derivative along (d_rho0, d_sigma) (forward mode) and rho_N."""
+    ev = np.ascontiguousarray(events, np.float64)
+    r0 = np.ascontiguousarray(rho0, np.float64)
+    tg = np.ascontiguousarray(target, np.float64)
+    sg = np.array(sigma if sigma is not None else (g.sigma_z, g.sigma_dl),
+                  np.float64)
+    dr = None if d_rho0 is None else np.ascontiguousarray(d_rho0, np.float64)
+    ds = None if d_sigma is None else np.array(d_sigma, np.float64)
+    loss = np.zeros(1)
+    jvp = np.zeros(1)
+    rho_n = np.empty_like(r0)
+    rc = lib().oracle_mlem_loss_f64(ctypes.byref(g), _dp(ev), ev.shape[0],
+                                    _dp(r0), int(iterations), _dp(tg), _dp(sg),
+                                    _dp(dr), _dp(ds), _dp(loss), _dp(jvp),
+                                    _dp(rho_n))
+    if rc:
+        raise RuntimeError("oracle_mlem_loss_f64 failed")
+    return float(loss[0]), float(jvp[0]), rho_n

@@ -124,6 +124,31 @@ double oracle_loglik_f64(const oracle_geometry* g, const oracle_mode* m,
                          const double* rho, const double* sens_weights,
                          int n_threads);
 
+/* ---- differentiation checker (FP64) ----
+ * The kernel is evaluated at sigma = {sigma_z, sigma_dl} while every event
+ * keeps the support of the geometry's own sigmas (the pixel sums are smooth in
+ * sigma at fixed support; their derivative is the almost-everywhere derivative
+ * of the reference's objective).
+ *
+ * L = sum_{e: D_e > 0} log D_e - sum_p rho(p) with its gradients d L/d rho
+ * (n_y x n_z row-major, may be NULL) and d L / d{sigma_z, sigma_dl}. */
+int oracle_loglik_grad_f64(const oracle_geometry* g, const double* events,
+                           int64_t n_events, const double* rho,
+                           const double* sigma, double* value,
+                           double* grad_rho, double* grad_sigma);
+
+/* Unrolled MLEM loss: rho_{k+1} = rho_k * sum_e K_e / D_e (guarded) for
+ * n_iterations steps from rho0, loss = mean_p (rho_N - target)^2.  When jvp is
+ * not NULL and a tangent (d_rho0 and/or d_sigma) is given, *jvp receives the
+ * directional derivative of the loss along it (forward-mode propagation).
+ * rho_out (may be NULL) receives rho_N. */
+int oracle_mlem_loss_f64(const oracle_geometry* g, const double* events,
+                         int64_t n_events, const double* rho0,
+                         int n_iterations, const double* target,
+                         const double* sigma, const double* d_rho0,
+                         const double* d_sigma, double* loss, double* jvp,
+                         double* rho_out);
+
 #ifdef __cplusplus
 }
 #endif
