@@ -57,6 +57,51 @@ def cpu_baseline(workload, n_sample, threads):
     }
 
 
+def adjoint_timing(pystrip, phantom, name, sigma, iterations):
+    """Cost of the differentiation form against plain MLEM iterations on the
+    same context: unrolled-MLEM loss gradient through `iterations` updates
+    (forward with checkpoints + reverse steps) and the log-likelihood
+    gradient, at the initial sigma guess."""
+    w = phantom.WORKLOADS[name]
+    g = w.geometry()
+    g.sigma_z, g.sigma_dl = sigma
+    rec = pystrip.Reconstruction(g, pystrip.MODE_CPU, device=0)
+    rec.set_events(w.events())
+    target = phantom.phantom_image(w.phantom, w.n_z, w.n_y,
+                                   w.pixel).astype(np.float32)
+    rho0 = np.ones(rec.shape, np.float32)
+    rec.iterate(1)
+    rec.unrolled_gradient(rho0, 1, target)
+    rec.synchronize()
+    t = time.perf_counter()
+    rec.iterate(iterations)
+    rec.synchronize()
+    t_iter = (time.perf_counter() - t) / iterations
+    t = time.perf_counter()
+    loss, _, gs = rec.unrolled_gradient(rho0, iterations, target)
+    t_unrolled = time.perf_counter() - t
+    t = time.perf_counter()
+    _, _, gl = rec.loglik_gradient()
+    t_loglik = time.perf_counter() - t
+    reverse = (t_unrolled - iterations * t_iter) / iterations
+    rec.close()
+    return {
+        "workload": name,
+        "n_events": w.n_events,
+        "sigma_mm": list(sigma),
+        "iterations": iterations,
+        "mlem_iteration_ms": t_iter * 1e3,
+        "unrolled_loss_gradient_ms": t_unrolled * 1e3,
+        "reverse_step_ms": reverse * 1e3,
+        "reverse_step_over_iteration": reverse / t_iter,
+        "loglik_gradient_ms": t_loglik * 1e3,
+        "loglik_gradient_over_iteration": t_loglik / t_iter,
+        "loss": loss,
+        "d_loss_d_sigma": [float(x) for x in gs],
+        "d_loglik_d_sigma": [float(x) for x in gl],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,12 +113,14 @@ def main():
     ap.add_argument("--algorithm", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=4_000_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--adjoint-workload", default="strip-512x256-1mm-5e6")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for "
-              f"{args.gpus}); the multi-GPU sharding is described in "
-              f"DESIGN.md but not launched", file=sys.stderr)
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); "
+              f"event sharding over several GPUs is not implemented",
+              file=sys.stderr)
         sys.exit(2)
 
     import pystrip
@@ -182,6 +229,11 @@ def main():
         "timing_s": {"event_generation": t_gen, "upload_and_setup": t_up},
         "check": {"image_mass": mass},
     }
+    if not args.no_adjoint:
+        rec.close()
+        line["adjoint"] = adjoint_timing(pystrip, phantom,
+                                         args.adjoint_workload, (14.0, 60.0),
+                                         8)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, threads)

@@ -177,6 +177,42 @@ int strip_mlem_kernel_evaluations(strip_mlem_ctx* ctx, int64_t* count);
  * current image (host array of n_events floats). */
 int strip_mlem_forward(strip_mlem_ctx* ctx, float* denominators);
 
+/* ---- differentiation (no counterpart in the reference; the adjoint form of
+ * the same kernels).  Defined for the CPU-path semantics (Gaussian kernel, no
+ * sensitivity, guarded 1/D) on the row-band algorithm; other modes return
+ * STRIP_MLEM_ERR_UNSUPPORTED.  The support of every event (clamped bbox and
+ * Q <= 9 at the context's sigmas) is held fixed, so the sigma derivatives are
+ * the almost-everywhere derivatives of the pixel sums.  Images are row-major
+ * n_y x n_z (PixelMap order). ---- */
+
+/* Poisson list-mode log-likelihood of image rho (NULL = the context image)
+ *   L = sum_{e: D_e > 0} log D_e - sum_p rho(p)
+ * (the objective whose EM step is the reference's update,
+ * reconstruction.h:110-154), its gradient d L / d rho (may be NULL) and
+ * d L / d sigma_z, d L / d sigma_dl in grad_sigma[0..1] (may be NULL).
+ * Host pointers; the _device variant takes device pointers. */
+int strip_mlem_loglik_gradient(strip_mlem_ctx* ctx, const float* rho,
+                               double* value, float* grad_rho,
+                               double* grad_sigma);
+int strip_mlem_loglik_gradient_device(strip_mlem_ctx* ctx, const float* rho,
+                                      double* value, float* grad_rho,
+                                      double* grad_sigma);
+
+/* Unrolled MLEM: n_iterations reference updates from rho0 (NULL = the context
+ * image), every iterate kept as a checkpoint; loss = mean_p (rho_N(p) -
+ * target(p))^2; its gradient with respect to rho0 (may be NULL) and to
+ * (sigma_z, sigma_dl) (may be NULL), by reverse steps through the iterations.
+ * The context image becomes rho_N.  Host pointers; the _device variant takes
+ * device pointers. */
+int strip_mlem_unrolled_gradient(strip_mlem_ctx* ctx, const float* rho0,
+                                 int n_iterations, const float* target,
+                                 double* loss, float* grad_rho0,
+                                 double* grad_sigma);
+int strip_mlem_unrolled_gradient_device(strip_mlem_ctx* ctx, const float* rho0,
+                                        int n_iterations, const float* target,
+                                        double* loss, float* grad_rho0,
+                                        double* grad_sigma);
+
 /* Timing of the hot kernel(s) with HIP events on the context stream.
  * After strip_mlem_set_timing(ctx, 1), every iteration records the start and
  * end of its event-loop kernel(s); strip_mlem_get_timing returns the summed

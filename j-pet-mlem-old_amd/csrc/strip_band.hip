@@ -486,7 +486,8 @@ int launch_forward(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
 
 template <int H>
 int launch_backward(BandPlan& plan, const DeviceGeometry& g,
-                    const EventSoA& ev, hipStream_t stream, std::string* err) {
+                    const float* weights, float* outT, hipStream_t stream,
+                    std::string* err) {
   const PairList& L = plan.bwd;
   if (L.n_tasks == 0) return 0;
   const bool sp = plan.spans != nullptr;
@@ -500,7 +501,7 @@ int launch_backward(BandPlan& plan, const DeviceGeometry& g,
   hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * plan.bwd_waves),
                      plan.bwd_lds, stream, g, plan.records, L.pair_event,
                      L.tasks, L.n_tasks, plan.spans, plan.span_stride,
-                     plan.inv, plan.Wt, plan.bpT);
+                     weights, plan.Wt, outT);
   BAND_TRY(hipGetLastError());
   return 0;
 }
@@ -517,11 +518,11 @@ int dispatch_forward(BandPlan& plan, const DeviceGeometry& g,
 }
 
 int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
-                      const EventSoA& ev, hipStream_t stream,
+                      const float* weights, float* outT, hipStream_t stream,
                       std::string* err) {
   switch (plan.bwd.H) {
-    case 32: return launch_backward<32>(plan, g, ev, stream, err);
-    case 64: return launch_backward<64>(plan, g, ev, stream, err);
+    case 32: return launch_backward<32>(plan, g, weights, outT, stream, err);
+    case 64: return launch_backward<64>(plan, g, weights, outT, stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -899,7 +900,7 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                        nullptr);
     BAND_TRY(hipGetLastError());
   }
-  if (dispatch_backward(plan, g, ev, stream, err)) return -1;
+  if (dispatch_backward(plan, g, plan.inv, plan.bpT, stream, err)) return -1;
   if (t_bwd) BAND_TRY(hipEventRecord(t_bwd, stream));
   const int blocks = (int)std::min<int64_t>((n_pixels + 255) / 256, 4096);
   hipLaunchKernelGGL(band_finalize_kernel, dim3(blocks), dim3(256), 0, stream,
@@ -928,6 +929,34 @@ int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
     BAND_TRY(hipGetLastError());
   }
   return 0;
+}
+
+int band_to_column_major(const DeviceGeometry& g, const float* in, float* outT,
+                         hipStream_t stream, std::string* err) {
+  dim3 grid((g.n_z + 31) / 32, (g.n_y + 31) / 32);
+  hipLaunchKernelGGL(to_column_major_kernel, grid, dim3(256), 0, stream, in,
+                     nullptr, g.n_z, g.n_y, outT);
+  BAND_TRY(hipGetLastError());
+  return 0;
+}
+
+int band_to_row_major(const DeviceGeometry& g, const float* inT, float* out,
+                      hipStream_t stream, std::string* err) {
+  dim3 grid((g.n_y + 31) / 32, (g.n_z + 31) / 32);
+  hipLaunchKernelGGL(to_row_major_kernel, grid, dim3(256), 0, stream, inT,
+                     g.n_z, g.n_y, out);
+  BAND_TRY(hipGetLastError());
+  return 0;
+}
+
+int band_backproject(BandPlan& plan, const DeviceGeometry& g,
+                     const float* weights, float* outT, hipStream_t stream,
+                     std::string* err) {
+  if (!plan.ready) {
+    if (err) *err = "band plan not prepared";
+    return -1;
+  }
+  return dispatch_backward(plan, g, weights, outT, stream, err);
 }
 
 int band_count(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,

@@ -123,6 +123,65 @@ int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
                       const EventSoA& ev, float* denominators,
                       hipStream_t stream, std::string* err);
 
+// Image layout changes: row-major (y * n_z + x) <-> column-major
+// (x * n_y + y).
+int band_to_column_major(const DeviceGeometry& g, const float* in, float* outT,
+                         hipStream_t stream, std::string* err);
+int band_to_row_major(const DeviceGeometry& g, const float* inT, float* out,
+                      hipStream_t stream, std::string* err);
+
+// outT (column-major, not cleared) += sum_e K_e * weights[e] over the
+// resident events (weights in record order).
+int band_backproject(BandPlan& plan, const DeviceGeometry& g,
+                     const float* weights, float* outT, hipStream_t stream,
+                     std::string* err);
+
+// ---------------------------------------------------------------------------
+// differentiation (strip_adjoint.hip)
+// ---------------------------------------------------------------------------
+
+// Device work buffers of the gradient passes.
+struct AdjointWork {
+  float* moments = nullptr;  // 6 floats per (event, forward band)
+  int64_t moments_cap = 0;
+  float* ev_buf = nullptr;   // 5 per-event arrays: w1, w2, value, gz, gd
+  int64_t ev_cap = 0;
+  float* img = nullptr;      // 3 column-major images: h, bp1, bp2
+  int64_t img_cap = 0;
+  float* sums = nullptr;     // device reduction results
+  void* tmp = nullptr;
+  size_t tmp_bytes = 0;
+};
+
+void adjoint_release(AdjointWork& wk);
+
+// Poisson log-likelihood L = sum_{e: D_e > 0} log D_e - sum_p rho(p) of the
+// column-major image rhoT, with d L / d rho into gradT (column-major, may be
+// null) and d L / d ln sigma_z, d L / d ln sigma_dl.  The support of every
+// event is held fixed (the almost-everywhere derivative).
+int band_loglik_gradient(BandPlan& plan, const DeviceGeometry& g,
+                         AdjointWork& wk, const float* rhoT, float* gradT,
+                         double* value, double grad_ln_sigma[2],
+                         hipStream_t stream, std::string* err);
+
+// One reverse step through rho_{k+1} = rho_k * sum_e K_e / D_e(rho_k):
+// gbarT (column-major d loss / d rho_{k+1}) is replaced by d loss / d rho_k
+// and the step's d loss / d ln sigma is added to grad_ln_sigma.
+int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
+                           AdjointWork& wk, const float* rhoT_k, float* gbarT,
+                           double grad_ln_sigma[2], hipStream_t stream,
+                           std::string* err);
+
+// Allocates the work buffers for the plan's event count and grid.
+int band_adjoint_prepare(BandPlan& plan, const DeviceGeometry& g,
+                         AdjointWork& wk, std::string* err);
+
+// Seed of the reverse pass of loss = mean_p (rho(p) - target(p))^2:
+// gbarT = 2 (rhoT - targetT) / n_pixels; *loss receives the loss.
+int band_mse_seed(const DeviceGeometry& g, AdjointWork& wk, const float* rhoT,
+                  const float* targetT, float* gbarT, double* loss,
+                  hipStream_t stream, std::string* err);
+
 // Kernel-evaluation count of the span walk (must equal the exact count).
 int band_count(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                unsigned long long* d_total, hipStream_t stream,

@@ -322,6 +322,12 @@ struct strip_mlem_ctx {
   int cu_count = 0;
   strip::BandPlan band;
   bool band_dirty = true;
+  // differentiation: work buffers, per-iteration image checkpoints
+  // (column-major) and row-major input/output staging
+  strip::AdjointWork adjoint;
+  float* ckpt = nullptr;
+  int64_t ckpt_images = 0;
+  float* io = nullptr;  // 2 row-major images
   char name[256] = {0};
   // timing
   bool timing = false;
@@ -549,6 +555,124 @@ int band_iterations(strip_mlem_ctx* ctx, int n_iterations) {
   return STRIP_MLEM_OK;
 }
 
+// Differentiation needs the CPU-path semantics on the row-band plan.
+int adjoint_ready(strip_mlem_ctx* ctx) {
+  if (ctx->mode.kernel != 0 || ctx->mode.sensitivity != 0 ||
+      ctx->mode.guard_zero != 1)
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "gradients are defined for the CPU-path semantics "
+                       "(Gaussian kernel, no sensitivity, guarded 1/D)");
+  if (effective_algorithm(ctx) != 2)
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "gradients need the row-band algorithm");
+  int rc = ensure_band(ctx);
+  if (rc) return rc;
+  std::string err;
+  if (strip::band_adjoint_prepare(ctx->band, ctx->g, ctx->adjoint, &err))
+    return strip::fail(STRIP_MLEM_ERR_ALLOC, err);
+  return STRIP_MLEM_OK;
+}
+
+// At least n_images column-major checkpoint images and the row-major staging.
+int ensure_images(strip_mlem_ctx* ctx, int64_t n_images) {
+  if (!ctx->io) {
+    if (hipMalloc(&ctx->io, sizeof(float) * 2 * ctx->n_pixels) != hipSuccess) {
+      ctx->io = nullptr;
+      return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate image staging");
+    }
+  }
+  if (n_images > ctx->ckpt_images) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->ckpt) HIP_TRY(hipFree(ctx->ckpt));
+    ctx->ckpt = nullptr;
+    ctx->ckpt_images = 0;
+    if (hipMalloc(&ctx->ckpt, sizeof(float) * n_images * ctx->n_pixels) !=
+        hipSuccess) {
+      ctx->ckpt = nullptr;
+      return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate checkpoints");
+    }
+    ctx->ckpt_images = n_images;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int loglik_gradient_device(strip_mlem_ctx* ctx, const float* d_rho,
+                           double* value, float* d_grad, double* grad_sigma) {
+  int rc = adjoint_ready(ctx);
+  if (rc) return rc;
+  if ((rc = ensure_images(ctx, 2))) return rc;
+  std::string err;
+  float* rhoT = ctx->ckpt;
+  float* gradT = ctx->ckpt + ctx->n_pixels;
+  double gl[2] = {0.0, 0.0};
+  if (strip::band_to_column_major(ctx->g, d_rho ? d_rho : ctx->rho[ctx->cur],
+                                  rhoT, ctx->stream, &err) ||
+      strip::band_loglik_gradient(ctx->band, ctx->g, ctx->adjoint, rhoT,
+                                  d_grad ? gradT : nullptr, value, gl,
+                                  ctx->stream, &err) ||
+      (d_grad &&
+       strip::band_to_row_major(ctx->g, gradT, d_grad, ctx->stream, &err)))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (grad_sigma) {
+    grad_sigma[0] = gl[0] / ctx->geometry.sigma_z;
+    grad_sigma[1] = gl[1] / ctx->geometry.sigma_dl;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
+                             int n_iterations, const float* d_target,
+                             double* loss, float* d_grad, double* grad_sigma) {
+  if (n_iterations < 0 || !d_target)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "n_iterations >= 0 and a target image are required");
+  int rc = adjoint_ready(ctx);
+  if (rc) return rc;
+  // checkpoints rho_0 .. rho_N, then the target and the adjoint image
+  if ((rc = ensure_images(ctx, (int64_t)n_iterations + 3))) return rc;
+  std::string err;
+  strip::BandPlan& plan = ctx->band;
+  const int64_t np = ctx->n_pixels;
+  auto ck = [&](int64_t k) { return ctx->ckpt + k * np; };
+  float* targetT = ck(n_iterations + 1);
+  float* gbarT = ck(n_iterations + 2);
+  if (strip::band_to_column_major(ctx->g, d_rho0 ? d_rho0 : ctx->rho[ctx->cur],
+                                  ck(0), ctx->stream, &err) ||
+      strip::band_to_column_major(ctx->g, d_target, targetT, ctx->stream,
+                                  &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  for (int k = 0; k < n_iterations; ++k) {
+    HIP_TRY(hipMemcpyAsync(plan.rhoT[plan.curT], ck(k), sizeof(float) * np,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    if (strip::band_iteration(plan, ctx->g, ctx->ev, 0, 1, ctx->stream,
+                              nullptr, nullptr, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    HIP_TRY(hipMemcpyAsync(ck(k + 1), plan.rhoT[plan.curT], sizeof(float) * np,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  double gl[2] = {0.0, 0.0};
+  if (strip::band_mse_seed(ctx->g, ctx->adjoint, ck(n_iterations), targetT,
+                           gbarT, loss, ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  for (int k = n_iterations - 1; k >= 0; --k)
+    if (strip::band_mlem_adjoint_step(plan, ctx->g, ctx->adjoint, ck(k),
+                                      gbarT, gl, ctx->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  // the context keeps rho_N as its image, as after n_iterations iterations
+  if ((d_grad &&
+       strip::band_to_row_major(ctx->g, gbarT, d_grad, ctx->stream, &err)) ||
+      strip::band_to_row_major(ctx->g, ck(n_iterations), ctx->rho[ctx->cur],
+                               ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (grad_sigma) {
+    grad_sigma[0] = gl[0] / ctx->geometry.sigma_z;
+    grad_sigma[1] = gl[1] / ctx->geometry.sigma_dl;
+  }
+  return STRIP_MLEM_OK;
+}
+
 int fill(strip_mlem_ctx* ctx, float* ptr, int64_t n, float value) {
   int blocks = (int)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 4096));
   hipLaunchKernelGGL(fill_kernel, dim3(blocks), dim3(256), 0, ctx->stream,
@@ -628,6 +752,9 @@ void strip_mlem_destroy(strip_mlem_ctx* ctx) {
   (void)hipFree(ctx->ev.c);
   (void)hipFree(ctx->ev.box);
   strip::band_release(ctx->band);
+  strip::adjoint_release(ctx->adjoint);
+  (void)hipFree(ctx->ckpt);
+  (void)hipFree(ctx->io);
   (void)hipFree(ctx->rho[0]);
   (void)hipFree(ctx->rho[1]);
   (void)hipFree(ctx->bp);
@@ -726,6 +853,74 @@ int strip_mlem_get_rho_device(strip_mlem_ctx* ctx, float* device_rho) {
                          sizeof(float) * ctx->n_pixels,
                          hipMemcpyDeviceToDevice, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_loglik_gradient_device(strip_mlem_ctx* ctx, const float* rho,
+                                     double* value, float* grad_rho,
+                                     double* grad_sigma) {
+  CTX_CHECK(ctx);
+  return loglik_gradient_device(ctx, rho, value, grad_rho, grad_sigma);
+}
+
+int strip_mlem_loglik_gradient(strip_mlem_ctx* ctx, const float* rho,
+                               double* value, float* grad_rho,
+                               double* grad_sigma) {
+  CTX_CHECK(ctx);
+  int rc = ensure_images(ctx, 2);
+  if (rc) return rc;
+  const size_t bytes = sizeof(float) * ctx->n_pixels;
+  if (rho)
+    HIP_TRY(hipMemcpyAsync(ctx->io, rho, bytes, hipMemcpyHostToDevice,
+                           ctx->stream));
+  if ((rc = loglik_gradient_device(ctx, rho ? ctx->io : nullptr, value,
+                                   grad_rho ? ctx->io + ctx->n_pixels : nullptr,
+                                   grad_sigma)))
+    return rc;
+  if (grad_rho) {
+    HIP_TRY(hipMemcpyAsync(grad_rho, ctx->io + ctx->n_pixels, bytes,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_unrolled_gradient_device(strip_mlem_ctx* ctx, const float* rho0,
+                                        int n_iterations, const float* target,
+                                        double* loss, float* grad_rho0,
+                                        double* grad_sigma) {
+  CTX_CHECK(ctx);
+  return unrolled_gradient_device(ctx, rho0, n_iterations, target, loss,
+                                  grad_rho0, grad_sigma);
+}
+
+int strip_mlem_unrolled_gradient(strip_mlem_ctx* ctx, const float* rho0,
+                                 int n_iterations, const float* target,
+                                 double* loss, float* grad_rho0,
+                                 double* grad_sigma) {
+  CTX_CHECK(ctx);
+  if (!target) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "target is NULL");
+  int rc = ensure_images(ctx, 2);
+  if (rc) return rc;
+  const size_t bytes = sizeof(float) * ctx->n_pixels;
+  float* d_rho0 = ctx->io;
+  float* d_target = ctx->io + ctx->n_pixels;
+  if (rho0)
+    HIP_TRY(hipMemcpyAsync(d_rho0, rho0, bytes, hipMemcpyHostToDevice,
+                           ctx->stream));
+  HIP_TRY(hipMemcpyAsync(d_target, target, bytes, hipMemcpyHostToDevice,
+                         ctx->stream));
+  // the gradient reuses the first staging image once rho0 has been read
+  if ((rc = unrolled_gradient_device(ctx, rho0 ? d_rho0 : nullptr,
+                                     n_iterations, d_target, loss,
+                                     grad_rho0 ? d_rho0 : nullptr,
+                                     grad_sigma)))
+    return rc;
+  if (grad_rho0) {
+    HIP_TRY(hipMemcpyAsync(grad_rho0, d_rho0, bytes, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
   return STRIP_MLEM_OK;
 }
 

@@ -100,6 +100,7 @@ def load_library():
     P = ctypes.c_void_p
     i64 = ctypes.c_int64
     fptr = ctypes.POINTER(ctypes.c_float)
+    dptr = ctypes.POINTER(ctypes.c_double)
     sigs = {
         "strip_mlem_create": (ctypes.c_int, [ctypes.POINTER(Geometry),
                                              ctypes.POINTER(Mode), ctypes.c_int,
@@ -122,6 +123,12 @@ def load_library():
         "strip_mlem_set_band_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 5),
         "strip_mlem_set_exp_mode": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_set_span_table": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_loglik_gradient": (ctypes.c_int, [P, fptr, dptr, fptr, dptr]),
+        "strip_mlem_loglik_gradient_device": (ctypes.c_int, [P, P, dptr, P, dptr]),
+        "strip_mlem_unrolled_gradient": (ctypes.c_int, [P, fptr, ctypes.c_int, fptr,
+                                                        dptr, fptr, dptr]),
+        "strip_mlem_unrolled_gradient_device": (ctypes.c_int, [P, P, ctypes.c_int, P,
+                                                               dptr, P, dptr]),
         "strip_mlem_get_span_table": (ctypes.c_int,
                                       [P, ctypes.POINTER(ctypes.c_int)]),
         "strip_mlem_get_band_tiling": (ctypes.c_int, [
@@ -170,7 +177,9 @@ def exported_symbols():
         "strip_mlem_set_algorithm", "strip_mlem_get_algorithm",
         "strip_mlem_set_band_tiling", "strip_mlem_get_band_tiling",
         "strip_mlem_set_exp_mode", "strip_mlem_set_span_table",
-        "strip_mlem_get_span_table",
+        "strip_mlem_get_span_table", "strip_mlem_loglik_gradient",
+        "strip_mlem_loglik_gradient_device", "strip_mlem_unrolled_gradient",
+        "strip_mlem_unrolled_gradient_device",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
@@ -327,6 +336,65 @@ class Reconstruction:
         out = np.empty(self.n_events, dtype=np.float32)
         _check(self._lib.strip_mlem_forward(self._ctx, _fptr(out)))
         return out
+
+    # ---- differentiation -------------------------------------------------
+
+    def loglik_gradient(self, rho=None, want_grad_rho=True):
+        """Poisson log-likelihood sum_{D>0} log D_e - sum rho of `rho`
+        (default: the current image) with d/d rho and d/d (sigma_z, sigma_dl),
+        the event supports held fixed.  Returns (value, grad_rho, grad_sigma).
+        """
+        value = ctypes.c_double()
+        gs = (ctypes.c_double * 2)()
+        r = None
+        if rho is not None:
+            r = np.ascontiguousarray(rho, dtype=np.float32).reshape(self.shape)
+        g = np.empty(self.shape, np.float32) if want_grad_rho else None
+        _check(self._lib.strip_mlem_loglik_gradient(
+            self._ctx, _fptr(r) if r is not None else None,
+            ctypes.byref(value), _fptr(g) if g is not None else None, gs))
+        return value.value, g, np.array(list(gs))
+
+    def unrolled_gradient(self, rho0, iterations, target, want_grad_rho=True):
+        """Loss mean((rho_N - target)^2) after `iterations` MLEM updates from
+        rho0 (None: the current image) and its gradients with respect to rho0
+        and (sigma_z, sigma_dl).  Returns (loss, grad_rho0, grad_sigma); the
+        image becomes rho_N."""
+        loss = ctypes.c_double()
+        gs = (ctypes.c_double * 2)()
+        r = None
+        if rho0 is not None:
+            r = np.ascontiguousarray(rho0, dtype=np.float32).reshape(self.shape)
+        t = np.ascontiguousarray(target, dtype=np.float32).reshape(self.shape)
+        g = np.empty(self.shape, np.float32) if want_grad_rho else None
+        _check(self._lib.strip_mlem_unrolled_gradient(
+            self._ctx, _fptr(r) if r is not None else None, int(iterations),
+            _fptr(t), ctypes.byref(loss), _fptr(g) if g is not None else None,
+            gs))
+        return loss.value, g, np.array(list(gs))
+
+    def loglik_gradient_device(self, rho_ptr, grad_ptr):
+        """Device-pointer form (row-major float32 images on this context's
+        device; rho_ptr 0 = the current image, grad_ptr 0 = no image
+        gradient).  Returns (value, grad_sigma)."""
+        value = ctypes.c_double()
+        gs = (ctypes.c_double * 2)()
+        _check(self._lib.strip_mlem_loglik_gradient_device(
+            self._ctx, ctypes.c_void_p(rho_ptr or None), ctypes.byref(value),
+            ctypes.c_void_p(grad_ptr or None), gs))
+        return value.value, np.array(list(gs))
+
+    def unrolled_gradient_device(self, rho0_ptr, iterations, target_ptr,
+                                 grad_ptr):
+        """Device-pointer form of unrolled_gradient.  Returns
+        (loss, grad_sigma)."""
+        loss = ctypes.c_double()
+        gs = (ctypes.c_double * 2)()
+        _check(self._lib.strip_mlem_unrolled_gradient_device(
+            self._ctx, ctypes.c_void_p(rho0_ptr or None), int(iterations),
+            ctypes.c_void_p(target_ptr), ctypes.byref(loss),
+            ctypes.c_void_p(grad_ptr or None), gs))
+        return loss.value, np.array(list(gs))
 
     def set_timing(self, enable: bool):
         _check(self._lib.strip_mlem_set_timing(self._ctx, int(bool(enable))))
