@@ -102,6 +102,45 @@ def adjoint_timing(pystrip, phantom, name, sigma, iterations):
     }
 
 
+def batched_timing(pystrip, phantom, n_members, iterations):
+    """Batched form against the plain form on the same members: n_members
+    reconstructions with uneven event counts and grids, `iterations` MLEM
+    iterations each, after the autotuner has chosen the tiling."""
+    members = phantom.batched_workload(n_members, seed=0)
+    b = pystrip.Batch(device=0)
+    for w in members:
+        b.add(w.geometry(), w.events())
+    b.iterate(1)
+    tuned_ms = b.autotune()
+    t = time.perf_counter()
+    b.iterate(iterations)
+    batch_s = time.perf_counter() - t
+    views = [b.member(i) for i in range(len(members))]
+    for m in views:
+        m.iterate(1)
+        m.synchronize()
+    t = time.perf_counter()
+    for m in views:
+        m.iterate(iterations)
+    plain_s = time.perf_counter() - t
+    n_events = sum(w.n_events for w in members)
+    out = {
+        "members": len(members),
+        "events": n_events,
+        "grids": sorted({f"{w.n_z}x{w.n_y}@{w.pixel:g}mm" for w in members}),
+        "iterations": iterations,
+        "tiling": list(b.tiling()),
+        "autotuned_iteration_ms": tuned_ms,
+        "batched_ms_per_iteration": batch_s * 1e3 / iterations,
+        "plain_ms_per_iteration": plain_s * 1e3 / iterations,
+        "batched_event_updates_per_s": n_events * iterations / batch_s,
+        "plain_event_updates_per_s": n_events * iterations / plain_s,
+        "speedup_over_plain": plain_s / batch_s,
+    }
+    b.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,6 +154,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--adjoint-workload", default="strip-512x256-1mm-5e6")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--batch-members", type=int, default=64)
+    ap.add_argument("--no-batch", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -132,12 +173,6 @@ def main():
     events = w.events(n_events)
     t_gen = time.perf_counter() - t_gen
 
-    try:
-        import torch
-        have_torch = torch.cuda.is_available()
-    except Exception:
-        have_torch = False
-
     rec = pystrip.Reconstruction(w.geometry(), pystrip.MODE_CPU, device=0)
     if args.algorithm:
         rec.set_algorithm(args.algorithm)
@@ -146,16 +181,14 @@ def main():
     t_up = time.perf_counter() - t_up
     n_evals = rec.kernel_evaluations()
 
+    # the context's stream carries all device work: synchronising it brackets
+    # the timed iterations
     rec.iterate(args.warmup)
     rec.synchronize()
-    if have_torch:
-        torch.cuda.synchronize()
     rec.set_timing(True)
     t0 = time.perf_counter()
     rec.iterate_async(args.steps)
     rec.synchronize()
-    if have_torch:
-        torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     fwd_ms, bwd_ms, iter_ms, n_timed = rec.get_timing_detail()
     rec.set_timing(False)
@@ -234,6 +267,10 @@ def main():
         line["adjoint"] = adjoint_timing(pystrip, phantom,
                                          args.adjoint_workload, (14.0, 60.0),
                                          8)
+    if not args.no_batch:
+        rec.close()
+        line["batched"] = batched_timing(pystrip, phantom, args.batch_members,
+                                         10)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, threads)

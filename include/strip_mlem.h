@@ -213,6 +213,48 @@ int strip_mlem_unrolled_gradient_device(strip_mlem_ctx* ctx, const float* rho0,
                                         double* loss, float* grad_rho0,
                                         double* grad_sigma);
 
+/* ---- batched reconstructions (no counterpart in the reference, which runs
+ * one reconstruction per process).  Many independent event lists, each with
+ * its own geometry and grid, iterated together: every pass of an iteration is
+ * one launch over all members (persistent workgroups over the merged task
+ * list) and the iterations are recorded once into a HIP graph and replayed.
+ * Members use the reference CPU semantics and start at rho = 1. ---- */
+typedef struct strip_mlem_batch strip_mlem_batch;
+
+int strip_mlem_batch_create(int device, strip_mlem_batch** out);
+void strip_mlem_batch_destroy(strip_mlem_batch* batch);
+/* Adds a reconstruction (geometry + host AoS events); *index = its member
+ * number (may be NULL). */
+int strip_mlem_batch_add(strip_mlem_batch* batch,
+                         const strip_mlem_geometry* geometry,
+                         const float* responses, int64_t n_responses,
+                         int* index);
+int strip_mlem_batch_size(const strip_mlem_batch* batch);
+/* The member's own context (owned by the batch): per-member calls such as
+ * strip_mlem_iterate run it alone, the plain single-reconstruction form. */
+int strip_mlem_batch_context(strip_mlem_batch* batch, int index,
+                             strip_mlem_ctx** ctx);
+/* n_iterations MLEM iterations of every member. */
+int strip_mlem_batch_iterate(strip_mlem_batch* batch, int n_iterations);
+int strip_mlem_batch_set_rho(strip_mlem_batch* batch, int index,
+                             const float* rho);
+int strip_mlem_batch_get_rho(strip_mlem_batch* batch, int index, float* rho);
+/* Shared tiling of the members' plans (as strip_mlem_set_band_tiling) and
+ * persistent workgroups per compute unit; tiling[6] reads them back. */
+int strip_mlem_batch_set_tiling(strip_mlem_batch* batch, int forward_rows,
+                                int backward_rows, int bucket_columns,
+                                int forward_pairs, int backward_pairs,
+                                int grid_factor);
+int strip_mlem_batch_get_tiling(const strip_mlem_batch* batch, int* tiling);
+/* 1 (default) = replay graph-recorded iterations, 0 = launch each. */
+int strip_mlem_batch_set_graph(strip_mlem_batch* batch, int enable);
+/* Device time (ms) of the last strip_mlem_batch_iterate call. */
+int strip_mlem_batch_last_ms(const strip_mlem_batch* batch, double* ms);
+/* Autotuner: coordinate search over the tiling knobs, each candidate timed on
+ * batch iterations; keeps the fastest and restores the members' images.
+ * *best_ms (may be NULL) = time of one iteration with the chosen tiling. */
+int strip_mlem_batch_autotune(strip_mlem_batch* batch, double* best_ms);
+
 /* Timing of the hot kernel(s) with HIP events on the context stream.
  * After strip_mlem_set_timing(ctx, 1), every iteration records the start and
  * end of its event-loop kernel(s); strip_mlem_get_timing returns the summed

@@ -4,8 +4,11 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "strip_band.hpp"
@@ -214,19 +217,34 @@ __global__ __launch_bounds__(256) void band_inverse_kernel(
 
 template <int H, int MODE, bool REC, bool SPANS>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
-    DeviceGeometry g, const BandRecord* __restrict__ rec,
-    const int* __restrict__ pair_event, const int4* __restrict__ tasks,
-    const uint16_t* __restrict__ spans, int span_stride,
-    const float* __restrict__ rhoT, int Wt,
-    float* __restrict__ partial, int max_bands,
-    unsigned long long* __restrict__ count) {
+    const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
+    int n_tasks, unsigned long long* __restrict__ count) {
   extern __shared__ float win[];
-  const int4 task = tasks[blockIdx.x];
-  const int b = task.x;
+  constexpr int LR = 64 / H;  // lanes per image row
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (H - 1);
+  const int q = lane / H;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned long long counted = 0;
+  // tasks in grid strides: a launch smaller than the task list keeps its
+  // workgroups resident and walks the list
+  for (int t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+  const int4 task = tasks[t];
+  const int b = task.x & 0xffff;
+  const BandInstance& d = desc[task.x >> 16];
+  const DeviceGeometry g = d.g;
+  const BandRecord* __restrict__ rec = d.rec;
+  const int* __restrict__ pair_event = d.fwd_pairs;
+  const uint16_t* __restrict__ spans = d.spans;
+  const int span_stride = d.span_stride;
+  float* __restrict__ partial = d.partial;
+  const int max_bands = d.fwd_max_bands;
   const int c0 = task.w;
   const int y_base = b * H;
   if (MODE == 0) {
-    const int n_words = Wt * H;
+    const float* __restrict__ rhoT = d.rhoT;
+    const int n_words = d.Wt * H;
+    __syncthreads();  // the previous task's window is no longer read
     for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
       const int col = c0 + i / H, iy = y_base + (i & (H - 1));
       win[i] = (col < g.n_z && iy < g.n_y)
@@ -235,12 +253,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     }
     __syncthreads();
   }
-  constexpr int LR = 64 / H;  // lanes per image row
-  const int lane = threadIdx.x & 63;
-  const int r = lane & (H - 1);
-  const int q = lane / H;
   const float step = (float)LR * g.pixel_width;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int total = task.z - task.y;
   const int share = (total + kFwdWaves - 1) / kFwdWaves;
   const int begin = task.y + wave * share;
@@ -248,7 +261,6 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   const float qc = g.C * kExp2Scale;
   const float two_qc_step = 2.0f * qc * step;
   const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
-  unsigned long long counted = 0;
   PairBatch next = gather_batch<false>(rec, pair_event, nullptr, begin, end,
                                        lane);
   for (int base = begin; base < end; base += 64) {
@@ -314,9 +326,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
                              acc);
         dz += step;
       }
-      const float d = wave_total(acc * s.pre);
-      if (lane == 0) partial[(int64_t)v.e * max_bands + (b - v.y0 / H)] = d;
+      const float dsum = wave_total(acc * s.pre);
+      if (lane == 0)
+        partial[(int64_t)v.e * max_bands + (b - v.y0 / H)] = dsum;
     }
+  }
   }
   if (MODE == 1 && lane == 0 && counted) atomicAdd(count, counted);
 }
@@ -329,20 +343,28 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
 // column-major image.
 template <int H, bool REC, bool SPANS>
 __global__ __launch_bounds__(256) void band_backward_kernel(
-    DeviceGeometry g, const BandRecord* __restrict__ rec,
-    const int* __restrict__ pair_event, const int4* __restrict__ tasks,
-    int n_tasks, const uint16_t* __restrict__ spans, int span_stride,
-    const float* __restrict__ inv, int Wt, float* __restrict__ bpT) {
+    const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
+    int n_tasks, int window_words) {
   extern __shared__ float windows[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n_waves = blockDim.x >> 6;
-  const int t = blockIdx.x * n_waves + wave;
-  if (t >= n_tasks) return;
-  float* win = windows + wave * Wt * H;
-  for (int i = lane; i < Wt * H; i += 64) win[i] = 0.0f;
+  float* win = windows + wave * window_words;
+  // tasks in grid strides of the launch's wavefronts
+  for (int t = blockIdx.x * n_waves + wave; t < n_tasks;
+       t += gridDim.x * n_waves) {
   const int4 task = tasks[t];
-  const int b = task.x;
+  const int b = task.x & 0xffff;
+  const BandInstance& d = desc[task.x >> 16];
+  const DeviceGeometry g = d.g;
+  const BandRecord* __restrict__ rec = d.rec;
+  const int* __restrict__ pair_event = d.bwd_pairs;
+  const uint16_t* __restrict__ spans = d.spans;
+  const int span_stride = d.span_stride;
+  const float* __restrict__ inv = d.inv;
+  float* __restrict__ bpT = d.bpT;
+  const int Wt = d.Wt;
+  for (int i = lane; i < Wt * H; i += 64) win[i] = 0.0f;
   const int c0 = task.w;
   const int y_base = b * H;
   constexpr int LR = 64 / H;  // lanes per image row
@@ -442,76 +464,125 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
     if (val != 0.0f && iy < g.n_y && col < g.n_z)
       unsafeAtomicAdd(&bpT[(int64_t)col * g.n_y + iy], val);
   }
+  }
 }
 
+// Raises a kernel's dynamic LDS limit only when a launch needs more than
+// was granted before, so launches recorded into a graph make no attribute
+// calls once the sizes have been seen.
 template <typename Kern>
 int set_lds(Kern kernel, size_t bytes, std::string* err) {
-  BAND_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                               hipFuncAttributeMaxDynamicSharedMemorySize,
+  static std::mutex lock;
+  static std::vector<std::pair<const void*, size_t>> granted;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> guard(lock);
+  size_t* have = nullptr;
+  for (auto& kv : granted)
+    if (kv.first == key) have = &kv.second;
+  if (have && bytes <= *have) return 0;
+  BAND_TRY(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)bytes));
+  if (have)
+    *have = bytes;
+  else
+    granted.emplace_back(key, bytes);
   return 0;
 }
 
+// Per instance (blockIdx.y): D_e = sum of the record's band partials,
+// inv_e = D>0 ? 1/D : 0 (the reference's guard).
+__global__ __launch_bounds__(256) void batch_inverse_kernel(
+    const BandInstance* __restrict__ desc, int H) {
+  const BandInstance& d = desc[blockIdx.y];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= d.n_events) return;
+  int x0, x1, y0, y1;
+  record_box(d.rec[e], x0, x1, y0, y1);
+  float sum = 0.0f;
+  if (x1 > x0 && y1 > y0) {
+    const int nb = (y1 - 1) / H - y0 / H + 1;
+    for (int k = 0; k < nb; ++k) sum += d.partial[e * d.fwd_max_bands + k];
+  }
+  d.inv[e] = (sum > 0.0f) ? 1.0f / sum : 0.0f;
+}
+
+// Per instance (blockIdx.y): rho_next = rho * bp, bp = 0.
+__global__ __launch_bounds__(256) void batch_finalize_kernel(
+    const BandInstance* __restrict__ desc) {
+  const BandInstance& d = desc[blockIdx.y];
+  const int64_t n = (int64_t)d.g.n_z * d.g.n_y;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    d.rho_next[i] = d.rhoT[i] * d.bpT[i];
+    d.bpT[i] = 0.0f;
+  }
+}
+
 template <int H>
-int launch_forward(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
-                   int mode, unsigned long long* count, hipStream_t stream,
-                   std::string* err) {
-  const PairList& L = plan.fwd;
-  if (L.n_tasks == 0) return 0;
-  const bool sp = plan.spans != nullptr;
+int launch_forward_h(const BandInstance* desc, const int4* tasks, int n_tasks,
+                     bool sp, bool recurrence, int mode, size_t lds, int grid,
+                     unsigned long long* count, hipStream_t stream,
+                     std::string* err) {
+  if (n_tasks == 0) return 0;
   if (mode == 0) {
-    auto kern =
-        plan.tuning.recurrence
-            ? (sp ? band_forward_kernel<H, 0, true, true>
-                  : band_forward_kernel<H, 0, true, false>)
-            : (sp ? band_forward_kernel<H, 0, false, true>
-                  : band_forward_kernel<H, 0, false, false>);
-    if (set_lds(kern, plan.fwd_lds, err)) return -1;
-    hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves),
-                       plan.fwd_lds, stream, g, plan.records, L.pair_event,
-                       L.tasks, plan.spans, plan.span_stride,
-                       plan.rhoT[plan.curT], plan.Wt, plan.partial,
-                       L.max_bands, count);
+    auto kern = recurrence ? (sp ? band_forward_kernel<H, 0, true, true>
+                                 : band_forward_kernel<H, 0, true, false>)
+                           : (sp ? band_forward_kernel<H, 0, false, true>
+                                 : band_forward_kernel<H, 0, false, false>);
+    if (set_lds(kern, lds, err)) return -1;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kFwdWaves), lds, stream,
+                       desc, tasks, n_tasks, count);
   } else {
     auto kern = sp ? band_forward_kernel<H, 1, false, true>
                    : band_forward_kernel<H, 1, false, false>;
-    hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves), 0,
-                       stream, g, plan.records, L.pair_event, L.tasks,
-                       plan.spans, plan.span_stride, plan.rhoT[plan.curT],
-                       plan.Wt, plan.partial, L.max_bands, count);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kFwdWaves), 0, stream,
+                       desc, tasks, n_tasks, count);
   }
   BAND_TRY(hipGetLastError());
   return 0;
 }
 
 template <int H>
-int launch_backward(BandPlan& plan, const DeviceGeometry& g,
-                    const float* weights, float* outT, hipStream_t stream,
-                    std::string* err) {
-  const PairList& L = plan.bwd;
-  if (L.n_tasks == 0) return 0;
-  const bool sp = plan.spans != nullptr;
-  auto kern = plan.tuning.recurrence
-                  ? (sp ? band_backward_kernel<H, true, true>
-                        : band_backward_kernel<H, true, false>)
-                  : (sp ? band_backward_kernel<H, false, true>
-                        : band_backward_kernel<H, false, false>);
-  if (set_lds(kern, plan.bwd_lds, err)) return -1;
-  const int blocks = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
-  hipLaunchKernelGGL(kern, dim3(blocks), dim3(64 * plan.bwd_waves),
-                     plan.bwd_lds, stream, g, plan.records, L.pair_event,
-                     L.tasks, L.n_tasks, plan.spans, plan.span_stride,
-                     weights, plan.Wt, outT);
+int launch_backward_h(const BandInstance* desc, const int4* tasks,
+                      int n_tasks, bool sp, bool recurrence, int window_words,
+                      int waves, int grid, hipStream_t stream,
+                      std::string* err) {
+  if (n_tasks == 0) return 0;
+  auto kern = recurrence ? (sp ? band_backward_kernel<H, true, true>
+                               : band_backward_kernel<H, true, false>)
+                         : (sp ? band_backward_kernel<H, false, true>
+                               : band_backward_kernel<H, false, false>);
+  const size_t lds = sizeof(float) * (size_t)window_words * waves;
+  if (set_lds(kern, lds, err)) return -1;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), lds, stream, desc,
+                     tasks, n_tasks, window_words);
   BAND_TRY(hipGetLastError());
   return 0;
 }
 
-int dispatch_forward(BandPlan& plan, const DeviceGeometry& g,
-                     const EventSoA& ev, int mode, unsigned long long* count,
-                     hipStream_t stream, std::string* err) {
-  switch (plan.fwd.H) {
-    case 32: return launch_forward<32>(plan, g, ev, mode, count, stream, err);
-    case 64: return launch_forward<64>(plan, g, ev, mode, count, stream, err);
+int upload_slot(BandPlan& plan, int slot, hipStream_t stream,
+                std::string* err) {
+  BAND_TRY(hipMemcpyAsync(plan.desc + slot, &plan.desc_host[slot],
+                          sizeof(BandInstance), hipMemcpyHostToDevice,
+                          stream));
+  BAND_TRY(hipStreamSynchronize(stream));
+  return 0;
+}
+
+int dispatch_forward(BandPlan& plan, const DeviceGeometry& g, int mode,
+                     unsigned long long* count, hipStream_t stream,
+                     std::string* err) {
+  const PairList& L = plan.fwd;
+  const bool sp = plan.spans != nullptr;
+  const bool rec = plan.tuning.recurrence != 0;
+  const BandInstance* desc = plan.desc + plan.curT;
+  switch (L.H) {
+    case 32:
+      return launch_forward_h<32>(desc, L.tasks, L.n_tasks, sp, rec, mode,
+                                  plan.fwd_lds, L.n_tasks, count, stream, err);
+    case 64:
+      return launch_forward_h<64>(desc, L.tasks, L.n_tasks, sp, rec, mode,
+                                  plan.fwd_lds, L.n_tasks, count, stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -520,9 +591,35 @@ int dispatch_forward(BandPlan& plan, const DeviceGeometry& g,
 int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
                       const float* weights, float* outT, hipStream_t stream,
                       std::string* err) {
-  switch (plan.bwd.H) {
-    case 32: return launch_backward<32>(plan, g, weights, outT, stream, err);
-    case 64: return launch_backward<64>(plan, g, weights, outT, stream, err);
+  const PairList& L = plan.bwd;
+  int slot = plan.curT;
+  if (weights != plan.inv || outT != plan.bpT) {
+    slot = -1;
+    for (int k = 2; k < kDescSlots; ++k)
+      if (plan.desc_host[k].inv == weights && plan.desc_host[k].bpT == outT &&
+          plan.desc_host[k].rec == plan.records)
+        slot = k;
+    if (slot < 0) {
+      slot = 2 + (plan.n_custom++ % (kDescSlots - 2));
+      plan.desc_host[slot] = band_descriptor(plan, g, plan.curT);
+      plan.desc_host[slot].inv = const_cast<float*>(weights);
+      plan.desc_host[slot].bpT = outT;
+      if (upload_slot(plan, slot, stream, err)) return -1;
+    }
+  }
+  const bool sp = plan.spans != nullptr;
+  const bool rec = plan.tuning.recurrence != 0;
+  const int window_words = plan.Wt * L.H;
+  const int grid = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
+  switch (L.H) {
+    case 32:
+      return launch_backward_h<32>(plan.desc + slot, L.tasks, L.n_tasks, sp,
+                                   rec, window_words, plan.bwd_waves, grid,
+                                   stream, err);
+    case 64:
+      return launch_backward_h<64>(plan.desc + slot, L.tasks, L.n_tasks, sp,
+                                   rec, window_words, plan.bwd_waves, grid,
+                                   stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -663,6 +760,7 @@ int build_tasks(PairList& L, int shift, const std::vector<int>& first,
       tasks.push_back(make_int4(band, p, std::min(end[k], p + chunk), c0));
   }
   L.n_tasks = (int)tasks.size();
+  L.tasks_host = tasks;
   if (!tasks.empty()) {
     BAND_TRY(hipMalloc(&L.tasks, sizeof(int4) * tasks.size()));
     BAND_TRY(hipMemcpyAsync(L.tasks, tasks.data(), sizeof(int4) * tasks.size(),
@@ -691,6 +789,7 @@ void band_release(BandPlan& plan) {
   free_ptr(plan.rhoT[1]);
   free_ptr(plan.bpT);
   free_ptr(plan.sensT);
+  free_ptr(plan.desc);
   BandTuning keep = plan.tuning;
   plan = BandPlan{};
   plan.tuning = keep;
@@ -840,9 +939,94 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                        sizeof(float) * n * plan.fwd.max_bands));
     BAND_TRY(hipMalloc(&plan.inv, sizeof(float) * n));
   }
+  // descriptors of the two iteration images; cached custom back-projection
+  // descriptors refer to the old records and are dropped
+  if (!plan.desc)
+    BAND_TRY(hipMalloc(&plan.desc, sizeof(BandInstance) * kDescSlots));
+  for (int k = 0; k < kDescSlots; ++k) plan.desc_host[k] = BandInstance{};
+  plan.n_custom = 0;
+  static std::atomic<int64_t> generations{0};
+  plan.generation = ++generations;
+  for (int c = 0; c < 2; ++c) plan.desc_host[c] = band_descriptor(plan, g, c);
+  BAND_TRY(hipMemcpyAsync(plan.desc, plan.desc_host,
+                          sizeof(BandInstance) * kDescSlots,
+                          hipMemcpyHostToDevice, stream));
   BAND_TRY(hipStreamSynchronize(stream));
   plan.ready = true;
   return 0;
+}
+
+BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
+                             int c) {
+  BandInstance d{};
+  d.g = g;
+  d.rec = plan.records;
+  d.fwd_pairs = plan.fwd.pair_event;
+  d.bwd_pairs = plan.bwd.pair_event;
+  d.spans = plan.spans;
+  d.rhoT = plan.rhoT[c];
+  d.rho_next = plan.rhoT[c ^ 1];
+  d.partial = plan.partial;
+  d.inv = plan.inv;
+  d.bpT = plan.bpT;
+  d.sensT = plan.sensT;
+  d.n_events = plan.n_events;
+  d.span_stride = plan.span_stride;
+  d.Wt = plan.Wt;
+  d.fwd_max_bands = plan.fwd.max_bands;
+  return d;
+}
+
+int band_launch_forward(const BandInstance* desc, const int4* tasks,
+                        int n_tasks, int H, bool spans, bool recurrence,
+                        size_t lds, int grid, hipStream_t stream,
+                        std::string* err) {
+  switch (H) {
+    case 32:
+      return launch_forward_h<32>(desc, tasks, n_tasks, spans, recurrence, 0,
+                                  lds, grid, nullptr, stream, err);
+    case 64:
+      return launch_forward_h<64>(desc, tasks, n_tasks, spans, recurrence, 0,
+                                  lds, grid, nullptr, stream, err);
+  }
+  if (err) *err = "unsupported band height";
+  return -1;
+}
+
+int band_launch_batch_update(const BandInstance* desc, int n_instances,
+                             int64_t max_events, int64_t max_pixels, int H,
+                             bool inverse, hipStream_t stream,
+                             std::string* err) {
+  if (n_instances <= 0) return 0;
+  if (inverse) {
+    if (max_events <= 0) return 0;
+    dim3 grid((unsigned)((max_events + 255) / 256), (unsigned)n_instances);
+    hipLaunchKernelGGL(batch_inverse_kernel, grid, dim3(256), 0, stream, desc,
+                       H);
+  } else {
+    const int64_t blocks = std::min<int64_t>((max_pixels + 255) / 256, 1024);
+    dim3 grid((unsigned)std::max<int64_t>(blocks, 1), (unsigned)n_instances);
+    hipLaunchKernelGGL(batch_finalize_kernel, grid, dim3(256), 0, stream,
+                       desc);
+  }
+  BAND_TRY(hipGetLastError());
+  return 0;
+}
+
+int band_launch_backward(const BandInstance* desc, const int4* tasks,
+                         int n_tasks, int H, bool spans, bool recurrence,
+                         int window_words, int waves, int grid,
+                         hipStream_t stream, std::string* err) {
+  switch (H) {
+    case 32:
+      return launch_backward_h<32>(desc, tasks, n_tasks, spans, recurrence,
+                                   window_words, waves, grid, stream, err);
+    case 64:
+      return launch_backward_h<64>(desc, tasks, n_tasks, spans, recurrence,
+                                   window_words, waves, grid, stream, err);
+  }
+  if (err) *err = "unsupported band height";
+  return -1;
 }
 
 int band_load_images(BandPlan& plan, const DeviceGeometry& g,
@@ -890,7 +1074,7 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
     if (err) *err = "the band algorithm does not fold sensitivity into D";
     return -1;
   }
-  if (dispatch_forward(plan, g, ev, 0, nullptr, stream, err)) return -1;
+  if (dispatch_forward(plan, g, 0, nullptr, stream, err)) return -1;
   if (t_fwd) BAND_TRY(hipEventRecord(t_fwd, stream));
   if (plan.n_events > 0) {
     const unsigned blocks = (unsigned)((plan.n_events + 255) / 256);
@@ -914,7 +1098,7 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
 int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
                       const EventSoA& ev, float* denominators,
                       hipStream_t stream, std::string* err) {
-  if (dispatch_forward(plan, g, ev, 0, nullptr, stream, err)) return -1;
+  if (dispatch_forward(plan, g, 0, nullptr, stream, err)) return -1;
   if (plan.n_events > 0) {
     const unsigned blocks = (unsigned)((plan.n_events + 255) / 256);
     // D in record order into plan.inv (free between iterations), then back
@@ -962,7 +1146,7 @@ int band_backproject(BandPlan& plan, const DeviceGeometry& g,
 int band_count(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                unsigned long long* d_total, hipStream_t stream,
                std::string* err) {
-  return dispatch_forward(plan, g, ev, 1, d_total, stream, err);
+  return dispatch_forward(plan, g, 1, d_total, stream, err);
 }
 
 }  // namespace strip

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "strip_device.hpp"
 
@@ -62,10 +63,37 @@ struct PairList {
   int max_bands = 0;          // most bands one event touches
   int64_t n_pairs = 0;
   int* pair_event = nullptr;  // pair -> event
-  int4* tasks = nullptr;      // (band, begin, end, first column)
+  int4* tasks = nullptr;      // (band | instance << 16, begin, end, column)
   int n_tasks = 0;
   bool owns_pairs = true;
+  std::vector<int4> tasks_host;  // host copy (instance 0)
 };
+
+// What the band kernels need of one reconstruction, read per task from a
+// device array so that one launch can serve many reconstructions (a task's
+// x field carries its instance index above 16 bits of band index).
+struct BandInstance {
+  DeviceGeometry g;
+  const BandRecord* rec;
+  const int* fwd_pairs;     // pair -> record, forward band height
+  const int* bwd_pairs;     // pair -> record, back-projection band height
+  const uint16_t* spans;    // span table or null
+  const float* rhoT;        // image the forward pass reads (column-major)
+  float* rho_next;          // image the update writes
+  float* partial;           // partial D per (record, forward band)
+  float* inv;               // back-projection weight per record
+  float* bpT;               // back-projection accumulator (column-major)
+  const float* sensT;       // sensitivity (numerator mode) or null
+  int64_t n_events;
+  int span_stride;
+  int Wt;                   // task window width (columns)
+  int fwd_max_bands;
+  int pad;
+};
+
+// descriptor slots of a plan: 0/1 = iteration with image rhoT[0]/rhoT[1],
+// the rest cache back-projections with caller-given weights and output
+constexpr int kDescSlots = 6;
 
 struct BandPlan {
   bool ready = false;
@@ -93,7 +121,37 @@ struct BandPlan {
   float* bpT = nullptr;
   float* sensT = nullptr;
   size_t fwd_lds = 0, bwd_lds = 0;
+  int64_t generation = 0;  // changes at every band_prepare
+  BandInstance* desc = nullptr;  // device, kDescSlots
+  BandInstance desc_host[kDescSlots] = {};
+  int n_custom = 0;              // custom slots in use (round robin)
 };
+
+// Descriptor of a prepared plan reading image rhoT[c] (weights plan.inv, into
+// plan.bpT).
+BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
+                             int c);
+
+// Launches of the band kernels over a task list of one or many instances.
+// grid = workgroups (tasks are taken in grid strides); lds = dynamic LDS
+// bytes; backward: window words per wavefront (largest Wt x H) and
+// wavefronts per workgroup.
+int band_launch_forward(const BandInstance* desc, const int4* tasks,
+                        int n_tasks, int H, bool spans, bool recurrence,
+                        size_t lds, int grid, hipStream_t stream,
+                        std::string* err);
+int band_launch_backward(const BandInstance* desc, const int4* tasks,
+                         int n_tasks, int H, bool spans, bool recurrence,
+                         int window_words, int waves, int grid,
+                         hipStream_t stream, std::string* err);
+
+// Per-instance updates of a batch (blockIdx.y = instance): inverse = true
+// forms inv = D>0 ? 1/D : 0 from the partials, inverse = false writes
+// rho_next = rhoT * bpT and clears bpT.
+int band_launch_batch_update(const BandInstance* desc, int n_instances,
+                             int64_t max_events, int64_t max_pixels, int H,
+                             bool inverse, hipStream_t stream,
+                             std::string* err);
 
 // True when the row-band algorithm can run on this grid.
 bool band_supported_grid(const DeviceGeometry& g);

@@ -1233,3 +1233,466 @@ int strip_mlem_run(const strip_mlem_geometry* geometry,
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batched reconstructions: many independent event lists and grids iterated
+// together.  Each member keeps its own row-band plan (records, span table,
+// pair lists); the batch merges the members' task lists, tagging every task
+// with its member's index, and runs each pass of an iteration as ONE launch
+// over all members (persistent workgroups walking the merged task list).
+// Iterations are recorded once into a HIP graph and replayed.
+// ===========================================================================
+
+struct strip_mlem_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<strip_mlem_ctx*> items;
+  strip::BandTuning tuning;
+  int grid_factor = 4;  // persistent workgroups per CU
+  bool use_graph = true;
+  // merged state, valid for the members' plan generations below
+  std::vector<int64_t> generations;
+  strip::BandInstance* desc[2] = {nullptr, nullptr};  // image parity 0 / 1
+  int4* fwd_tasks = nullptr;
+  int4* bwd_tasks = nullptr;
+  int n_fwd = 0, n_bwd = 0;
+  int H_fwd = 0, H_bwd = 0;
+  int max_Wt = 0, bwd_waves = 1;
+  size_t fwd_lds = 0;
+  bool spans = true;
+  int64_t max_events = 0, max_pixels = 0;
+  int cu_count = 0;
+  int cur = 0;  // image parity of every member's plan
+  bool images_loaded = false;
+  hipGraphExec_t graph = nullptr;  // two iterations, parity 0 then 1
+  // timing of the last iterate call
+  hipEvent_t t0 = nullptr, t1 = nullptr;
+  double last_ms = 0.0;
+};
+
+namespace {
+
+#define BATCH_CHECK(b)                                                 \
+  do {                                                                 \
+    if (!(b)) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "batch is NULL"); \
+    HIP_TRY(hipSetDevice((b)->device));                                \
+  } while (0)
+
+void batch_drop_merge(strip_mlem_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  for (auto& d : b->desc) {
+    if (d) (void)hipFree(d);
+    d = nullptr;
+  }
+  if (b->fwd_tasks) (void)hipFree(b->fwd_tasks);
+  if (b->bwd_tasks) (void)hipFree(b->bwd_tasks);
+  b->fwd_tasks = b->bwd_tasks = nullptr;
+  b->n_fwd = b->n_bwd = 0;
+  b->generations.clear();
+}
+
+bool batch_current(const strip_mlem_batch* b) {
+  if (b->generations.size() != b->items.size()) return false;
+  for (size_t i = 0; i < b->items.size(); ++i)
+    if (b->items[i]->band_dirty || !b->items[i]->band.ready ||
+        b->items[i]->band.generation != b->generations[i])
+      return false;
+  return true;
+}
+
+// Member plans (same tiling everywhere) and the merged task lists.
+int batch_merge(strip_mlem_batch* b) {
+  if (batch_current(b)) return STRIP_MLEM_OK;
+  batch_drop_merge(b);
+  const int n = (int)b->items.size();
+  if (n > 32767)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "at most 32767 members");
+  std::vector<strip::BandInstance> d0(n), d1(n);
+  std::vector<int4> fwd, bwd;
+  b->max_Wt = 1;
+  b->spans = true;
+  b->max_events = b->max_pixels = 0;
+  for (int i = 0; i < n; ++i) {
+    strip_mlem_ctx* c = b->items[i];
+    if (c->band.tuning.fwd_rows != b->tuning.fwd_rows ||
+        c->band.tuning.bwd_rows != b->tuning.bwd_rows ||
+        c->band.tuning.bucket != b->tuning.bucket ||
+        c->band.tuning.fwd_pairs != b->tuning.fwd_pairs ||
+        c->band.tuning.bwd_pairs != b->tuning.bwd_pairs ||
+        c->band.tuning.spans != b->tuning.spans) {
+      c->band.tuning = b->tuning;
+      c->band_dirty = true;
+    }
+    int rc = ensure_band(c);
+    if (rc) return rc;
+    const strip::BandPlan& P = c->band;
+    if (i == 0) {
+      b->H_fwd = P.fwd.H;
+      b->H_bwd = P.bwd.H;
+    } else if (P.fwd.H != b->H_fwd || P.bwd.H != b->H_bwd) {
+      return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                         "members ended with different band heights");
+    }
+    b->max_Wt = std::max(b->max_Wt, P.Wt);
+    b->spans = b->spans && P.spans != nullptr;
+    b->max_events = std::max<int64_t>(b->max_events, P.n_events);
+    b->max_pixels = std::max<int64_t>(b->max_pixels, c->n_pixels);
+    d0[i] = strip::band_descriptor(P, c->g, 0);
+    d1[i] = strip::band_descriptor(P, c->g, 1);
+    for (int4 t : P.fwd.tasks_host) {
+      t.x |= i << 16;
+      fwd.push_back(t);
+    }
+    for (int4 t : P.bwd.tasks_host) {
+      t.x |= i << 16;
+      bwd.push_back(t);
+    }
+    b->generations.push_back(P.generation);
+  }
+  b->fwd_lds = sizeof(float) * (size_t)b->max_Wt * b->H_fwd;
+  const size_t window = sizeof(float) * (size_t)b->max_Wt * b->H_bwd;
+  if (b->fwd_lds > 160 * 1024 || window > 160 * 1024)
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "a member's task window does not fit in LDS");
+  b->bwd_waves = (int)std::max<size_t>(1, std::min<size_t>(4, 160 * 1024 / window));
+  for (int c = 0; c < 2; ++c) {
+    HIP_TRY(hipMalloc(&b->desc[c], sizeof(strip::BandInstance) * std::max(n, 1)));
+    HIP_TRY(hipMemcpy(b->desc[c], c ? d1.data() : d0.data(),
+                      sizeof(strip::BandInstance) * n, hipMemcpyHostToDevice));
+  }
+  b->n_fwd = (int)fwd.size();
+  b->n_bwd = (int)bwd.size();
+  if (b->n_fwd) {
+    HIP_TRY(hipMalloc(&b->fwd_tasks, sizeof(int4) * fwd.size()));
+    HIP_TRY(hipMemcpy(b->fwd_tasks, fwd.data(), sizeof(int4) * fwd.size(),
+                      hipMemcpyHostToDevice));
+  }
+  if (b->n_bwd) {
+    HIP_TRY(hipMalloc(&b->bwd_tasks, sizeof(int4) * bwd.size()));
+    HIP_TRY(hipMemcpy(b->bwd_tasks, bwd.data(), sizeof(int4) * bwd.size(),
+                      hipMemcpyHostToDevice));
+  }
+  b->images_loaded = false;
+  return STRIP_MLEM_OK;
+}
+
+// Every member's image into its plan's parity-0 column-major buffer.
+int batch_load_images(strip_mlem_batch* b) {
+  if (b->images_loaded) return STRIP_MLEM_OK;
+  std::string err;
+  for (strip_mlem_ctx* c : b->items) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->band.curT = 0;
+    if (strip::band_load_images(c->band, c->g, c->rho[c->cur], nullptr,
+                                b->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  }
+  b->cur = 0;
+  b->images_loaded = true;
+  return STRIP_MLEM_OK;
+}
+
+int batch_store_images(strip_mlem_batch* b) {
+  if (!b->images_loaded) return STRIP_MLEM_OK;
+  std::string err;
+  for (strip_mlem_ctx* c : b->items) {
+    c->band.curT = b->cur;
+    if (strip::band_store_image(c->band, c->g, c->rho[c->cur], b->stream,
+                                &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->images_loaded = false;
+  return STRIP_MLEM_OK;
+}
+
+// One iteration of every member with image parity c (four launches).
+int batch_launch_iteration(strip_mlem_batch* b, int c) {
+  std::string err;
+  const int n = (int)b->items.size();
+  const int persistent = std::max(1, b->cu_count * b->grid_factor);
+  const bool rec = b->tuning.recurrence != 0;
+  if (strip::band_launch_forward(b->desc[c], b->fwd_tasks, b->n_fwd, b->H_fwd,
+                                 b->spans, rec, b->fwd_lds,
+                                 std::min(b->n_fwd, persistent), b->stream,
+                                 &err) ||
+      strip::band_launch_batch_update(b->desc[c], n, b->max_events,
+                                      b->max_pixels, b->H_fwd, true,
+                                      b->stream, &err) ||
+      strip::band_launch_backward(
+          b->desc[c], b->bwd_tasks, b->n_bwd, b->H_bwd, b->spans, rec,
+          b->max_Wt * b->H_bwd, b->bwd_waves,
+          std::min((b->n_bwd + b->bwd_waves - 1) / b->bwd_waves, persistent),
+          b->stream, &err) ||
+      strip::band_launch_batch_update(b->desc[c], n, b->max_events,
+                                      b->max_pixels, b->H_fwd, false,
+                                      b->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  return STRIP_MLEM_OK;
+}
+
+int batch_iterate(strip_mlem_batch* b, int n_iterations) {
+  int rc = batch_merge(b);
+  if (rc) return rc;
+  if ((rc = batch_load_images(b))) return rc;
+  if (b->items.empty() || n_iterations <= 0) return batch_store_images(b);
+  HIP_TRY(hipEventRecord(b->t0, b->stream));
+  int done = 0;
+  if (b->use_graph) {
+    // eager iterations until the parity is 0 (and the LDS limits are set),
+    // then the two-iteration graph
+    if (!b->graph) {
+      if ((rc = batch_launch_iteration(b, b->cur))) return rc;
+      b->cur ^= 1;
+      ++done;
+      if (b->cur != 0 && done < n_iterations) {
+        if ((rc = batch_launch_iteration(b, b->cur))) return rc;
+        b->cur ^= 1;
+        ++done;
+      }
+      if (b->cur == 0 && n_iterations - done >= 2) {
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        int r0 = batch_launch_iteration(b, 0);
+        int r1 = r0 ? r0 : batch_launch_iteration(b, 1);
+        hipError_t ec = hipStreamEndCapture(b->stream, &g);
+        if (r1) {
+          if (g) (void)hipGraphDestroy(g);
+          return r1;
+        }
+        if (ec != hipSuccess) return strip::fail(STRIP_MLEM_ERR_HIP,
+                                                 "graph capture failed");
+        hipError_t ei = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+          b->graph = nullptr;
+          return strip::fail(STRIP_MLEM_ERR_HIP, "graph instantiation failed");
+        }
+      }
+    } else if (b->cur != 0) {
+      if ((rc = batch_launch_iteration(b, b->cur))) return rc;
+      b->cur ^= 1;
+      ++done;
+    }
+    while (b->graph && n_iterations - done >= 2) {
+      HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+      done += 2;
+    }
+  }
+  for (; done < n_iterations; ++done) {
+    if ((rc = batch_launch_iteration(b, b->cur))) return rc;
+    b->cur ^= 1;
+  }
+  HIP_TRY(hipEventRecord(b->t1, b->stream));
+  HIP_TRY(hipEventSynchronize(b->t1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->t0, b->t1));
+  b->last_ms = ms;
+  return batch_store_images(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int strip_mlem_batch_create(int device, strip_mlem_batch** out) {
+  strip::g_last_error.clear();
+  if (!out) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  HIP_TRY(hipSetDevice(device));
+  auto* b = new strip_mlem_batch();
+  b->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess ||
+      hipStreamCreate(&b->stream) != hipSuccess ||
+      hipEventCreate(&b->t0) != hipSuccess ||
+      hipEventCreate(&b->t1) != hipSuccess) {
+    delete b;
+    return strip::fail(STRIP_MLEM_ERR_HIP, "cannot create batch stream");
+  }
+  b->cu_count = prop.multiProcessorCount;
+  *out = b;
+  return STRIP_MLEM_OK;
+}
+
+void strip_mlem_batch_destroy(strip_mlem_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_drop_merge(b);
+  for (strip_mlem_ctx* c : b->items) strip_mlem_destroy(c);
+  if (b->t0) (void)hipEventDestroy(b->t0);
+  if (b->t1) (void)hipEventDestroy(b->t1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int strip_mlem_batch_add(strip_mlem_batch* b,
+                         const strip_mlem_geometry* geometry,
+                         const float* responses, int64_t n_responses,
+                         int* index) {
+  BATCH_CHECK(b);
+  if ((int)b->items.size() >= 32767)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "at most 32767 members");
+  strip_mlem_mode mode{0, 0, 0, 1};
+  strip_mlem_ctx* c = nullptr;
+  int rc = strip_mlem_create(geometry, &mode, b->device, &c);
+  if (rc) return rc;
+  if (!strip::band_supported_grid(c->g)) {
+    strip_mlem_destroy(c);
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED, "grid too large for a batch");
+  }
+  c->algorithm = 2;
+  c->band.tuning = b->tuning;
+  if ((rc = batch_store_images(b)) ||
+      (rc = strip_mlem_set_events(c, responses, n_responses))) {
+    strip_mlem_destroy(c);
+    return rc;
+  }
+  b->items.push_back(c);
+  if (index) *index = (int)b->items.size() - 1;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_size(const strip_mlem_batch* b) {
+  return b ? (int)b->items.size() : -1;
+}
+
+int strip_mlem_batch_context(strip_mlem_batch* b, int index,
+                             strip_mlem_ctx** ctx) {
+  BATCH_CHECK(b);
+  if (!ctx || index < 0 || index >= (int)b->items.size())
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad member index");
+  int rc = batch_store_images(b);
+  if (rc) return rc;
+  *ctx = b->items[index];
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
+                                int backward_rows, int bucket_columns,
+                                int forward_pairs, int backward_pairs,
+                                int grid_factor) {
+  BATCH_CHECK(b);
+  if ((forward_rows != 32 && forward_rows != 64) ||
+      (backward_rows != 32 && backward_rows != 64) ||
+      (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
+      forward_pairs < 1 || backward_pairs < 1 || grid_factor < 1 ||
+      grid_factor > 64)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "band rows 32/64, bucket 16/32/64, pairs >= 1, "
+                       "grid factor 1..64");
+  int rc = batch_store_images(b);
+  if (rc) return rc;
+  b->tuning.fwd_rows = forward_rows;
+  b->tuning.bwd_rows = backward_rows;
+  b->tuning.bucket = bucket_columns;
+  b->tuning.fwd_pairs = forward_pairs;
+  b->tuning.bwd_pairs = backward_pairs;
+  if (b->grid_factor != grid_factor && b->graph) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+  }
+  b->grid_factor = grid_factor;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_get_tiling(const strip_mlem_batch* b, int* tiling) {
+  if (!b || !tiling) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL");
+  tiling[0] = b->tuning.fwd_rows;
+  tiling[1] = b->tuning.bwd_rows;
+  tiling[2] = b->tuning.bucket;
+  tiling[3] = b->tuning.fwd_pairs;
+  tiling[4] = b->tuning.bwd_pairs;
+  tiling[5] = b->grid_factor;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_set_graph(strip_mlem_batch* b, int enable) {
+  BATCH_CHECK(b);
+  b->use_graph = enable != 0;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_iterate(strip_mlem_batch* b, int n_iterations) {
+  BATCH_CHECK(b);
+  if (n_iterations < 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "n_iterations < 0");
+  return batch_iterate(b, n_iterations);
+}
+
+int strip_mlem_batch_last_ms(const strip_mlem_batch* b, double* ms) {
+  if (!b || !ms) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL");
+  *ms = b->last_ms;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_set_rho(strip_mlem_batch* b, int index, const float* rho) {
+  strip_mlem_ctx* c = nullptr;
+  int rc = strip_mlem_batch_context(b, index, &c);
+  return rc ? rc : strip_mlem_set_rho(c, rho);
+}
+
+int strip_mlem_batch_get_rho(strip_mlem_batch* b, int index, float* rho) {
+  strip_mlem_ctx* c = nullptr;
+  int rc = strip_mlem_batch_context(b, index, &c);
+  return rc ? rc : strip_mlem_get_rho(c, rho);
+}
+
+// Coordinate search over the tiling knobs: every candidate is timed on one
+// batch iteration (images restored afterwards), the fastest is kept.
+int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
+  BATCH_CHECK(b);
+  if (b->items.empty()) return STRIP_MLEM_OK;
+  int rc = batch_store_images(b);
+  if (rc) return rc;
+  // keep the members' images: tuning iterations must not advance them
+  std::vector<std::vector<float>> saved(b->items.size());
+  for (size_t i = 0; i < b->items.size(); ++i) {
+    saved[i].resize(b->items[i]->n_pixels);
+    if ((rc = strip_mlem_get_rho(b->items[i], saved[i].data()))) return rc;
+  }
+  int cur[6];
+  strip_mlem_batch_get_tiling(b, cur);
+  auto time_it = [&](const int* t, double* ms) -> int {
+    int r = strip_mlem_batch_set_tiling(b, t[0], t[1], t[2], t[3], t[4], t[5]);
+    if (r) return r;
+    const bool graph = b->use_graph;
+    b->use_graph = false;
+    r = batch_iterate(b, 1);  // plans + LDS limits
+    if (!r) r = batch_iterate(b, 1);
+    b->use_graph = graph;
+    *ms = b->last_ms;
+    return r;
+  };
+  double best = 0.0;
+  if ((rc = time_it(cur, &best))) return rc;
+  const std::vector<std::vector<int>> choices = {
+      {64, 32}, {32, 64}, {16, 32, 64}, {2048, 4096, 8192}, {512, 1024, 2048},
+      {2, 4, 8}};
+  for (int k = 0; k < 6; ++k) {
+    for (int v : choices[k]) {
+      if (v == cur[k]) continue;
+      int cand[6];
+      std::copy(cur, cur + 6, cand);
+      cand[k] = v;
+      double ms = 0.0;
+      if (time_it(cand, &ms)) continue;  // a shape that does not fit
+      if (ms < best) {
+        best = ms;
+        std::copy(cand, cand + 6, cur);
+      }
+    }
+  }
+  if ((rc = strip_mlem_batch_set_tiling(b, cur[0], cur[1], cur[2], cur[3],
+                                        cur[4], cur[5])))
+    return rc;
+  for (size_t i = 0; i < b->items.size(); ++i)
+    if ((rc = strip_mlem_set_rho(b->items[i], saved[i].data()))) return rc;
+  if (best_ms) *best_ms = best;
+  strip::g_last_error.clear();
+  return STRIP_MLEM_OK;
+}
+
+}  // extern "C"

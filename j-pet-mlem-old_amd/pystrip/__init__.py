@@ -123,6 +123,20 @@ def load_library():
         "strip_mlem_set_band_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 5),
         "strip_mlem_set_exp_mode": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_set_span_table": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
+        "strip_mlem_batch_destroy": (None, [P]),
+        "strip_mlem_batch_add": (ctypes.c_int, [P, ctypes.POINTER(Geometry), fptr, i64,
+                                                ctypes.POINTER(ctypes.c_int)]),
+        "strip_mlem_batch_size": (ctypes.c_int, [P]),
+        "strip_mlem_batch_context": (ctypes.c_int, [P, ctypes.c_int, ctypes.POINTER(P)]),
+        "strip_mlem_batch_iterate": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_batch_set_rho": (ctypes.c_int, [P, ctypes.c_int, fptr]),
+        "strip_mlem_batch_get_rho": (ctypes.c_int, [P, ctypes.c_int, fptr]),
+        "strip_mlem_batch_set_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 6),
+        "strip_mlem_batch_get_tiling": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
+        "strip_mlem_batch_set_graph": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_batch_last_ms": (ctypes.c_int, [P, dptr]),
+        "strip_mlem_batch_autotune": (ctypes.c_int, [P, dptr]),
         "strip_mlem_loglik_gradient": (ctypes.c_int, [P, fptr, dptr, fptr, dptr]),
         "strip_mlem_loglik_gradient_device": (ctypes.c_int, [P, P, dptr, P, dptr]),
         "strip_mlem_unrolled_gradient": (ctypes.c_int, [P, fptr, ctypes.c_int, fptr,
@@ -179,12 +193,22 @@ def exported_symbols():
         "strip_mlem_set_exp_mode", "strip_mlem_set_span_table",
         "strip_mlem_get_span_table", "strip_mlem_loglik_gradient",
         "strip_mlem_loglik_gradient_device", "strip_mlem_unrolled_gradient",
-        "strip_mlem_unrolled_gradient_device",
+        "strip_mlem_unrolled_gradient_device", "strip_mlem_batch_create",
+        "strip_mlem_batch_destroy", "strip_mlem_batch_add",
+        "strip_mlem_batch_size", "strip_mlem_batch_context",
+        "strip_mlem_batch_iterate", "strip_mlem_batch_set_rho",
+        "strip_mlem_batch_get_rho", "strip_mlem_batch_set_tiling",
+        "strip_mlem_batch_get_tiling", "strip_mlem_batch_set_graph",
+        "strip_mlem_batch_last_ms", "strip_mlem_batch_autotune",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
         "strip_mlem_device_name", "strip_mlem_last_error", "strip_mlem_run",
     ]
+
+
+def _library_loaded() -> bool:
+    return _lib is not None
 
 
 def _check(rc):
@@ -459,3 +483,98 @@ def run(geometry: Geometry, responses, rho, n_blocks: int,
                               n_iterations_in_block, cb_out, cb_prog, None,
                               device, cuda_blocks, cuda_threads, cb_name))
     return r
+
+
+class _MemberView(Reconstruction):
+    """A batch member's own context, owned by the batch (valid while the
+    batch lives)."""
+
+    def __init__(self, lib, ctx, geometry):  # noqa: D401 - no new context
+        self._lib = lib
+        self._ctx = ctx
+        self.geometry = geometry
+        self.shape = (geometry.n_y_pixels, geometry.n_z_pixels)
+
+    def close(self):
+        self._ctx = ctypes.c_void_p()
+
+
+class Batch:
+    """Many independent reconstructions iterated together
+    (strip_mlem_batch_*): one launch per pass over all members, iterations
+    replayed from a HIP graph."""
+
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        self._b = ctypes.c_void_p()
+        _check(self._lib.strip_mlem_batch_create(device, ctypes.byref(self._b)))
+        self.geometries = []
+
+    def add(self, geometry: "Geometry", events) -> int:
+        ev = np.ascontiguousarray(events, dtype=np.float32).reshape(-1, 3)
+        idx = ctypes.c_int()
+        _check(self._lib.strip_mlem_batch_add(self._b, ctypes.byref(geometry),
+                                              _fptr(ev), ev.shape[0],
+                                              ctypes.byref(idx)))
+        self.geometries.append(geometry)
+        return idx.value
+
+    def __len__(self):
+        return int(self._lib.strip_mlem_batch_size(self._b))
+
+    def iterate(self, n: int = 1):
+        _check(self._lib.strip_mlem_batch_iterate(self._b, int(n)))
+
+    def last_ms(self) -> float:
+        v = ctypes.c_double()
+        _check(self._lib.strip_mlem_batch_last_ms(self._b, ctypes.byref(v)))
+        return v.value
+
+    def rho(self, index: int) -> np.ndarray:
+        g = self.geometries[index]
+        out = np.empty((g.n_y_pixels, g.n_z_pixels), np.float32)
+        _check(self._lib.strip_mlem_batch_get_rho(self._b, index, _fptr(out)))
+        return out
+
+    def set_rho(self, index: int, value):
+        g = self.geometries[index]
+        v = np.ascontiguousarray(value, np.float32).reshape(
+            g.n_y_pixels, g.n_z_pixels)
+        _check(self._lib.strip_mlem_batch_set_rho(self._b, index, _fptr(v)))
+
+    def member(self, index: int) -> Reconstruction:
+        """The member's own context: its iterate() is the plain form."""
+        c = ctypes.c_void_p()
+        _check(self._lib.strip_mlem_batch_context(self._b, index,
+                                                  ctypes.byref(c)))
+        return _MemberView(self._lib, c, self.geometries[index])
+
+    def set_tiling(self, forward_rows=64, backward_rows=32, bucket_columns=32,
+                   forward_pairs=4096, backward_pairs=1024, grid_factor=4):
+        _check(self._lib.strip_mlem_batch_set_tiling(
+            self._b, forward_rows, backward_rows, bucket_columns,
+            forward_pairs, backward_pairs, grid_factor))
+
+    def tiling(self):
+        t = (ctypes.c_int * 6)()
+        _check(self._lib.strip_mlem_batch_get_tiling(self._b, t))
+        return tuple(t)
+
+    def set_graph(self, enable: bool):
+        _check(self._lib.strip_mlem_batch_set_graph(self._b, int(bool(enable))))
+
+    def autotune(self) -> float:
+        v = ctypes.c_double()
+        _check(self._lib.strip_mlem_batch_autotune(self._b, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self._b:
+            self._lib.strip_mlem_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

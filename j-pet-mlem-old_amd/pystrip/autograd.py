@@ -12,6 +12,10 @@ first); torch only supplies device memory.
 The event supports (bbox and the 3-sigma ellipse) are those of the sigma the
 context was built with, so sigma gradients are the almost-everywhere
 derivatives of the pixel sums; a new sigma value builds a new context.
+
+Import this module (or torch) before anything loads the HIP library: torch
+ships its own HIP runtime under the same soname, and the library then binds
+to it, so both share one runtime and device pointers pass between them.
 """
 from __future__ import annotations
 
@@ -19,6 +23,14 @@ import numpy as np
 import torch
 
 from . import MODE_CPU, Reconstruction
+from . import _library_loaded
+
+if _library_loaded() and not torch.cuda.is_available():
+    raise ImportError(
+        "pystrip.autograd must be imported before the HIP library is loaded "
+        "(a second HIP runtime would not share torch's device memory)")
+if torch.cuda.is_available():
+    torch.cuda.init()
 
 
 class StripModel:

@@ -267,3 +267,34 @@ def random_phantom(rng, half_z, half_y, n_regions):
                                rng.uniform(0.2, 4.0)))
     regions.append(Ellipse(0, 0, outer_a, outer_b, 0, 1.0))
     return regions
+
+
+# --------------------------------------------------------------------------
+# The batched workload: many independent reconstructions
+# --------------------------------------------------------------------------
+
+BATCH_GRIDS = [(128, 128), (256, 128), (512, 256), (512, 512)]
+
+
+def batched_workload(n_instances=64, seed=0, min_events=10_000,
+                     max_events=2_000_000, event_scale=1.0):
+    """Members of the batched benchmark: grid drawn from BATCH_GRIDS, pixel
+    size uniform in [1, 2] mm (0.25 mm steps), event count log-uniform in
+    [min_events, max_events] (times event_scale), a random 2-8 ellipse
+    phantom, sigma_z 10 mm, sigma_dl 42 mm.  The strips sit 50 mm beyond the
+    grid (R = half grid height + 50 mm) and are as long as the grid is wide
+    plus 100 mm.  Returns a list of Workload."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_instances):
+        n_z, n_y = BATCH_GRIDS[rng.integers(len(BATCH_GRIDS))]
+        pixel = 1.0 + 0.25 * rng.integers(5)
+        half_z, half_y = n_z * pixel / 2, n_y * pixel / 2
+        n = int(round(math.exp(rng.uniform(math.log(min_events),
+                                           math.log(max_events)))))
+        n = max(1, int(n * event_scale))
+        ph = random_phantom(rng, half_z, half_y, int(rng.integers(2, 9)))
+        out.append(Workload(f"batch-{i}-{n_z}x{n_y}-{pixel:g}mm-{n}",
+                            half_y + 50.0, 2 * half_z + 100.0, n_z, n_y,
+                            pixel, 10.0, 42.0, ph, n, 10, seed=seed * 1000 + i))
+    return out

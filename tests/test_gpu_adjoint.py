@@ -109,41 +109,16 @@ def test_unsupported_modes(pystrip_mod, phantom_mod):
         rec.loglik_gradient()
 
 
-def test_torch_autograd(pystrip_mod, phantom_mod, oracle_mod):
-    torch = pytest.importorskip("torch")
-    if not torch.cuda.is_available():
-        pytest.skip("no device")
-    from pystrip import autograd as sag
-    print_case("torch.autograd Functions")
-    w = phantom_mod.WORKLOADS[CFG]
-    ev = w.events(3000, seed=8)
-    model = sag.StripModel(w.geometry(), ev)
-    rng = np.random.default_rng(9)
-    rho_np = rng.uniform(0.5, 1.5, size=(w.n_y, w.n_z)).astype(np.float32)
-    rho = torch.tensor(rho_np, device="cuda", requires_grad=True)
-    sigma = torch.tensor([w.sigma_z * 1.2, w.sigma_dl * 1.2],
-                         dtype=torch.float64, requires_grad=True)
-    L = sag.loglik(rho, sigma, model)
-    (2.0 * L).backward()
-    rec = model.context(sigma.detach().tolist())
-    v, g, gs = rec.loglik_gradient(rho_np)
-    assert abs(L.item() - v) <= 1e-9 * abs(v)
-    assert np.allclose(rho.grad.cpu().numpy(), 2 * g, rtol=0, atol=1e-6)
-    assert np.allclose(sigma.grad.numpy(), 2 * gs, rtol=1e-12)
-    # unrolled loss
-    rho0 = torch.tensor(rho_np, device="cuda", requires_grad=True)
-    target = torch.tensor(rng.uniform(0, 2, size=rho_np.shape),
-                          dtype=torch.float32, device="cuda",
-                          requires_grad=True)
-    sigma.grad = None
-    loss = sag.unrolled_mlem_loss(rho0, sigma, target, model, iterations=3)
-    loss.backward()
-    l2, g2, s2 = rec.unrolled_gradient(rho_np, 3, target.detach().cpu().numpy())
-    # a second run of the same computation: the back-projection adds with
-    # atomics, so the two agree to rounding, not bit for bit
-    assert abs(loss.item() - l2) <= 1e-5 * abs(l2)
-    assert oracle_mod.rel_l2(rho0.grad.cpu().numpy(), g2) <= 1e-4
-    assert np.allclose(sigma.grad.numpy(), s2, rtol=1e-4)
-    rho_n = rec.rho
-    gt = -2.0 * (rho_n - target.detach().cpu().numpy()) / rho_n.size
-    assert np.allclose(target.grad.cpu().numpy(), gt, rtol=1e-5, atol=1e-9)
+def test_torch_autograd():
+    # its own process: torch must load its HIP runtime before the library
+    # (this test process loaded the library first)
+    import os
+    import subprocess
+    import sys
+    print_case("torch.autograd Functions (tests/torch_autograd_check.py)")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                          "torch_autograd_check.py")
+    r = subprocess.run([sys.executable, script], capture_output=True,
+                       text=True, timeout=300)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
