@@ -98,6 +98,28 @@ int strip_mlem_set_events_device(strip_mlem_ctx* ctx,
                                  int64_t n_responses);
 int64_t strip_mlem_n_events(const strip_mlem_ctx* ctx);
 
+/* Streaming: strip_mlem_reserve_events sizes the device event arrays for
+ * n_events (resident events kept); strip_mlem_append_events adds host AoS
+ * events after the resident ones.  Uploads go in chunks of up to 2^23
+ * events through two pinned buffers on a copy stream, overlapped with the
+ * per-event setup of the previous chunk. */
+int strip_mlem_reserve_events(strip_mlem_ctx* ctx, int64_t n_events);
+int strip_mlem_append_events(strip_mlem_ctx* ctx, const float* responses,
+                             int64_t n_responses);
+
+/* Ordered subsets (no counterpart in the reference, whose update uses every
+ * event): n_subsets > 1 splits the resident events by index into that many
+ * contiguous ranges, and each strip_mlem_iterate iteration then applies one
+ * update per subset s:  rho' = rho * (sum_{e in s} K_e / D_e) * (N / N_s),
+ * the count ratio N / N_s applied in FP64.  1 = plain MLEM (default).  CPU-path
+ * semantics on the row-band algorithm only. */
+int strip_mlem_set_subsets(strip_mlem_ctx* ctx, int n_subsets);
+
+/* Sum of the current image's pixels, accumulated in FP64 on the device (for
+ * MLEM it equals the number of events with D_e > 0 after an iteration,
+ * reconstruction.h:290-296). */
+int strip_mlem_image_total(strip_mlem_ctx* ctx, double* total);
+
 /* Start image / current image, n_y*n_z floats row-major (index y*n_z+z,
  * src/2d/geometry/pixel.h:29).  (Replaces output_rho.copy_to_device /
  * copy_from_device, cuda/reconstruction.cu:73-74,107.) */

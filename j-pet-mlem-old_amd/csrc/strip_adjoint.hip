@@ -271,6 +271,27 @@ __global__ __launch_bounds__(256) void mse_seed_kernel(
   }
 }
 
+__global__ void zero_double_kernel(double* __restrict__ out) {
+  if (threadIdx.x == 0) *out = 0.0;
+}
+
+// *out += sum of in[0..n) in FP64 (block sums in LDS, one atomic per block)
+__global__ __launch_bounds__(256) void sum_double_kernel(
+    int64_t n, const float* __restrict__ in, double* __restrict__ out) {
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    acc += (double)in[i];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, part[0]);
+}
+
 __global__ __launch_bounds__(256) void zero_kernel(int64_t n,
                                                    float* __restrict__ out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -311,7 +332,7 @@ int ensure_work(AdjointWork& wk, const BandPlan& plan, int64_t n_pixels,
     free_dev(wk.tmp);
     wk.tmp_bytes = 0;
   }
-  if (!wk.sums) BAND_TRY(hipMalloc(&wk.sums, sizeof(float) * 4));
+  if (!wk.sums) BAND_TRY(hipMalloc(&wk.sums, sizeof(float) * 8));
   if (!wk.tmp) {
     size_t bytes = 0;
     const int64_t longest = std::max<int64_t>(n, n_pixels);
@@ -372,10 +393,10 @@ void adjoint_release(AdjointWork& wk) {
   wk = AdjointWork{};
 }
 
-int band_loglik_gradient(BandPlan& plan, const DeviceGeometry& g,
-                         AdjointWork& wk, const float* rhoT, float* gradT,
-                         double* value, double grad_ln_sigma[2],
-                         hipStream_t stream, std::string* err) {
+int band_loglik_terms(BandPlan& plan, const DeviceGeometry& g,
+                      AdjointWork& wk, const float* rhoT, float* bp_accum,
+                      double* sum_log, double grad_ln_sigma[2],
+                      hipStream_t stream, std::string* err) {
   if (!plan.ready) {
     if (err) *err = "band plan not prepared";
     return -1;
@@ -383,42 +404,78 @@ int band_loglik_gradient(BandPlan& plan, const DeviceGeometry& g,
   const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
   if (ensure_work(wk, plan, n_pixels, err)) return -1;
   const int64_t n = plan.n_events;
+  *sum_log = 0.0;
+  if (n == 0) return 0;
   float* w1 = wk.ev_buf;
   float* val = wk.ev_buf + 2 * wk.ev_cap;
   float* gz = wk.ev_buf + 3 * wk.ev_cap;
   float* gd = wk.ev_buf + 4 * wk.ev_cap;
-  float sums[4] = {0.f, 0.f, 0.f, 0.f};
-  if (n > 0) {
-    if (launch_moments<1>(plan, g, wk, rhoT, nullptr, stream, err)) return -1;
-    const unsigned blocks = (unsigned)((n + 255) / 256);
-    hipLaunchKernelGGL(adjoint_event_kernel<1>, dim3(blocks), dim3(256), 0,
-                       stream, plan.records, n, plan.fwd.H,
-                       plan.fwd.max_bands, wk.moments, w1, nullptr, val, gz,
-                       gd);
-    BAND_TRY(hipGetLastError());
-    if (reduce_sum(wk, val, n, 0, stream, err) ||
-        reduce_sum(wk, gz, n, 1, stream, err) ||
-        reduce_sum(wk, gd, n, 2, stream, err))
-      return -1;
-  }
-  if (reduce_sum(wk, rhoT, n_pixels, 3, stream, err)) return -1;
-  if (gradT) {
-    float* bp = wk.img + wk.img_cap;
-    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n_pixels)), dim3(256), 0,
-                       stream, n_pixels, bp);
-    BAND_TRY(hipGetLastError());
-    if (n > 0 && band_backproject(plan, g, w1, bp, stream, err)) return -1;
-    hipLaunchKernelGGL(minus_one_kernel, dim3(grid_for(n_pixels)), dim3(256),
-                       0, stream, n_pixels, bp, gradT);
-    BAND_TRY(hipGetLastError());
-  }
-  BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(float) * 4,
+  if (launch_moments<1>(plan, g, wk, rhoT, nullptr, stream, err)) return -1;
+  const unsigned blocks = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(adjoint_event_kernel<1>, dim3(blocks), dim3(256), 0,
+                     stream, plan.records, n, plan.fwd.H, plan.fwd.max_bands,
+                     wk.moments, w1, nullptr, val, gz, gd);
+  BAND_TRY(hipGetLastError());
+  if (reduce_sum(wk, val, n, 0, stream, err) ||
+      reduce_sum(wk, gz, n, 1, stream, err) ||
+      reduce_sum(wk, gd, n, 2, stream, err))
+    return -1;
+  if (bp_accum && band_backproject(plan, g, w1, bp_accum, stream, err))
+    return -1;
+  float sums[3] = {0.f, 0.f, 0.f};
+  BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(float) * 3,
                           hipMemcpyDeviceToHost, stream));
   BAND_TRY(hipStreamSynchronize(stream));
-  if (n == 0) sums[0] = sums[1] = sums[2] = 0.f;
-  if (value) *value = (double)sums[0] - (double)sums[3];
-  grad_ln_sigma[0] = sums[1];
-  grad_ln_sigma[1] = sums[2];
+  *sum_log = sums[0];
+  grad_ln_sigma[0] += sums[1];
+  grad_ln_sigma[1] += sums[2];
+  return 0;
+}
+
+int band_image_total(const DeviceGeometry& g, AdjointWork& wk,
+                     const float* imgT, double* total, hipStream_t stream,
+                     std::string* err) {
+  const int64_t n = (int64_t)g.n_z * g.n_y;
+  if (!wk.sums) BAND_TRY(hipMalloc(&wk.sums, sizeof(float) * 8));
+  double* acc = reinterpret_cast<double*>(wk.sums) + 2;  // bytes 16..23
+  hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(64), 0, stream, acc);
+  const unsigned blocks =
+      (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(sum_double_kernel, dim3(blocks), dim3(256), 0, stream,
+                     n, imgT, acc);
+  BAND_TRY(hipGetLastError());
+  double host = 0.0;
+  BAND_TRY(hipMemcpyAsync(&host, acc, sizeof(double), hipMemcpyDeviceToHost,
+                          stream));
+  BAND_TRY(hipStreamSynchronize(stream));
+  *total = host;
+  return 0;
+}
+
+int band_loglik_gradient(BandPlan& plan, const DeviceGeometry& g,
+                         AdjointWork& wk, const float* rhoT, float* gradT,
+                         double* value, double grad_ln_sigma[2],
+                         hipStream_t stream, std::string* err) {
+  const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
+  if (ensure_work(wk, plan, n_pixels, err)) return -1;
+  if (gradT) {
+    hipLaunchKernelGGL(zero_kernel, dim3(grid_for(n_pixels)), dim3(256), 0,
+                       stream, n_pixels, gradT);
+    BAND_TRY(hipGetLastError());
+  }
+  double sum_log = 0.0, total = 0.0;
+  grad_ln_sigma[0] = grad_ln_sigma[1] = 0.0;
+  if (band_loglik_terms(plan, g, wk, rhoT, gradT, &sum_log, grad_ln_sigma,
+                        stream, err) ||
+      band_image_total(g, wk, rhoT, &total, stream, err))
+    return -1;
+  if (gradT) {
+    hipLaunchKernelGGL(minus_one_kernel, dim3(grid_for(n_pixels)), dim3(256),
+                       0, stream, n_pixels, gradT, gradT);
+    BAND_TRY(hipGetLastError());
+  }
+  BAND_TRY(hipStreamSynchronize(stream));
+  if (value) *value = sum_log - total;
   return 0;
 }
 

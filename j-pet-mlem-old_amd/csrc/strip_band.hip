@@ -181,13 +181,17 @@ __global__ __launch_bounds__(256) void to_row_major_kernel(
 }
 
 // rhoT'(p) = rhoT(p) * bpT(p) [/ sT(p)]; bpT = 0
+// (scale != 1: the update is multiplied by scale in FP64, the count ratio of
+// an ordered-subset update)
 __global__ __launch_bounds__(256) void band_finalize_kernel(
     int64_t n, const float* __restrict__ rho, float* __restrict__ bp,
-    const float* __restrict__ sens, int divide, float* __restrict__ next) {
+    const float* __restrict__ sens, int divide, double scale,
+    float* __restrict__ next) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     float v = rho[i] * bp[i];
     if (divide) v = v / sens[i];
+    if (scale != 1.0) v = (float)((double)v * scale);
     next[i] = v;
     bp[i] = 0.0f;
   }
@@ -785,9 +789,11 @@ void band_release(BandPlan& plan) {
   free_ptr(plan.records);
   free_ptr(plan.order);
   free_ptr(plan.spans);
-  free_ptr(plan.rhoT[0]);
-  free_ptr(plan.rhoT[1]);
-  free_ptr(plan.bpT);
+  if (plan.owns_images) {
+    free_ptr(plan.rhoT[0]);
+    free_ptr(plan.rhoT[1]);
+    free_ptr(plan.bpT);
+  }
   free_ptr(plan.sensT);
   free_ptr(plan.desc);
   BandTuning keep = plan.tuning;
@@ -1062,7 +1068,7 @@ int band_store_image(BandPlan& plan, const DeviceGeometry& g, float* rho,
 int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                    const EventSoA& ev, int sensitivity_mode, int guard_zero,
                    hipStream_t stream, hipEvent_t t_fwd, hipEvent_t t_bwd,
-                   std::string* err) {
+                   std::string* err, double scale) {
   if (!plan.ready) {
     if (err) *err = "band plan not prepared";
     return -1;
@@ -1089,7 +1095,7 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
   const int blocks = (int)std::min<int64_t>((n_pixels + 255) / 256, 4096);
   hipLaunchKernelGGL(band_finalize_kernel, dim3(blocks), dim3(256), 0, stream,
                      n_pixels, rho, plan.bpT, plan.sensT,
-                     sensitivity_mode == 2 ? 1 : 0, next);
+                     sensitivity_mode == 2 ? 1 : 0, scale, next);
   BAND_TRY(hipGetLastError());
   plan.curT ^= 1;
   return 0;

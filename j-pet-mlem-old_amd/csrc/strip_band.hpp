@@ -117,6 +117,7 @@ struct BandPlan {
   float* partial = nullptr;   // n_events * fwd.max_bands
   float* inv = nullptr;       // n_events
   float* rhoT[2] = {nullptr, nullptr};  // column-major images (n_z * n_y)
+  bool owns_images = true;  // false: rhoT and bpT are shared, set by the owner
   int curT = 0;
   float* bpT = nullptr;
   float* sensT = nullptr;
@@ -171,10 +172,12 @@ int band_store_image(BandPlan& plan, const DeviceGeometry& g, float* rho,
 
 // One iteration.  t_fwd (optional) is recorded after the forward kernel,
 // t_bwd after the back-projection kernel.
+// scale multiplies the update (in FP64): 1 for MLEM, the ratio of all events
+// to the plan's events for an ordered-subset update.
 int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                    const EventSoA& ev, int sensitivity_mode, int guard_zero,
                    hipStream_t stream, hipEvent_t t_fwd, hipEvent_t t_bwd,
-                   std::string* err);
+                   std::string* err, double scale = 1.0);
 
 // Per-event forward projections of the current column-major image.
 int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
@@ -206,12 +209,25 @@ struct AdjointWork {
   int64_t ev_cap = 0;
   float* img = nullptr;      // 3 column-major images: h, bp1, bp2
   int64_t img_cap = 0;
-  float* sums = nullptr;     // device reduction results
+  float* sums = nullptr;     // device reduction results (8 floats)
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
 };
 
 void adjoint_release(AdjointWork& wk);
+
+// The plan's part of the log-likelihood of column-major image rhoT:
+// *sum_log = sum_{e: D_e > 0} log D_e, grad_ln_sigma += its d / d ln sigma,
+// and (bp_accum not null) bp_accum += sum_e K_e / D_e.
+int band_loglik_terms(BandPlan& plan, const DeviceGeometry& g,
+                      AdjointWork& wk, const float* rhoT, float* bp_accum,
+                      double* sum_log, double grad_ln_sigma[2],
+                      hipStream_t stream, std::string* err);
+
+// Sum of a column-major image in FP64 (*total) and in FP32 order-free form.
+int band_image_total(const DeviceGeometry& g, AdjointWork& wk,
+                     const float* imgT, double* total, hipStream_t stream,
+                     std::string* err);
 
 // Poisson log-likelihood L = sum_{e: D_e > 0} log D_e - sum_p rho(p) of the
 // column-major image rhoT, with d L / d rho into gradT (column-major, may be

@@ -215,6 +215,13 @@ __global__ __launch_bounds__(256) void finalize_kernel(
   }
 }
 
+__global__ __launch_bounds__(256) void add_kernel(int64_t n, float value,
+                                                  float* __restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] += value;
+}
+
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ out,
                                                    int64_t n, float value) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -325,6 +332,19 @@ struct strip_mlem_ctx {
   // differentiation: work buffers, per-iteration image checkpoints
   // (column-major) and row-major input/output staging
   strip::AdjointWork adjoint;
+  // ordered subsets: plans over contiguous event ranges sharing images
+  int n_subsets = 1;
+  std::vector<strip::BandPlan> subsets;
+  bool subsets_dirty = true;
+  float* osemT = nullptr;  // 3 column-major images: rho (2), back-projection
+  int osem_cur = 0;
+  // streamed uploads: pinned host and device staging, double-buffered
+  hipStream_t copy_stream = nullptr;
+  float* pinned[2] = {nullptr, nullptr};
+  float* dstage[2] = {nullptr, nullptr};
+  int64_t stage_cap = 0;
+  hipEvent_t copied[2] = {nullptr, nullptr};
+  hipEvent_t consumed[2] = {nullptr, nullptr};
   float* ckpt = nullptr;
   int64_t ckpt_images = 0;
   float* io = nullptr;  // 2 row-major images
@@ -418,20 +438,38 @@ int ensure_mode_sensitivity(strip_mlem_ctx* ctx) {
   return STRIP_MLEM_OK;
 }
 
-int reserve_events(strip_mlem_ctx* ctx, int64_t n) {
+// Event arrays for at least n events; the first `keep` resident events are
+// preserved when the arrays grow.
+int reserve_events(strip_mlem_ctx* ctx, int64_t n, int64_t keep = 0) {
   if (n <= ctx->cap_events) return STRIP_MLEM_OK;
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  if (ctx->ev.a) HIP_TRY(hipFree(ctx->ev.a));
-  if (ctx->ev.b) HIP_TRY(hipFree(ctx->ev.b));
-  if (ctx->ev.c) HIP_TRY(hipFree(ctx->ev.c));
-  if (ctx->ev.box) HIP_TRY(hipFree(ctx->ev.box));
-  ctx->ev = EventSoA{nullptr, nullptr, nullptr, nullptr};
-  ctx->cap_events = 0;
-  if (hipMalloc(&ctx->ev.a, sizeof(float4) * n) != hipSuccess ||
-      hipMalloc(&ctx->ev.b, sizeof(float4) * n) != hipSuccess ||
-      hipMalloc(&ctx->ev.c, sizeof(float4) * n) != hipSuccess ||
-      hipMalloc(&ctx->ev.box, sizeof(uint2) * n) != hipSuccess)
+  EventSoA next{nullptr, nullptr, nullptr, nullptr};
+  if (hipMalloc(&next.a, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&next.b, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&next.c, sizeof(float4) * n) != hipSuccess ||
+      hipMalloc(&next.box, sizeof(uint2) * n) != hipSuccess) {
+    (void)hipFree(next.a);
+    (void)hipFree(next.b);
+    (void)hipFree(next.c);
+    (void)hipFree(next.box);
     return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate event arrays");
+  }
+  if (keep > 0) {
+    HIP_TRY(hipMemcpyAsync(next.a, ctx->ev.a, sizeof(float4) * keep,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(next.b, ctx->ev.b, sizeof(float4) * keep,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(next.c, ctx->ev.c, sizeof(float4) * keep,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(next.box, ctx->ev.box, sizeof(uint2) * keep,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+  }
+  (void)hipFree(ctx->ev.a);
+  (void)hipFree(ctx->ev.b);
+  (void)hipFree(ctx->ev.c);
+  (void)hipFree(ctx->ev.box);
+  ctx->ev = next;
   ctx->cap_events = n;
   return STRIP_MLEM_OK;
 }
@@ -445,6 +483,60 @@ int run_setup(strip_mlem_ctx* ctx, const float* d_responses, int64_t offset,
                      ctx->ev.a + offset, ctx->ev.b + offset,
                      ctx->ev.c + offset, ctx->ev.box + offset);
   HIP_TRY(hipGetLastError());
+  return STRIP_MLEM_OK;
+}
+
+// Host AoS events -> resident events [offset, offset + n): chunks of up to
+// kStreamChunk events are copied into pinned memory, sent by a copy stream
+// and set up on the compute stream, two chunks in flight.
+constexpr int64_t kStreamChunk = (int64_t)1 << 23;
+
+int upload_events(strip_mlem_ctx* ctx, const float* responses, int64_t offset,
+                  int64_t n) {
+  if (n <= 0) return STRIP_MLEM_OK;
+  const int64_t chunk = std::min(n, kStreamChunk);
+  if (!ctx->copy_stream) {
+    HIP_TRY(hipStreamCreate(&ctx->copy_stream));
+    for (int i = 0; i < 2; ++i) {
+      HIP_TRY(hipEventCreateWithFlags(&ctx->copied[i], hipEventDisableTiming));
+      HIP_TRY(hipEventCreateWithFlags(&ctx->consumed[i],
+                                      hipEventDisableTiming));
+    }
+  }
+  if (chunk > ctx->stage_cap) {
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+    for (int i = 0; i < 2; ++i) {
+      if (ctx->pinned[i]) HIP_TRY(hipHostFree(ctx->pinned[i]));
+      if (ctx->dstage[i]) HIP_TRY(hipFree(ctx->dstage[i]));
+      ctx->pinned[i] = nullptr;
+      ctx->dstage[i] = nullptr;
+    }
+    ctx->stage_cap = 0;
+    for (int i = 0; i < 2; ++i)
+      if (hipHostMalloc(&ctx->pinned[i], sizeof(float) * 3 * chunk) !=
+              hipSuccess ||
+          hipMalloc(&ctx->dstage[i], sizeof(float) * 3 * chunk) != hipSuccess)
+        return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate staging");
+    ctx->stage_cap = chunk;
+  }
+  int64_t k = 0;
+  for (int64_t off = 0; off < n; off += chunk, ++k) {
+    const int64_t m = std::min(chunk, n - off);
+    const int i = (int)(k & 1);
+    // buffers i are free once chunk k-2 has been set up
+    if (k >= 2) HIP_TRY(hipEventSynchronize(ctx->consumed[i]));
+    std::memcpy(ctx->pinned[i], responses + 3 * off, sizeof(float) * 3 * m);
+    HIP_TRY(hipMemcpyAsync(ctx->dstage[i], ctx->pinned[i],
+                           sizeof(float) * 3 * m, hipMemcpyHostToDevice,
+                           ctx->copy_stream));
+    HIP_TRY(hipEventRecord(ctx->copied[i], ctx->copy_stream));
+    HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->copied[i], 0));
+    int rc = run_setup(ctx, ctx->dstage[i], offset + off, m);
+    if (rc) return rc;
+    HIP_TRY(hipEventRecord(ctx->consumed[i], ctx->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
   return STRIP_MLEM_OK;
 }
 
@@ -556,7 +648,10 @@ int band_iterations(strip_mlem_ctx* ctx, int n_iterations) {
 }
 
 // Differentiation needs the CPU-path semantics on the row-band plan.
-int adjoint_ready(strip_mlem_ctx* ctx) {
+int fill(strip_mlem_ctx* ctx, float* ptr, int64_t n, float value);
+int ensure_subsets(strip_mlem_ctx* ctx);
+
+int adjoint_ready(strip_mlem_ctx* ctx, bool full_plan = true) {
   if (ctx->mode.kernel != 0 || ctx->mode.sensitivity != 0 ||
       ctx->mode.guard_zero != 1)
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
@@ -565,6 +660,7 @@ int adjoint_ready(strip_mlem_ctx* ctx) {
   if (effective_algorithm(ctx) != 2)
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
                        "gradients need the row-band algorithm");
+  if (!full_plan) return STRIP_MLEM_OK;
   int rc = ensure_band(ctx);
   if (rc) return rc;
   std::string err;
@@ -598,7 +694,7 @@ int ensure_images(strip_mlem_ctx* ctx, int64_t n_images) {
 
 int loglik_gradient_device(strip_mlem_ctx* ctx, const float* d_rho,
                            double* value, float* d_grad, double* grad_sigma) {
-  int rc = adjoint_ready(ctx);
+  int rc = adjoint_ready(ctx, ctx->n_subsets == 1);
   if (rc) return rc;
   if ((rc = ensure_images(ctx, 2))) return rc;
   std::string err;
@@ -606,13 +702,40 @@ int loglik_gradient_device(strip_mlem_ctx* ctx, const float* d_rho,
   float* gradT = ctx->ckpt + ctx->n_pixels;
   double gl[2] = {0.0, 0.0};
   if (strip::band_to_column_major(ctx->g, d_rho ? d_rho : ctx->rho[ctx->cur],
-                                  rhoT, ctx->stream, &err) ||
-      strip::band_loglik_gradient(ctx->band, ctx->g, ctx->adjoint, rhoT,
-                                  d_grad ? gradT : nullptr, value, gl,
-                                  ctx->stream, &err) ||
-      (d_grad &&
-       strip::band_to_row_major(ctx->g, gradT, d_grad, ctx->stream, &err)))
+                                  rhoT, ctx->stream, &err))
     return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  if (ctx->n_subsets > 1) {
+    // the sum over the subsets' plans (each holds a part of the events)
+    if ((rc = ensure_subsets(ctx))) return rc;
+    if (d_grad && (rc = fill(ctx, gradT, ctx->n_pixels, 0.0f))) return rc;
+    double sum_log = 0.0, total = 0.0;
+    for (auto& p : ctx->subsets) {
+      double part = 0.0;
+      if (strip::band_adjoint_prepare(p, ctx->g, ctx->adjoint, &err) ||
+          strip::band_loglik_terms(p, ctx->g, ctx->adjoint, rhoT,
+                                   d_grad ? gradT : nullptr, &part, gl,
+                                   ctx->stream, &err))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+      sum_log += part;
+    }
+    if (strip::band_image_total(ctx->g, ctx->adjoint, rhoT, &total,
+                                ctx->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    if (value) *value = sum_log - total;
+    if (d_grad) {
+      if (strip::band_to_row_major(ctx->g, gradT, d_grad, ctx->stream, &err))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+      hipLaunchKernelGGL(add_kernel, dim3(256), dim3(256), 0, ctx->stream,
+                         (int64_t)ctx->n_pixels, -1.0f, d_grad);
+      HIP_TRY(hipGetLastError());
+    }
+  } else if (strip::band_loglik_gradient(ctx->band, ctx->g, ctx->adjoint, rhoT,
+                                         d_grad ? gradT : nullptr, value, gl,
+                                         ctx->stream, &err) ||
+             (d_grad && strip::band_to_row_major(ctx->g, gradT, d_grad,
+                                                 ctx->stream, &err))) {
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (grad_sigma) {
     grad_sigma[0] = gl[0] / ctx->geometry.sigma_z;
@@ -670,6 +793,85 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
     grad_sigma[0] = gl[0] / ctx->geometry.sigma_z;
     grad_sigma[1] = gl[1] / ctx->geometry.sigma_dl;
   }
+  return STRIP_MLEM_OK;
+}
+
+// Plans of the ordered subsets: contiguous event ranges, shared images.
+int ensure_subsets(strip_mlem_ctx* ctx) {
+  const int S = ctx->n_subsets;
+  if (!ctx->subsets_dirty && (int)ctx->subsets.size() == S) {
+    bool ready = true;
+    for (auto& p : ctx->subsets) ready = ready && p.ready;
+    if (ready) return STRIP_MLEM_OK;
+  }
+  for (auto& p : ctx->subsets) strip::band_release(p);
+  ctx->subsets.clear();
+  const int64_t np = ctx->n_pixels;
+  if (!ctx->osemT) {
+    if (hipMalloc(&ctx->osemT, sizeof(float) * 3 * np) != hipSuccess) {
+      ctx->osemT = nullptr;
+      return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate images");
+    }
+    int rc = fill(ctx, ctx->osemT + 2 * np, np, 0.0f);
+    if (rc) return rc;
+  }
+  ctx->subsets.resize(S);
+  std::string err;
+  for (int s = 0; s < S; ++s) {
+    const int64_t lo = ctx->n_events * s / S;
+    const int64_t hi = ctx->n_events * (s + 1) / S;
+    strip::BandPlan& p = ctx->subsets[s];
+    p.tuning = ctx->band.tuning;
+    p.rhoT[0] = ctx->osemT;
+    p.rhoT[1] = ctx->osemT + np;
+    p.bpT = ctx->osemT + 2 * np;
+    p.owns_images = false;
+    EventSoA view{ctx->ev.a + lo, ctx->ev.b + lo, ctx->ev.c + lo,
+                  ctx->ev.box + lo};
+    if (strip::band_prepare(p, ctx->g, view, hi - lo, ctx->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, "subset plan: " + err);
+  }
+  ctx->subsets_dirty = false;
+  return STRIP_MLEM_OK;
+}
+
+// n full OSEM iterations: every subset s updates the image with
+//   rho' = rho * (sum_{e in s} K_e / D_e) * (N / N_s),
+// the count ratio N / N_s applied in FP64.
+int osem_iterations(strip_mlem_ctx* ctx, int n_iterations) {
+  int rc = ensure_subsets(ctx);
+  if (rc) return rc;
+  std::string err;
+  const int64_t np = ctx->n_pixels;
+  if (strip::band_to_column_major(ctx->g, ctx->rho[ctx->cur], ctx->osemT,
+                                  ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  ctx->osem_cur = 0;
+  for (int it = 0; it < n_iterations; ++it) {
+    hipEvent_t marks[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (ctx->timing) {
+      if ((rc = record_marks(ctx, marks))) return rc;
+      HIP_TRY(hipEventRecord(marks[0], ctx->stream));
+    }
+    for (auto& p : ctx->subsets) {
+      if (p.n_events == 0) continue;
+      p.curT = ctx->osem_cur;
+      const double ratio = (double)ctx->n_events / (double)p.n_events;
+      if (strip::band_iteration(p, ctx->g, ctx->ev, 0, ctx->mode.guard_zero,
+                                ctx->stream, nullptr, nullptr, &err, ratio))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+      ctx->osem_cur = p.curT;
+    }
+    if (ctx->timing) {
+      // a whole pass over the subsets is reported as its forward part
+      HIP_TRY(hipEventRecord(marks[1], ctx->stream));
+      HIP_TRY(hipEventRecord(marks[2], ctx->stream));
+      HIP_TRY(hipEventRecord(marks[3], ctx->stream));
+    }
+  }
+  if (strip::band_to_row_major(ctx->g, ctx->osemT + ctx->osem_cur * np,
+                               ctx->rho[ctx->cur], ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
   return STRIP_MLEM_OK;
 }
 
@@ -752,6 +954,15 @@ void strip_mlem_destroy(strip_mlem_ctx* ctx) {
   (void)hipFree(ctx->ev.c);
   (void)hipFree(ctx->ev.box);
   strip::band_release(ctx->band);
+  for (auto& p : ctx->subsets) strip::band_release(p);
+  (void)hipFree(ctx->osemT);
+  for (int i = 0; i < 2; ++i) {
+    if (ctx->pinned[i]) (void)hipHostFree(ctx->pinned[i]);
+    (void)hipFree(ctx->dstage[i]);
+    if (ctx->copied[i]) (void)hipEventDestroy(ctx->copied[i]);
+    if (ctx->consumed[i]) (void)hipEventDestroy(ctx->consumed[i]);
+  }
+  if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
   strip::adjoint_release(ctx->adjoint);
   (void)hipFree(ctx->ckpt);
   (void)hipFree(ctx->io);
@@ -774,27 +985,37 @@ int strip_mlem_set_events(strip_mlem_ctx* ctx, const float* responses,
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad event array");
   int rc = reserve_events(ctx, n);
   if (rc) return rc;
-  // upload in chunks through one device staging buffer
-  const int64_t chunk = std::min<int64_t>(n, (int64_t)1 << 24);
-  if (chunk > ctx->staging_cap) {
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    if (ctx->staging) HIP_TRY(hipFree(ctx->staging));
-    ctx->staging = nullptr;
-    ctx->staging_cap = 0;
-    if (hipMalloc(&ctx->staging, sizeof(float) * 3 * chunk) != hipSuccess)
-      return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate staging");
-    ctx->staging_cap = chunk;
-  }
-  for (int64_t off = 0; off < n; off += chunk) {
-    const int64_t m = std::min(chunk, n - off);
-    HIP_TRY(hipMemcpyAsync(ctx->staging, responses + 3 * off,
-                           sizeof(float) * 3 * m, hipMemcpyHostToDevice,
-                           ctx->stream));
-    if ((rc = run_setup(ctx, ctx->staging, off, m))) return rc;
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-  }
-  ctx->n_events = n;
+  ctx->n_events = 0;
   ctx->band_dirty = true;
+  ctx->subsets_dirty = true;
+  if ((rc = upload_events(ctx, responses, 0, n))) return rc;
+  ctx->n_events = n;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_reserve_events(strip_mlem_ctx* ctx, int64_t n_events) {
+  CTX_CHECK(ctx);
+  if (n_events < 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "n_events < 0");
+  return reserve_events(ctx, n_events, ctx->n_events);
+}
+
+int strip_mlem_append_events(strip_mlem_ctx* ctx, const float* responses,
+                             int64_t n) {
+  CTX_CHECK(ctx);
+  if (n < 0 || (n > 0 && !responses))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad event array");
+  const int64_t have = ctx->n_events;
+  if (have + n > ctx->cap_events) {
+    const int64_t grow = std::max(have + n, have + have / 2);
+    int rc = reserve_events(ctx, grow, have);
+    if (rc) return rc;
+  }
+  ctx->band_dirty = true;
+  ctx->subsets_dirty = true;
+  int rc = upload_events(ctx, responses, have, n);
+  if (rc) return rc;
+  ctx->n_events = have + n;
   return STRIP_MLEM_OK;
 }
 
@@ -809,6 +1030,7 @@ int strip_mlem_set_events_device(strip_mlem_ctx* ctx,
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->n_events = n;
   ctx->band_dirty = true;
+  ctx->subsets_dirty = true;
   return STRIP_MLEM_OK;
 }
 
@@ -947,11 +1169,40 @@ int strip_mlem_iterate_async(strip_mlem_ctx* ctx, int n_iterations) {
   if (n_iterations < 0)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "negative iteration count");
   if (n_iterations == 0) return STRIP_MLEM_OK;
+  if (ctx->n_subsets > 1) return osem_iterations(ctx, n_iterations);
   if (effective_algorithm(ctx) == 2) return band_iterations(ctx, n_iterations);
   for (int i = 0; i < n_iterations; ++i) {
     int rc = launch_iteration(ctx);
     if (rc) return rc;
   }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_subsets(strip_mlem_ctx* ctx, int n_subsets) {
+  CTX_CHECK(ctx);
+  if (n_subsets < 1 || n_subsets > 4096)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "n_subsets must be 1..4096");
+  if (n_subsets > 1 &&
+      (ctx->mode.kernel != 0 || ctx->mode.sensitivity != 0 ||
+       effective_algorithm(ctx) != 2))
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "ordered subsets need the CPU-path semantics on the "
+                       "row-band algorithm");
+  if (n_subsets != ctx->n_subsets) {
+    ctx->n_subsets = n_subsets;
+    ctx->subsets_dirty = true;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_image_total(strip_mlem_ctx* ctx, double* total) {
+  CTX_CHECK(ctx);
+  if (!total) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "total is NULL");
+  std::string err;
+  // a row-major image sums like a column-major one of the same size
+  if (strip::band_image_total(ctx->g, ctx->adjoint, ctx->rho[ctx->cur], total,
+                              ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
   return STRIP_MLEM_OK;
 }
 

@@ -123,6 +123,10 @@ def load_library():
         "strip_mlem_set_band_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 5),
         "strip_mlem_set_exp_mode": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_set_span_table": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_reserve_events": (ctypes.c_int, [P, i64]),
+        "strip_mlem_append_events": (ctypes.c_int, [P, fptr, i64]),
+        "strip_mlem_set_subsets": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_image_total": (ctypes.c_int, [P, dptr]),
         "strip_mlem_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
         "strip_mlem_batch_destroy": (None, [P]),
         "strip_mlem_batch_add": (ctypes.c_int, [P, ctypes.POINTER(Geometry), fptr, i64,
@@ -200,6 +204,8 @@ def exported_symbols():
         "strip_mlem_batch_get_rho", "strip_mlem_batch_set_tiling",
         "strip_mlem_batch_get_tiling", "strip_mlem_batch_set_graph",
         "strip_mlem_batch_last_ms", "strip_mlem_batch_autotune",
+        "strip_mlem_reserve_events", "strip_mlem_append_events",
+        "strip_mlem_set_subsets", "strip_mlem_image_total",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
@@ -360,6 +366,26 @@ class Reconstruction:
         out = np.empty(self.n_events, dtype=np.float32)
         _check(self._lib.strip_mlem_forward(self._ctx, _fptr(out)))
         return out
+
+    # ---- streaming and ordered subsets ------------------------------------
+
+    def reserve_events(self, n: int):
+        _check(self._lib.strip_mlem_reserve_events(self._ctx, int(n)))
+
+    def append_events(self, events):
+        ev = np.ascontiguousarray(events, dtype=np.float32).reshape(-1, 3)
+        _check(self._lib.strip_mlem_append_events(self._ctx, _fptr(ev),
+                                                  ev.shape[0]))
+
+    def set_subsets(self, n_subsets: int):
+        """Ordered subsets by event index (1 = plain MLEM)."""
+        _check(self._lib.strip_mlem_set_subsets(self._ctx, int(n_subsets)))
+
+    def image_total(self) -> float:
+        """Sum of the image's pixels, accumulated in FP64 on the device."""
+        v = ctypes.c_double()
+        _check(self._lib.strip_mlem_image_total(self._ctx, ctypes.byref(v)))
+        return v.value
 
     # ---- differentiation -------------------------------------------------
 

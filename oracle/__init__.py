@@ -104,6 +104,10 @@ def lib():
             "oracle_to_projection_f64": (None, [d, d, d, d, dp, dp, dp]),
             "oracle_mlem_f32": (ctypes.c_int, [G, M, fp, i64, fp, ctypes.c_int, ctypes.c_int]),
             "oracle_mlem_f64": (ctypes.c_int, [G, M, dp, i64, dp, ctypes.c_int, ctypes.c_int]),
+            "oracle_osem_f32": (ctypes.c_int, [G, M, fp, i64, fp, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int]),
+            "oracle_osem_f64": (ctypes.c_int, [G, M, dp, i64, dp, ctypes.c_int,
+                                               ctypes.c_int, ctypes.c_int]),
             "oracle_forward_f32": (ctypes.c_int, [G, M, fp, i64, fp, fp, ctypes.c_int]),
             "oracle_forward_f64": (ctypes.c_int, [G, M, dp, i64, dp, dp, ctypes.c_int]),
             "oracle_support_counts_f32": (ctypes.c_int, [G, fp, i64, i64p, i64p, ctypes.c_int]),
@@ -298,3 +302,26 @@ def mlem_loss(g, events, rho0, iterations, target, sigma=None, d_rho0=None,
     if rc:
         raise RuntimeError("oracle_mlem_loss_f64 failed")
     return float(loss[0]), float(jvp[0]), rho_n
+
+
+def osem(g, events, n_subsets, rho0=None, iterations=1, mode=MODE_CPU,
+         n_threads=0, dtype=np.float32):
+    """Ordered-subset iterations (contiguous subsets by event index)."""
+    ev = np.ascontiguousarray(events, dtype=dtype)
+    shape = (g.n_y_pixels, g.n_z_pixels)
+    rho = (np.ones(shape, dtype) if rho0 is None
+           else np.ascontiguousarray(rho0, dtype=dtype).reshape(shape).copy())
+    m = OracleMode(*mode)
+    if dtype == np.float32:
+        rc = lib().oracle_osem_f32(ctypes.byref(g), ctypes.byref(m),
+                                   _p(ev, ctypes.c_float), ev.shape[0],
+                                   _p(rho, ctypes.c_float), iterations,
+                                   n_subsets, n_threads)
+    else:
+        rc = lib().oracle_osem_f64(ctypes.byref(g), ctypes.byref(m),
+                                   _p(ev, ctypes.c_double), ev.shape[0],
+                                   _p(rho, ctypes.c_double), iterations,
+                                   n_subsets, n_threads)
+    if rc:
+        raise RuntimeError("oracle_osem failed")
+    return rho

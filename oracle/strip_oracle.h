@@ -107,6 +107,15 @@ int oracle_mlem_f64(const oracle_geometry* g, const oracle_mode* m,
 int oracle_forward_f32(const oracle_geometry* g, const oracle_mode* m,
                        const float* events, int64_t n_events, const float* rho,
                        float* denominators, int n_threads);
+/* Ordered subsets: per iteration and subset s (events [N s/S, N (s+1)/S)),
+ * the reference update over the subset times N / N_s (in double). */
+int oracle_osem_f32(const oracle_geometry* g, const oracle_mode* m,
+                    const float* events, int64_t n_events, float* rho,
+                    int n_iterations, int n_subsets, int n_threads);
+int oracle_osem_f64(const oracle_geometry* g, const oracle_mode* m,
+                    const double* events, int64_t n_events, double* rho,
+                    int n_iterations, int n_subsets, int n_threads);
+
 int oracle_forward_f64(const oracle_geometry* g, const oracle_mode* m,
                        const double* events, int64_t n_events,
                        const double* rho, double* denominators, int n_threads);

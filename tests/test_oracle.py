@@ -372,3 +372,33 @@ def test_linearity_in_events(oracle_mod, phantom_mod):
         acc += oracle_mod.mlem(g, e[None, :], rho0=rho0, iterations=1,
                                dtype=np.float64)
     assert np.allclose(all1, acc, rtol=1e-12, atol=1e-14)
+
+
+def test_osem_oracle_composition(oracle_mod, phantom_mod):
+    """One OSEM iteration with 3 subsets = the reference update over each
+    contiguous subset in turn (oracle_mlem on that subset) times N / N_s; one
+    subset = MLEM."""
+    w = phantom_mod.WORKLOADS["strip-256x160-2mm-1e6"]
+    g = oracle_mod.geometry(*w.oracle_geometry_tuple())
+    ev = w.events(3001, seed=4).astype(np.float64)
+    got = oracle_mod.osem(g, ev, 3, iterations=1, dtype=np.float64)
+    rho = np.ones((w.n_y, w.n_z))
+    n = len(ev)
+    for s in range(3):
+        lo, hi = n * s // 3, n * (s + 1) // 3
+        rho = oracle_mod.mlem(g, ev[lo:hi], rho0=rho, iterations=1,
+                              dtype=np.float64) * (n / (hi - lo))
+    assert np.max(np.abs(got - rho)) <= 1e-12 * np.max(rho)
+    one = oracle_mod.osem(g, ev, 1, iterations=2, dtype=np.float64)
+    ref = oracle_mod.mlem(g, ev, iterations=2, dtype=np.float64)
+    assert np.max(np.abs(one - ref)) <= 1e-12 * np.max(ref)
+    # mass after the last subset update: N / N_s * #{e in s: D_e > 0}
+    before = np.ones((w.n_y, w.n_z))
+    for s in range(2):
+        lo, hi = n * s // 3, n * (s + 1) // 3
+        before = oracle_mod.mlem(g, ev[lo:hi], rho0=before, iterations=1,
+                                 dtype=np.float64) * (n / (hi - lo))
+    lo, hi = n * 2 // 3, n
+    d = oracle_mod.forward(g, ev[lo:hi], before, dtype=np.float64)
+    expect = n / (hi - lo) * np.count_nonzero(d > 0)
+    assert abs(got.sum() - expect) <= 1e-9 * expect
