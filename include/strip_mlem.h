@@ -159,7 +159,7 @@ int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx);
 
 /* Tile shapes of the row-band algorithm (the knobs an autotuner turns in
  * place of -B/-W): forward_rows / backward_rows = image rows per band of the
- * forward-projection and back-projection kernels (32 or 64),
+ * forward-projection (32 or 64) and back-projection (16, 32 or 64) kernels,
  * bucket_columns = x0 bucket width bounding a task's column window (16, 32
  * or 64), forward_pairs / backward_pairs = (event, band) pairs per forward
  * workgroup / per back-projection wavefront.  Takes effect at the next
@@ -173,6 +173,9 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
  * from exp2 every 8 (forward) / 4 (back-projection) pixels.  Both stay within
  * the FP32 parity tolerance of the oracle. */
 int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode);
+/* Wavefronts per back-projection workgroup (each owns an LDS window);
+ * 0 = as many as fit, at most 4. */
+int strip_mlem_set_backprojection_waves(strip_mlem_ctx* ctx, int waves);
 /* Row runs of the row-band algorithm: 1 (default) = the in-ellipse column run
  * of every bbox row of every event is solved once per upload and stored
  * (2 bytes per row; used when every clamped bbox is at most 255 columns wide

@@ -624,6 +624,10 @@ int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
       return launch_backward_h<64>(plan.desc + slot, L.tasks, L.n_tasks, sp,
                                    rec, window_words, plan.bwd_waves, grid,
                                    stream, err);
+    case 16:
+      return launch_backward_h<16>(plan.desc + slot, L.tasks, L.n_tasks, sp,
+                                   rec, window_words, plan.bwd_waves, grid,
+                                   stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -907,7 +911,7 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   const int shift = bucket == 16 ? 4 : (bucket == 64 ? 6 : 5);
   plan.Wt = bucket + max_width;
   const int Hf = tu.fwd_rows == 32 ? 32 : 64;
-  int Hb = tu.bwd_rows == 64 ? 64 : 32;
+  int Hb = tu.bwd_rows == 64 ? 64 : (tu.bwd_rows == 16 ? 16 : 32);
   plan.fwd_lds = (size_t)plan.Wt * Hf * sizeof(float);
   if (plan.fwd_lds > kLdsLimit) {
     if (err) *err = "task window does not fit in LDS";
@@ -916,6 +920,7 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if ((size_t)plan.Wt * Hb * sizeof(float) > kLdsLimit) Hb = 32;
   const size_t window_bytes = (size_t)plan.Wt * Hb * sizeof(float);
   plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
+  if (tu.bwd_waves > 0) plan.bwd_waves = std::min(plan.bwd_waves, tu.bwd_waves);
   if (plan.bwd_waves < 1) {
     if (err) *err = "back-projection window does not fit in LDS";
     return -1;
@@ -1029,6 +1034,9 @@ int band_launch_backward(const BandInstance* desc, const int4* tasks,
                                    window_words, waves, grid, stream, err);
     case 64:
       return launch_backward_h<64>(desc, tasks, n_tasks, spans, recurrence,
+                                   window_words, waves, grid, stream, err);
+    case 16:
+      return launch_backward_h<16>(desc, tasks, n_tasks, spans, recurrence,
                                    window_words, waves, grid, stream, err);
   }
   if (err) *err = "unsupported band height";

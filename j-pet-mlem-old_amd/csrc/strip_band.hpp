@@ -32,7 +32,9 @@ namespace strip {
 
 struct BandTuning {
   int fwd_rows = 64;     // forward band height: 64 or 32
-  int bwd_rows = 32;     // back-projection band height: 64 or 32
+  int bwd_rows = 32;     // back-projection band height: 64, 32 or 16
+  int bwd_waves = 0;     // wavefronts per back-projection workgroup (0 = as
+                         // many as fit, at most 4)
   int bucket = 32;       // x0 bucket width in columns: 16, 32 or 64
   int fwd_pairs = 4096;  // pairs per forward workgroup task
   int bwd_pairs = 1024;  // pairs per back-projection wavefront task

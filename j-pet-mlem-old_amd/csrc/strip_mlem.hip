@@ -19,6 +19,7 @@
 
 #include "strip_band.hpp"
 #include "strip_device.hpp"
+#include "strip_phantom.hpp"
 #include "strip_mlem.h"
 
 using namespace strip;
@@ -1178,6 +1179,53 @@ int strip_mlem_iterate_async(strip_mlem_ctx* ctx, int n_iterations) {
   return STRIP_MLEM_OK;
 }
 
+int strip_mlem_generate_events(strip_mlem_ctx* ctx,
+                               const strip_phantom_params* params,
+                               const strip_phantom_region* regions,
+                               int n_regions, int64_t n_detected) {
+  CTX_CHECK(ctx);
+  if (!params || n_detected < 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad arguments");
+  const int64_t have = ctx->n_events;
+  int rc = reserve_events(ctx, have + n_detected, have);
+  if (rc) return rc;
+  strip::PhantomDevice P;
+  std::string err;
+  const strip_mlem_geometry& G = ctx->geometry;
+  if (strip::phantom_prepare(P, *params, regions, n_regions, G.radius,
+                             G.scintillator_length, G.sigma_z, G.sigma_dl,
+                             &err)) {
+    strip::phantom_release(P);
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, err);
+  }
+  const int64_t chunk = std::min<int64_t>(std::max<int64_t>(n_detected, 1),
+                                          strip::kPhantomBatch);
+  float* stage = nullptr;
+  if (hipMalloc(&stage, sizeof(float) * 3 * chunk) != hipSuccess) {
+    strip::phantom_release(P);
+    return strip::fail(STRIP_MLEM_ERR_ALLOC, "cannot allocate staging");
+  }
+  ctx->band_dirty = true;
+  ctx->subsets_dirty = true;
+  int64_t done = 0;
+  while (done < n_detected && !rc) {
+    int64_t got = 0;
+    const int64_t want = std::min(chunk, n_detected - done);
+    if (strip::phantom_detect(P, want, INT64_MAX, stage, nullptr, nullptr,
+                              nullptr, nullptr, &got, ctx->stream, &err)) {
+      rc = strip::fail(STRIP_MLEM_ERR_HIP, err);
+      break;
+    }
+    if ((rc = run_setup(ctx, stage, have + done, got))) break;
+    done += got;
+    ctx->n_events = have + done;
+  }
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(stage);
+  strip::phantom_release(P);
+  return rc;
+}
+
 int strip_mlem_set_subsets(strip_mlem_ctx* ctx, int n_subsets) {
   CTX_CHECK(ctx);
   if (n_subsets < 1 || n_subsets > 4096)
@@ -1251,17 +1299,29 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
                                int forward_pairs, int backward_pairs) {
   if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
   if ((forward_rows != 32 && forward_rows != 64) ||
-      (backward_rows != 32 && backward_rows != 64) ||
+      (backward_rows != 16 && backward_rows != 32 && backward_rows != 64) ||
       (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
       forward_pairs < 1 || backward_pairs < 1)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "band rows 32/64, bucket 16/32/64, pairs >= 1");
+                       "forward rows 32/64, back-projection rows 16/32/64, "
+                       "bucket 16/32/64, pairs >= 1");
   ctx->band.tuning.fwd_rows = forward_rows;
   ctx->band.tuning.bwd_rows = backward_rows;
   ctx->band.tuning.bucket = bucket_columns;
   ctx->band.tuning.fwd_pairs = forward_pairs;
   ctx->band.tuning.bwd_pairs = backward_pairs;
   ctx->band_dirty = true;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_backprojection_waves(strip_mlem_ctx* ctx, int waves) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (waves < 0 || waves > 4)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "waves must be 0..4");
+  if (ctx->band.tuning.bwd_waves != waves) {
+    ctx->band.tuning.bwd_waves = waves;
+    ctx->band_dirty = true;
+  }
   return STRIP_MLEM_OK;
 }
 
@@ -1827,7 +1887,7 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
                                 int grid_factor) {
   BATCH_CHECK(b);
   if ((forward_rows != 32 && forward_rows != 64) ||
-      (backward_rows != 32 && backward_rows != 64) ||
+      (backward_rows != 16 && backward_rows != 32 && backward_rows != 64) ||
       (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
       forward_pairs < 1 || backward_pairs < 1 || grid_factor < 1 ||
       grid_factor > 64)

@@ -69,6 +69,29 @@ class Mode(ctypes.Structure):
     ]
 
 
+class PhantomRegion(ctypes.Structure):
+    """strip_phantom_region: kind 0 ellipse (x y a b angle_deg intensity),
+    1 rectangle (x y a b intensity; a, b half sizes), 2 point."""
+
+    _fields_ = [("kind", ctypes.c_int), ("x", ctypes.c_double),
+                ("y", ctypes.c_double), ("a", ctypes.c_double),
+                ("b", ctypes.c_double), ("angle_deg", ctypes.c_double),
+                ("intensity", ctypes.c_double)]
+
+
+class PhantomParams(ctypes.Structure):
+    """strip_phantom_params (include/strip_phantom.h)."""
+
+    _fields_ = [("radius", ctypes.c_double),
+                ("scintillator_length", ctypes.c_double),
+                ("sigma_z", ctypes.c_double), ("sigma_dl", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("additive", ctypes.c_int),
+                ("only_detected", ctypes.c_int), ("n_z", ctypes.c_int),
+                ("n_y", ctypes.c_int), ("pixel", ctypes.c_double),
+                ("image_origin_z", ctypes.c_double),
+                ("image_origin_y", ctypes.c_double)]
+
+
 # Reference CPU path (graded parity mode), reconstruction.h:220-298
 MODE_CPU = (0, 0, 0, 1)
 # Legacy CUDA warp kernel, cuda/reconstruction/warp_granularity.cuh:88-152
@@ -123,6 +146,19 @@ def load_library():
         "strip_mlem_set_band_tiling": (ctypes.c_int, [P] + [ctypes.c_int] * 5),
         "strip_mlem_set_exp_mode": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_set_span_table": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_phantom_generate": (ctypes.c_int, [
+            ctypes.c_int, ctypes.POINTER(PhantomParams),
+            ctypes.POINTER(PhantomRegion), ctypes.c_int, i64, fptr, fptr,
+            ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.c_void_p,
+            ctypes.c_void_p, ctypes.c_void_p]),
+        "strip_mlem_generate_events": (ctypes.c_int, [
+            P, ctypes.POINTER(PhantomParams), ctypes.POINTER(PhantomRegion),
+            ctypes.c_int, i64]),
+        "strip_phantom_random_blocks": (ctypes.c_int, [
+            ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+            ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]),
+        "strip_phantom_last_error": (ctypes.c_char_p, []),
+        "strip_mlem_set_backprojection_waves": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_reserve_events": (ctypes.c_int, [P, i64]),
         "strip_mlem_append_events": (ctypes.c_int, [P, fptr, i64]),
         "strip_mlem_set_subsets": (ctypes.c_int, [P, ctypes.c_int]),
@@ -195,6 +231,7 @@ def exported_symbols():
         "strip_mlem_set_algorithm", "strip_mlem_get_algorithm",
         "strip_mlem_set_band_tiling", "strip_mlem_get_band_tiling",
         "strip_mlem_set_exp_mode", "strip_mlem_set_span_table",
+        "strip_mlem_set_backprojection_waves",
         "strip_mlem_get_span_table", "strip_mlem_loglik_gradient",
         "strip_mlem_loglik_gradient_device", "strip_mlem_unrolled_gradient",
         "strip_mlem_unrolled_gradient_device", "strip_mlem_batch_create",
@@ -205,6 +242,8 @@ def exported_symbols():
         "strip_mlem_batch_get_tiling", "strip_mlem_batch_set_graph",
         "strip_mlem_batch_last_ms", "strip_mlem_batch_autotune",
         "strip_mlem_reserve_events", "strip_mlem_append_events",
+        "strip_phantom_generate", "strip_mlem_generate_events",
+        "strip_phantom_random_blocks", "strip_phantom_last_error",
         "strip_mlem_set_subsets", "strip_mlem_image_total",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
@@ -331,6 +370,11 @@ class Reconstruction:
         """0 = exp2 per pixel, 1 = restarted exp recurrence along rows."""
         _check(self._lib.strip_mlem_set_exp_mode(self._ctx, int(mode)))
 
+    def set_backprojection_waves(self, waves: int):
+        """Wavefronts per back-projection workgroup (0 = automatic)."""
+        _check(self._lib.strip_mlem_set_backprojection_waves(self._ctx,
+                                                             int(waves)))
+
     def set_span_table(self, enable: bool):
         """Store every event's per-row in-ellipse runs once per upload (True)
         or solve them in every iteration (False)."""
@@ -386,6 +430,17 @@ class Reconstruction:
         v = ctypes.c_double()
         _check(self._lib.strip_mlem_image_total(self._ctx, ctypes.byref(v)))
         return v.value
+
+    def generate_events(self, regions, n_detected, seed=0, additive=False):
+        """Appends n_detected events generated on the device from the
+        phantom `regions` (phantom.Ellipse list or PhantomRegion list)."""
+        arr = _regions_array(regions)
+        p = PhantomParams()
+        p.seed = int(seed)
+        p.additive = int(bool(additive))
+        p.only_detected = 1
+        _check(self._lib.strip_mlem_generate_events(
+            self._ctx, ctypes.byref(p), arr, len(arr), int(n_detected)))
 
     # ---- differentiation -------------------------------------------------
 
@@ -604,3 +659,63 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def _regions_array(regions):
+    regions = list(regions)
+    arr = (PhantomRegion * len(regions))()
+    for i, r in enumerate(regions):
+        if isinstance(r, PhantomRegion):
+            arr[i] = r
+        else:  # phantom.Ellipse
+            arr[i] = PhantomRegion(0, r.x, r.y, r.a, r.b, r.angle, r.intensity)
+    return arr
+
+
+def generate_events(regions, radius, length, sigma_z, sigma_dl, n, seed=0,
+                    additive=False, only_detected=True, exact=False,
+                    image_grid=None, device=0):
+    """Events generated on the GPU (strip_phantom_generate).
+
+    Returns dict(responses=(m, 3) float32, exact=(m, 3) or None, n_emitted,
+    images=(emitted, detected, detected_w_error) or None), m = detected
+    events (= n with only_detected).  image_grid = (n_z, n_y, pixel[, origin_z,
+    origin_y]) requests the three histograms."""
+    lib = load_library()
+    arr = _regions_array(regions)
+    p = PhantomParams(float(radius), float(length), float(sigma_z),
+                      float(sigma_dl), int(seed), int(bool(additive)),
+                      int(bool(only_detected)), 0, 0, 0.0, 0.0, 0.0)
+    imgs = None
+    ptrs = [None, None, None]
+    if image_grid is not None:
+        g = list(image_grid) + [0.0] * (5 - len(image_grid))
+        p.n_z, p.n_y, p.pixel = int(g[0]), int(g[1]), float(g[2])
+        p.image_origin_z, p.image_origin_y = float(g[3]), float(g[4])
+        imgs = [np.zeros((p.n_y, p.n_z), np.int32) for _ in range(3)]
+        ptrs = [im.ctypes.data_as(ctypes.c_void_p) for im in imgs]
+    resp = np.empty((max(int(n), 1), 3), np.float32)
+    ex = np.empty_like(resp) if exact else None
+    nd, ne = ctypes.c_int64(), ctypes.c_int64()
+    rc = lib.strip_phantom_generate(
+        device, ctypes.byref(p), arr, len(arr), int(n), _fptr(resp),
+        _fptr(ex) if ex is not None else None, ctypes.byref(nd),
+        ctypes.byref(ne), *ptrs)
+    if rc != 0:
+        raise StripMLEMError(
+            f"status {rc}: {lib.strip_phantom_last_error().decode()}")
+    m = nd.value
+    return {"responses": resp[:m], "exact": None if ex is None else ex[:m],
+            "n_emitted": ne.value, "images": imgs}
+
+
+def random_blocks(key0, key1, first, c1, c2, n_blocks, device=0):
+    """Raw Philox4x64-10 blocks of the generator's stream."""
+    lib = load_library()
+    out = np.empty((n_blocks, 4), np.uint64)
+    rc = lib.strip_phantom_random_blocks(device, key0, key1, first, c1, c2,
+                                         n_blocks,
+                                         out.ctypes.data_as(ctypes.c_void_p))
+    if rc != 0:
+        raise StripMLEMError(lib.strip_phantom_last_error().decode())
+    return out

@@ -49,7 +49,9 @@ def test_library_exports_declared_symbols(header):
 
 
 def test_binding_covers_abi(pystrip_mod):
-    names = declared_functions(os.path.join(ROOT, "include", "strip_mlem.h"))
+    names = set()
+    for h in ("strip_mlem.h", "strip_phantom.h"):
+        names |= declared_functions(os.path.join(ROOT, "include", h))
     assert set(pystrip_mod.exported_symbols()) == names
 
 

@@ -244,3 +244,23 @@ def test_recurrence_mode_parity(pystrip_mod, phantom_mod, oracle_mod, name):
     e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=5))
     print("rel-L2", e)
     assert e <= 1e-4
+
+
+@pytest.mark.parametrize("tiling,waves", [((64, 16, 32, 4096, 1024), 0),
+                                          ((64, 16, 16, 4096, 512), 2),
+                                          ((32, 32, 16, 2048, 1024), 1),
+                                          ((64, 64, 64, 8192, 2048), 0)])
+def test_band_tilings(pystrip_mod, phantom_mod, oracle_mod, tiling, waves):
+    # every band height / bucket / task size / workgroup shape is the same
+    # reconstruction
+    print_case("band tiling", tiling, "waves", waves)
+    w, ev, rec, og = _setup(pystrip_mod, phantom_mod, oracle_mod, CFG2,
+                            60_000, algorithm=2)
+    rec.set_band_tiling(*tiling)
+    rec.set_backprojection_waves(waves)
+    nb, ne = oracle_mod.support_counts(og, ev)
+    assert rec.kernel_evaluations() == int(ne.sum())
+    rec.iterate(3)
+    e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=3))
+    print("rel-L2", e, "tiling", rec.band_tiling())
+    assert e <= 1e-4

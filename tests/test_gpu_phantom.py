@@ -1,0 +1,155 @@
+"""The GPU event generator (include/strip_phantom.h) and the two command-line
+tools end to end.
+
+The reference's own generator stream cannot be reproduced (libstdc++
+distributions re-seeded through srand48, thread interleaving), so the
+generator is pinned by:
+  - its random stream: Philox4x64-10 blocks equal numpy's Philox bit
+    generator (same algorithm, key and counter);
+  - determinism: the same seed gives the same events however they are
+    produced (host arrays or appended in place on the device);
+  - the distributions the reference defines: exact responses inside the
+    strips, errors N(0, sigma_z), N(0, sigma_dl), emission counts per region
+    proportional to intensity x area, and agreement in moments with the
+    numpy generator of pystrip/phantom.py.
+"""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, print_case
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = "strip-256x160-2mm-1e6"
+BIN = os.path.join(ROOT, "j-pet-mlem-old_amd", "bin")
+
+
+def test_philox_matches_numpy(pystrip_mod):
+    print_case("Philox4x64-10 stream vs numpy")
+    got = pystrip_mod.random_blocks(123, 456, 1, 0, 0, 16)
+    ref = np.random.Philox(key=np.array([123, 456], np.uint64)).random_raw(64)
+    assert np.array_equal(got.reshape(-1), ref)
+    got2 = pystrip_mod.random_blocks(2**63 + 5, 7, 1000, 3, 9, 4)
+    bg = np.random.Philox(key=np.array([2**63 + 5, 7], np.uint64),
+                          counter=np.array([999, 3, 9, 0], np.uint64))
+    assert np.array_equal(got2.reshape(-1), bg.random_raw(16))
+
+
+def _phantom(phantom_mod):
+    return phantom_mod.WORKLOADS[CFG1]
+
+
+def test_generator_deterministic(pystrip_mod, phantom_mod):
+    print_case("generator determinism")
+    w = _phantom(phantom_mod)
+    args = (w.phantom, w.radius, w.length, w.sigma_z, w.sigma_dl)
+    a = pystrip_mod.generate_events(*args, 300_000, seed=5)["responses"]
+    b = pystrip_mod.generate_events(*args, 300_000, seed=5)["responses"]
+    c = pystrip_mod.generate_events(*args, 300_000, seed=6)["responses"]
+    assert a.shape == (300_000, 3) and np.array_equal(a, b)
+    assert not np.array_equal(a, c)
+    # a prefix of a longer run is the shorter run (emission order)
+    d = pystrip_mod.generate_events(*args, 100_000, seed=5)["responses"]
+    assert np.array_equal(a[:100_000], d)
+    # generated in place on the device (in 2^22-emission batches) = the
+    # host array uploaded
+    big = pystrip_mod.generate_events(*args, 5_000_000, seed=9)["responses"]
+    r1 = pystrip_mod.Reconstruction(w.geometry())
+    r1.generate_events(w.phantom, 5_000_000, seed=9)
+    r2 = pystrip_mod.Reconstruction(w.geometry())
+    r2.set_events(big)
+    assert r1.n_events == r2.n_events == 5_000_000
+    assert r1.kernel_evaluations() == r2.kernel_evaluations()
+    assert np.array_equal(r1.forward(), r2.forward())
+
+
+def test_generator_distributions(pystrip_mod, phantom_mod):
+    print_case("generator distributions")
+    w = _phantom(phantom_mod)
+    n = 2_000_000
+    out = pystrip_mod.generate_events(
+        w.phantom, w.radius, w.length, w.sigma_z, w.sigma_dl, n, seed=1,
+        exact=True, image_grid=(w.n_z, w.n_y, w.pixel))
+    resp, ex = out["responses"].astype(np.float64), out["exact"]
+    assert len(resp) == n and out["n_emitted"] > n
+    # exact detection (scanner.h:196-214)
+    assert np.all(np.abs(ex[:, :2]) < w.length / 2)
+    err = resp - ex
+    for k, s in ((0, w.sigma_z), (1, w.sigma_z), (2, w.sigma_dl)):
+        assert abs(err[:, k].mean()) < 5 * s / np.sqrt(n)
+        assert abs(err[:, k].std() / s - 1) < 0.005
+    assert abs(np.corrcoef(err[:, 0], err[:, 1])[0, 1]) < 0.005
+    # moments against the numpy generator (same semantics, other stream)
+    ref = w.events(n, seed=2).astype(np.float64)
+    for k in range(3):
+        se = np.sqrt(resp[:, k].var() / n + ref[:, k].var() / n)
+        assert abs(resp[:, k].mean() - ref[:, k].mean()) < 5 * se
+        assert abs(resp[:, k].std() / ref[:, k].std() - 1) < 0.01
+    # emissions per region ~ intensity x area (non-additive: earlier regions
+    # win, so region k holds the pixels of k not in earlier regions)
+    emitted, detected, naive = out["images"]
+    img = phantom_mod.phantom_image(w.phantom, w.n_z, w.n_y, w.pixel)
+    img = img[::-1]  # phantom_image rows run from +y down; the grid from -y up
+    tot = emitted.sum()
+    assert tot <= out["n_emitted"]
+    # interior pixels of each intensity level (all four neighbours at the
+    # same level): mean emissions per pixel / intensity is one constant
+    same = np.ones_like(img, dtype=bool)
+    same[1:-1, 1:-1] = ((img[1:-1, 1:-1] == img[:-2, 1:-1]) &
+                        (img[1:-1, 1:-1] == img[2:, 1:-1]) &
+                        (img[1:-1, 1:-1] == img[1:-1, :-2]) &
+                        (img[1:-1, 1:-1] == img[1:-1, 2:]))
+    ratios = []
+    for level in np.unique(img[img > 0]):
+        mask = (img == level) & same
+        if mask.sum() < 50:
+            continue
+        ratios.append(emitted[mask].mean() / level)
+    ratios = np.array(ratios)
+    print("emissions per pixel / intensity", ratios)
+    assert len(ratios) >= 2
+    assert np.max(np.abs(ratios / ratios.mean() - 1)) < 0.02
+    assert detected.sum() <= tot and naive.sum() <= detected.sum()
+
+
+def test_command_line_tools_end_to_end(tmp_path, oracle_mod, phantom_mod):
+    # 2d_strip_phantom writes events + .cfg; 2d_strip_reconstruction picks
+    # the .cfg up and reconstructs on the GPU; the image equals the oracle's
+    print_case("2d_strip_phantom | 2d_strip_reconstruction")
+    w = _phantom(phantom_mod)
+    desc = tmp_path / "three"
+    desc.write_text("".join(r.text() + "\n" for r in w.phantom))
+    out = tmp_path / "p.bin"
+    r = subprocess.run(
+        [os.path.join(BIN, "2d_strip_phantom"), str(desc), "-r",
+         str(w.radius), "-l", str(w.length), "-p", str(w.pixel),
+         "--n-z-pixels", str(w.n_z), "--n-y-pixels", str(w.n_y), "--s-z",
+         str(w.sigma_z), "--s-dl", str(w.sigma_dl), "-e", "20000",
+         "--detected", "-o", str(out), "--seed", "3", "-v"],
+        capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    ev = np.fromfile(out, dtype=np.float32).reshape(-1, 3)
+    assert len(ev) == 20000
+    assert (tmp_path / "p.cfg").exists()
+    for name in ("_detected", "_emitted", "_naive"):
+        assert (tmp_path / ("p" + name)).exists()
+    rec_out = tmp_path / "rec.bin"
+    r = subprocess.run(
+        [os.path.join(BIN, "2d_strip_reconstruction"), str(out), "-i", "1",
+         "-I", "3", "-o", str(rec_out)],
+        capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    files = sorted(glob.glob(str(tmp_path / "rec_*.bin")))
+    assert files, os.listdir(tmp_path)
+    img = np.fromfile(files[-1], dtype=np.float32).reshape(w.n_y, w.n_z)
+    og = oracle_mod.geometry(*w.oracle_geometry_tuple())
+    ref = oracle_mod.mlem(og, ev, iterations=3)
+    e = oracle_mod.rel_l2(img, ref)
+    print("CLI rel-L2", e)
+    assert e <= 1e-4
