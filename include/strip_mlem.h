@@ -170,7 +170,7 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
 /* How the row-band kernels evaluate exp(-Q/2) along an image row: 0 = one
  * hardware exp2 per pixel; 1 = the recurrence e[k+1] = e[k] r[k],
  * r[k+1] = r[k] G (exact for exp2 of a quadratic in the column), restarted
- * from exp2 every 8 (forward) / 4 (back-projection) pixels.  Both stay within
+ * from exp2 every 8 pixels (4 in a back-projection tail).  Both stay within
  * the FP32 parity tolerance of the oracle. */
 int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode);
 /* Wavefronts per back-projection workgroup (each owns an LDS window);

@@ -419,8 +419,27 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
       float* w = win + (col0 - c0) * H + r;
       int k = 0;
       if (REC) {
-        // e[k+1] = e[k] r[k], r[k+1] = r[k] G, restarted every 4 columns
+        // e[k+1] = e[k] r[k], r[k+1] = r[k] G, restarted every 8 columns
+        // (4 for a tail): the block's words are all read before any is
+        // written, so eight reads are in flight together
         const float db = __builtin_fmaf(qc * step, step, qb * step);
+        for (; k + 8 <= nq; k += 8) {
+          float a[8];
+#pragma unroll
+          for (int i = 0; i < 8; ++i) a[i] = w[i * S];
+          float e = __builtin_amdgcn_exp2f(
+              __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
+          float rr =
+              __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db));
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            w[i * S] = __builtin_fmaf(e, scale, a[i]);
+            e *= rr;
+            rr *= G;
+          }
+          dz += 8.0f * step;
+          w += 8 * S;
+        }
         for (; k + 4 <= nq; k += 4) {
           const float a0 = w[0], a1 = w[S], a2 = w[2 * S], a3 = w[3 * S];
           const float e0 = __builtin_amdgcn_exp2f(

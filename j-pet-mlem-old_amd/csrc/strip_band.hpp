@@ -32,16 +32,16 @@ namespace strip {
 
 struct BandTuning {
   int fwd_rows = 64;     // forward band height: 64 or 32
-  int bwd_rows = 32;     // back-projection band height: 64, 32 or 16
+  int bwd_rows = 16;     // back-projection band height: 64, 32 or 16
   int bwd_waves = 0;     // wavefronts per back-projection workgroup (0 = as
                          // many as fit, at most 4)
-  int bucket = 32;       // x0 bucket width in columns: 16, 32 or 64
-  int fwd_pairs = 4096;  // pairs per forward workgroup task
-  int bwd_pairs = 1024;  // pairs per back-projection wavefront task
+  int bucket = 16;       // x0 bucket width in columns: 16, 32 or 64
+  int fwd_pairs = 2048;  // pairs per forward workgroup task
+  int bwd_pairs = 512;   // pairs per back-projection wavefront task
   // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
   // two-term recurrence e[k+1] = e[k] r[k], r[k+1] = r[k] G of exp2 of a
-  // quadratic, restarted from exp2 every 8 (forward) / 4 (back-projection)
-  // pixels
+  // quadratic, restarted from exp2 every 8 pixels (4 in a back-projection
+  // tail)
   int recurrence = 1;
   // 1 = build the span table when it fits, 0 = solve every row's run in the
   // iteration kernels

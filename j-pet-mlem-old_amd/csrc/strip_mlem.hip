@@ -1980,8 +1980,8 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
   double best = 0.0;
   if ((rc = time_it(cur, &best))) return rc;
   const std::vector<std::vector<int>> choices = {
-      {64, 32}, {32, 64}, {16, 32, 64}, {2048, 4096, 8192}, {512, 1024, 2048},
-      {2, 4, 8}};
+      {64, 32}, {16, 32, 64}, {16, 32, 64}, {1024, 2048, 4096},
+      {256, 512, 1024}, {2, 4, 8}};
   for (int k = 0; k < 6; ++k) {
     for (int v : choices[k]) {
       if (v == cur[k]) continue;

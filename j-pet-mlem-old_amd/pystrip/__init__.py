@@ -359,9 +359,9 @@ class Reconstruction:
     def set_algorithm(self, algorithm: int):
         _check(self._lib.strip_mlem_set_algorithm(self._ctx, algorithm))
 
-    def set_band_tiling(self, forward_rows=64, backward_rows=32,
-                        bucket_columns=32, forward_pairs=4096,
-                        backward_pairs=1024):
+    def set_band_tiling(self, forward_rows=64, backward_rows=16,
+                        bucket_columns=16, forward_pairs=2048,
+                        backward_pairs=512):
         _check(self._lib.strip_mlem_set_band_tiling(
             self._ctx, forward_rows, backward_rows, bucket_columns,
             forward_pairs, backward_pairs))
@@ -630,8 +630,8 @@ class Batch:
                                                   ctypes.byref(c)))
         return _MemberView(self._lib, c, self.geometries[index])
 
-    def set_tiling(self, forward_rows=64, backward_rows=32, bucket_columns=32,
-                   forward_pairs=4096, backward_pairs=1024, grid_factor=4):
+    def set_tiling(self, forward_rows=64, backward_rows=16, bucket_columns=16,
+                   forward_pairs=2048, backward_pairs=512, grid_factor=4):
         _check(self._lib.strip_mlem_batch_set_tiling(
             self._b, forward_rows, backward_rows, bucket_columns,
             forward_pairs, backward_pairs, grid_factor))
