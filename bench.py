@@ -32,6 +32,15 @@ PEAK_FP32_TFLOPS = 157.3
 # exp2 (1), K = pre*e (1), D += K*rho (2); again Q (4), exp2 (1), K (1),
 # K*inv (1) and the image add (1)
 FLOPS_PER_KERNEL_EVAL = 16
+# HBM bytes per launch (read + written) of the band kernels on the default
+# workload, from two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) each scaled
+# by the counter's reading for a plain 256 MiB copy kernel (x2.0 and x1.0);
+# recorded in profiles/band_kernels_strip-1000x400-1mm-2e7_counters.md
+MEASURED_TRAFFIC_BYTES = {
+    "back-projection": (32.59 + 1.73) * 1e9,
+    "forward projection": (14.06 + 0.71) * 1e9,
+}
+TRAFFIC_WORKLOAD = "strip-1000x400-1mm-2e7"
 
 
 def cpu_baseline(workload, n_sample, threads):
@@ -248,7 +257,12 @@ def main():
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / PEAK_FP32_TFLOPS,
-            "traffic": None,
+            "traffic": (MEASURED_TRAFFIC_BYTES.get(dom_name)
+                        if w.name == TRAFFIC_WORKLOAD and
+                        n_events == w.n_events and args.algorithm in (0, 2)
+                        else None),
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + "
+                            "WRITE_SIZE, copy-calibrated)",
             "kernel": dom_name,
             "kernel_avg_ms": dom_ms,
             "forward_avg_ms": fwd_avg,

@@ -45,10 +45,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
     float* __restrict__ moments, int max_bands) {
   extern __shared__ float win[];
   const int4 task = tasks[blockIdx.x];
-  const int b = task.x;
-  const int c0 = task.w;
+  const int b = task.x & 0xffff;
+  const int c0 = task.w & 0xffff;
   const int y_base = b * H;
-  const int n_words = Wt * H;
+  // the columns the task's pairs reach (Wt: the window stride)
+  const int n_words = min(task.w >> 16, Wt) * H;
   for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
     const int col = c0 + i / H, iy = y_base + (i & (H - 1));
     const bool in = col < g.n_z && iy < g.n_y;
@@ -509,7 +510,7 @@ int band_adjoint_prepare(BandPlan& plan, const DeviceGeometry& g,
 int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
                            AdjointWork& wk, const float* rhoT_k, float* gbarT,
                            double grad_ln_sigma[2], hipStream_t stream,
-                           std::string* err) {
+                           std::string* err, const float* bpT_k) {
   if (!plan.ready) {
     if (err) *err = "band plan not prepared";
     return -1;
@@ -528,8 +529,9 @@ int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
   hipLaunchKernelGGL(mul_kernel, dim3(pg), dim3(256), 0, stream, n_pixels,
                      gbarT, rhoT_k, hT);
   BAND_TRY(hipGetLastError());
-  hipLaunchKernelGGL(zero_kernel, dim3(pg), dim3(256), 0, stream, n_pixels,
-                     bp1);
+  if (!bpT_k)
+    hipLaunchKernelGGL(zero_kernel, dim3(pg), dim3(256), 0, stream, n_pixels,
+                       bp1);
   hipLaunchKernelGGL(zero_kernel, dim3(pg), dim3(256), 0, stream, n_pixels,
                      bp2);
   BAND_TRY(hipGetLastError());
@@ -545,14 +547,14 @@ int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
     if (reduce_sum(wk, gz, n, 0, stream, err) ||
         reduce_sum(wk, gd, n, 1, stream, err))
       return -1;
-    if (band_backproject(plan, g, w1, bp1, stream, err) ||
+    if ((!bpT_k && band_backproject(plan, g, w1, bp1, stream, err)) ||
         band_backproject(plan, g, w2, bp2, stream, err))
       return -1;
     BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(float) * 2,
                             hipMemcpyDeviceToHost, stream));
   }
   hipLaunchKernelGGL(reverse_combine_kernel, dim3(pg), dim3(256), 0, stream,
-                     n_pixels, bp1, bp2, gbarT);
+                     n_pixels, bpT_k ? bpT_k : bp1, bp2, gbarT);
   BAND_TRY(hipGetLastError());
   BAND_TRY(hipStreamSynchronize(stream));
   grad_ln_sigma[0] += sums[0];

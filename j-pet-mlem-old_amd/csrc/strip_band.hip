@@ -135,6 +135,27 @@ __global__ __launch_bounds__(256) void group_bounds_kernel(
   if (i == 0 || (keys[i - 1] >> shift) != k) first[k] = (int)i;
 }
 
+// task.w |= (columns its pairs reach, counted from its first column) << 16,
+// at most Wt: one wavefront per task
+__global__ __launch_bounds__(256) void task_extent_kernel(
+    const BandRecord* __restrict__ rec, const int* __restrict__ pair_event,
+    int4* __restrict__ tasks, int n_tasks, int Wt) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= n_tasks) return;  // uniform over the wavefront
+  const int lane = threadIdx.x & 63;
+  const int4 task = tasks[t];
+  int reach = 1;
+  for (int p = task.y + lane; p < task.z; p += 64)
+    reach = max(reach, (int)(rec[pair_event[p]].bx >> 16) - task.w);
+  for (int o = 32; o > 0; o >>= 1) reach = max(reach, __shfl_xor(reach, o));
+  if (lane == 0) tasks[t].w = task.w | (min(reach, Wt) << 16);
+}
+
+__global__ __launch_bounds__(256) void zero_ints_kernel(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
 // ---------------------------------------------------------------------------
 // image layout kernels
 // ---------------------------------------------------------------------------
@@ -186,9 +207,10 @@ __global__ __launch_bounds__(256) void to_row_major_kernel(
 __global__ __launch_bounds__(256) void band_finalize_kernel(
     int64_t n, const float* __restrict__ rho, float* __restrict__ bp,
     const float* __restrict__ sens, int divide, double scale,
-    float* __restrict__ next) {
+    float* __restrict__ next, float* __restrict__ bp_out) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
+    if (bp_out) bp_out[i] = bp[i];
     float v = rho[i] * bp[i];
     if (divide) v = v / sens[i];
     if (scale != 1.0) v = (float)((double)v * scale);
@@ -222,18 +244,28 @@ __global__ __launch_bounds__(256) void band_inverse_kernel(
 template <int H, int MODE, bool REC, bool SPANS>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
-    int n_tasks, unsigned long long* __restrict__ count) {
+    int n_tasks, int* __restrict__ queue,
+    unsigned long long* __restrict__ count) {
   extern __shared__ float win[];
+  __shared__ int queued;
   constexpr int LR = 64 / H;  // lanes per image row
   const int lane = threadIdx.x & 63;
   const int r = lane & (H - 1);
   const int q = lane / H;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   unsigned long long counted = 0;
-  // tasks in grid strides: a launch smaller than the task list keeps its
-  // workgroups resident and walks the list
-  for (int t = blockIdx.x; t < n_tasks; t += gridDim.x) {
+  // Resident workgroups walk the task list: from a queue counter (tasks
+  // handed out as workgroups finish) or in grid strides.
+  for (int t = blockIdx.x;;) {
+  if (queue) {
+    __syncthreads();  // the previous task's window and index are done with
+    if (threadIdx.x == 0) queued = atomicAdd(queue, 1);
+    __syncthreads();
+    t = queued;
+  }
+  if (t >= n_tasks) break;
   const int4 task = tasks[t];
+  t += gridDim.x;
   const int b = task.x & 0xffff;
   const BandInstance& d = desc[task.x >> 16];
   const DeviceGeometry g = d.g;
@@ -243,11 +275,12 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   const int span_stride = d.span_stride;
   float* __restrict__ partial = d.partial;
   const int max_bands = d.fwd_max_bands;
-  const int c0 = task.w;
+  const int c0 = task.w & 0xffff;
   const int y_base = b * H;
   if (MODE == 0) {
     const float* __restrict__ rhoT = d.rhoT;
-    const int n_words = d.Wt * H;
+    // the columns the task's pairs reach
+    const int n_words = (task.w >> 16) * H;
     __syncthreads();  // the previous task's window is no longer read
     for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
       const int col = c0 + i / H, iy = y_base + (i & (H - 1));
@@ -348,16 +381,23 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
 template <int H, bool REC, bool SPANS>
 __global__ __launch_bounds__(256) void band_backward_kernel(
     const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
-    int n_tasks, int window_words) {
+    int n_tasks, int* __restrict__ queue, int window_words) {
   extern __shared__ float windows[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n_waves = blockDim.x >> 6;
   float* win = windows + wave * window_words;
-  // tasks in grid strides of the launch's wavefronts
-  for (int t = blockIdx.x * n_waves + wave; t < n_tasks;
-       t += gridDim.x * n_waves) {
+  // each wavefront walks the task list: from a queue counter or in grid
+  // strides of the launch's wavefronts
+  for (int t = blockIdx.x * n_waves + wave;;) {
+  if (queue) {
+    int got = 0;
+    if (lane == 0) got = atomicAdd(queue, 1);
+    t = __builtin_amdgcn_readfirstlane(got);
+  }
+  if (t >= n_tasks) break;
   const int4 task = tasks[t];
+  t += gridDim.x * n_waves;
   const int b = task.x & 0xffff;
   const BandInstance& d = desc[task.x >> 16];
   const DeviceGeometry g = d.g;
@@ -367,9 +407,9 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
   const int span_stride = d.span_stride;
   const float* __restrict__ inv = d.inv;
   float* __restrict__ bpT = d.bpT;
-  const int Wt = d.Wt;
+  const int Wt = task.w >> 16;  // the columns the task's pairs reach
   for (int i = lane; i < Wt * H; i += 64) win[i] = 0.0f;
-  const int c0 = task.w;
+  const int c0 = task.w & 0xffff;
   const int y_base = b * H;
   constexpr int LR = 64 / H;  // lanes per image row
   constexpr int S = LR * H;   // word stride between a lane's columns
@@ -544,8 +584,8 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(
 template <int H>
 int launch_forward_h(const BandInstance* desc, const int4* tasks, int n_tasks,
                      bool sp, bool recurrence, int mode, size_t lds, int grid,
-                     unsigned long long* count, hipStream_t stream,
-                     std::string* err) {
+                     int* queue, unsigned long long* count,
+                     hipStream_t stream, std::string* err) {
   if (n_tasks == 0) return 0;
   if (mode == 0) {
     auto kern = recurrence ? (sp ? band_forward_kernel<H, 0, true, true>
@@ -554,12 +594,12 @@ int launch_forward_h(const BandInstance* desc, const int4* tasks, int n_tasks,
                                  : band_forward_kernel<H, 0, false, false>);
     if (set_lds(kern, lds, err)) return -1;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kFwdWaves), lds, stream,
-                       desc, tasks, n_tasks, count);
+                       desc, tasks, n_tasks, queue, count);
   } else {
     auto kern = sp ? band_forward_kernel<H, 1, false, true>
                    : band_forward_kernel<H, 1, false, false>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kFwdWaves), 0, stream,
-                       desc, tasks, n_tasks, count);
+                       desc, tasks, n_tasks, queue, count);
   }
   BAND_TRY(hipGetLastError());
   return 0;
@@ -568,7 +608,7 @@ int launch_forward_h(const BandInstance* desc, const int4* tasks, int n_tasks,
 template <int H>
 int launch_backward_h(const BandInstance* desc, const int4* tasks,
                       int n_tasks, bool sp, bool recurrence, int window_words,
-                      int waves, int grid, hipStream_t stream,
+                      int waves, int grid, int* queue, hipStream_t stream,
                       std::string* err) {
   if (n_tasks == 0) return 0;
   auto kern = recurrence ? (sp ? band_backward_kernel<H, true, true>
@@ -578,7 +618,7 @@ int launch_backward_h(const BandInstance* desc, const int4* tasks,
   const size_t lds = sizeof(float) * (size_t)window_words * waves;
   if (set_lds(kern, lds, err)) return -1;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), lds, stream, desc,
-                     tasks, n_tasks, window_words);
+                     tasks, n_tasks, queue, window_words);
   BAND_TRY(hipGetLastError());
   return 0;
 }
@@ -602,10 +642,12 @@ int dispatch_forward(BandPlan& plan, const DeviceGeometry& g, int mode,
   switch (L.H) {
     case 32:
       return launch_forward_h<32>(desc, L.tasks, L.n_tasks, sp, rec, mode,
-                                  plan.fwd_lds, L.n_tasks, count, stream, err);
+                                  plan.fwd_lds, L.n_tasks, nullptr, count,
+                                  stream, err);
     case 64:
       return launch_forward_h<64>(desc, L.tasks, L.n_tasks, sp, rec, mode,
-                                  plan.fwd_lds, L.n_tasks, count, stream, err);
+                                  plan.fwd_lds, L.n_tasks, nullptr, count,
+                                  stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -638,15 +680,15 @@ int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
     case 32:
       return launch_backward_h<32>(plan.desc + slot, L.tasks, L.n_tasks, sp,
                                    rec, window_words, plan.bwd_waves, grid,
-                                   stream, err);
+                                   nullptr, stream, err);
     case 64:
       return launch_backward_h<64>(plan.desc + slot, L.tasks, L.n_tasks, sp,
                                    rec, window_words, plan.bwd_waves, grid,
-                                   stream, err);
+                                   nullptr, stream, err);
     case 16:
       return launch_backward_h<16>(plan.desc + slot, L.tasks, L.n_tasks, sp,
                                    rec, window_words, plan.bwd_waves, grid,
-                                   stream, err);
+                                   nullptr, stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -767,9 +809,11 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
   return 0;
 }
 
-// Tasks of at most `chunk` consecutive pairs inside one group.
+// Tasks of at most `chunk` consecutive pairs inside one group, each with the
+// column extent of its pairs' boxes.
 int build_tasks(PairList& L, int shift, const std::vector<int>& first,
-                int chunk, hipStream_t stream, std::string* err) {
+                int chunk, const BandRecord* rec, int Wt, hipStream_t stream,
+                std::string* err) {
   const int n_groups = (int)first.size() - 1;
   std::vector<int4> tasks;
   std::vector<int> end(n_groups, 0);
@@ -787,13 +831,19 @@ int build_tasks(PairList& L, int shift, const std::vector<int>& first,
       tasks.push_back(make_int4(band, p, std::min(end[k], p + chunk), c0));
   }
   L.n_tasks = (int)tasks.size();
-  L.tasks_host = tasks;
   if (!tasks.empty()) {
     BAND_TRY(hipMalloc(&L.tasks, sizeof(int4) * tasks.size()));
     BAND_TRY(hipMemcpyAsync(L.tasks, tasks.data(), sizeof(int4) * tasks.size(),
                             hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(task_extent_kernel, dim3((L.n_tasks + 3) / 4),
+                       dim3(256), 0, stream, rec, L.pair_event, L.tasks,
+                       L.n_tasks, Wt);
+    BAND_TRY(hipGetLastError());
+    BAND_TRY(hipMemcpyAsync(tasks.data(), L.tasks, sizeof(int4) * tasks.size(),
+                            hipMemcpyDeviceToHost, stream));
     BAND_TRY(hipStreamSynchronize(stream));
   }
+  L.tasks_host = tasks;
   return 0;
 }
 
@@ -949,7 +999,8 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if (build_pairs(plan.fwd, Hf, shift, g, plan.records, n, stream, first,
                   err))
     return -1;
-  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, stream, err))
+  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, plan.records,
+                  plan.Wt, stream, err))
     return -1;
   if (Hb == Hf) {
     plan.bwd.H = Hf;
@@ -962,7 +1013,8 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                          first, err)) {
     return -1;
   }
-  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, stream, err))
+  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records,
+                  plan.Wt, stream, err))
     return -1;
   if (n > 0) {
     BAND_TRY(hipMalloc(&plan.partial,
@@ -983,6 +1035,14 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                           hipMemcpyHostToDevice, stream));
   BAND_TRY(hipStreamSynchronize(stream));
   plan.ready = true;
+  return 0;
+}
+
+int band_zero_ints(int* p, int n, hipStream_t stream, std::string* err) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(zero_ints_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                     stream, p, n);
+  BAND_TRY(hipGetLastError());
   return 0;
 }
 
@@ -1009,15 +1069,15 @@ BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
 
 int band_launch_forward(const BandInstance* desc, const int4* tasks,
                         int n_tasks, int H, bool spans, bool recurrence,
-                        size_t lds, int grid, hipStream_t stream,
+                        size_t lds, int grid, int* queue, hipStream_t stream,
                         std::string* err) {
   switch (H) {
     case 32:
       return launch_forward_h<32>(desc, tasks, n_tasks, spans, recurrence, 0,
-                                  lds, grid, nullptr, stream, err);
+                                  lds, grid, queue, nullptr, stream, err);
     case 64:
       return launch_forward_h<64>(desc, tasks, n_tasks, spans, recurrence, 0,
-                                  lds, grid, nullptr, stream, err);
+                                  lds, grid, queue, nullptr, stream, err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -1045,18 +1105,21 @@ int band_launch_batch_update(const BandInstance* desc, int n_instances,
 
 int band_launch_backward(const BandInstance* desc, const int4* tasks,
                          int n_tasks, int H, bool spans, bool recurrence,
-                         int window_words, int waves, int grid,
+                         int window_words, int waves, int grid, int* queue,
                          hipStream_t stream, std::string* err) {
   switch (H) {
     case 32:
       return launch_backward_h<32>(desc, tasks, n_tasks, spans, recurrence,
-                                   window_words, waves, grid, stream, err);
+                                   window_words, waves, grid, queue, stream,
+                                   err);
     case 64:
       return launch_backward_h<64>(desc, tasks, n_tasks, spans, recurrence,
-                                   window_words, waves, grid, stream, err);
+                                   window_words, waves, grid, queue, stream,
+                                   err);
     case 16:
       return launch_backward_h<16>(desc, tasks, n_tasks, spans, recurrence,
-                                   window_words, waves, grid, stream, err);
+                                   window_words, waves, grid, queue, stream,
+                                   err);
   }
   if (err) *err = "unsupported band height";
   return -1;
@@ -1095,7 +1158,7 @@ int band_store_image(BandPlan& plan, const DeviceGeometry& g, float* rho,
 int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                    const EventSoA& ev, int sensitivity_mode, int guard_zero,
                    hipStream_t stream, hipEvent_t t_fwd, hipEvent_t t_bwd,
-                   std::string* err, double scale) {
+                   std::string* err, double scale, float* bp_out) {
   if (!plan.ready) {
     if (err) *err = "band plan not prepared";
     return -1;
@@ -1122,7 +1185,7 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
   const int blocks = (int)std::min<int64_t>((n_pixels + 255) / 256, 4096);
   hipLaunchKernelGGL(band_finalize_kernel, dim3(blocks), dim3(256), 0, stream,
                      n_pixels, rho, plan.bpT, plan.sensT,
-                     sensitivity_mode == 2 ? 1 : 0, scale, next);
+                     sensitivity_mode == 2 ? 1 : 0, scale, next, bp_out);
   BAND_TRY(hipGetLastError());
   plan.curT ^= 1;
   return 0;

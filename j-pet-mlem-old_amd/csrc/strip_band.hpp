@@ -65,7 +65,8 @@ struct PairList {
   int max_bands = 0;          // most bands one event touches
   int64_t n_pairs = 0;
   int* pair_event = nullptr;  // pair -> event
-  int4* tasks = nullptr;      // (band | instance << 16, begin, end, column)
+  // (band | instance << 16, begin, end, first column | columns << 16)
+  int4* tasks = nullptr;
   int n_tasks = 0;
   bool owns_pairs = true;
   std::vector<int4> tasks_host;  // host copy (instance 0)
@@ -139,14 +140,19 @@ BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
 // grid = workgroups (tasks are taken in grid strides); lds = dynamic LDS
 // bytes; backward: window words per wavefront (largest Wt x H) and
 // wavefronts per workgroup.
+// queue (may be null): a device counter, zero at launch, from which the
+// resident workgroups (forward) / wavefronts (back-projection) take tasks.
 int band_launch_forward(const BandInstance* desc, const int4* tasks,
                         int n_tasks, int H, bool spans, bool recurrence,
-                        size_t lds, int grid, hipStream_t stream,
+                        size_t lds, int grid, int* queue, hipStream_t stream,
                         std::string* err);
 int band_launch_backward(const BandInstance* desc, const int4* tasks,
                          int n_tasks, int H, bool spans, bool recurrence,
-                         int window_words, int waves, int grid,
+                         int window_words, int waves, int grid, int* queue,
                          hipStream_t stream, std::string* err);
+
+// Zeroes n ints on the stream with a kernel (graph-recordable).
+int band_zero_ints(int* p, int n, hipStream_t stream, std::string* err);
 
 // Per-instance updates of a batch (blockIdx.y = instance): inverse = true
 // forms inv = D>0 ? 1/D : 0 from the partials, inverse = false writes
@@ -175,11 +181,13 @@ int band_store_image(BandPlan& plan, const DeviceGeometry& g, float* rho,
 // One iteration.  t_fwd (optional) is recorded after the forward kernel,
 // t_bwd after the back-projection kernel.
 // scale multiplies the update (in FP64): 1 for MLEM, the ratio of all events
-// to the plan's events for an ordered-subset update.
+// to the plan's events for an ordered-subset update.  bp_out (column-major,
+// may be null) receives the iteration's back-projection sum_e K_e / D_e.
 int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                    const EventSoA& ev, int sensitivity_mode, int guard_zero,
                    hipStream_t stream, hipEvent_t t_fwd, hipEvent_t t_bwd,
-                   std::string* err, double scale = 1.0);
+                   std::string* err, double scale = 1.0,
+                   float* bp_out = nullptr);
 
 // Per-event forward projections of the current column-major image.
 int band_forward_only(BandPlan& plan, const DeviceGeometry& g,
@@ -242,11 +250,13 @@ int band_loglik_gradient(BandPlan& plan, const DeviceGeometry& g,
 
 // One reverse step through rho_{k+1} = rho_k * sum_e K_e / D_e(rho_k):
 // gbarT (column-major d loss / d rho_{k+1}) is replaced by d loss / d rho_k
-// and the step's d loss / d ln sigma is added to grad_ln_sigma.
+// and the step's d loss / d ln sigma is added to grad_ln_sigma.  bpT_k (may
+// be null) is the forward iteration's back-projection sum_e K_e / D_e kept
+// from the forward pass; without it the step computes it again.
 int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
                            AdjointWork& wk, const float* rhoT_k, float* gbarT,
                            double grad_ln_sigma[2], hipStream_t stream,
-                           std::string* err);
+                           std::string* err, const float* bpT_k = nullptr);
 
 // Allocates the work buffers for the plan's event count and grid.
 int band_adjoint_prepare(BandPlan& plan, const DeviceGeometry& g,

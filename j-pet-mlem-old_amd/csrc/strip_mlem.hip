@@ -753,14 +753,16 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
                        "n_iterations >= 0 and a target image are required");
   int rc = adjoint_ready(ctx);
   if (rc) return rc;
-  // checkpoints rho_0 .. rho_N, then the target and the adjoint image
-  if ((rc = ensure_images(ctx, (int64_t)n_iterations + 3))) return rc;
+  // checkpoints rho_0 .. rho_N, the target, the adjoint image, then the
+  // back-projections B_0 .. B_{N-1} of the forward iterations
+  if ((rc = ensure_images(ctx, 2 * (int64_t)n_iterations + 3))) return rc;
   std::string err;
   strip::BandPlan& plan = ctx->band;
   const int64_t np = ctx->n_pixels;
   auto ck = [&](int64_t k) { return ctx->ckpt + k * np; };
   float* targetT = ck(n_iterations + 1);
   float* gbarT = ck(n_iterations + 2);
+  auto bk = [&](int64_t k) { return ck(n_iterations + 3 + k); };
   if (strip::band_to_column_major(ctx->g, d_rho0 ? d_rho0 : ctx->rho[ctx->cur],
                                   ck(0), ctx->stream, &err) ||
       strip::band_to_column_major(ctx->g, d_target, targetT, ctx->stream,
@@ -770,7 +772,7 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
     HIP_TRY(hipMemcpyAsync(plan.rhoT[plan.curT], ck(k), sizeof(float) * np,
                            hipMemcpyDeviceToDevice, ctx->stream));
     if (strip::band_iteration(plan, ctx->g, ctx->ev, 0, 1, ctx->stream,
-                              nullptr, nullptr, &err))
+                              nullptr, nullptr, &err, 1.0, bk(k)))
       return strip::fail(STRIP_MLEM_ERR_HIP, err);
     HIP_TRY(hipMemcpyAsync(ck(k + 1), plan.rhoT[plan.curT], sizeof(float) * np,
                            hipMemcpyDeviceToDevice, ctx->stream));
@@ -781,7 +783,7 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
     return strip::fail(STRIP_MLEM_ERR_HIP, err);
   for (int k = n_iterations - 1; k >= 0; --k)
     if (strip::band_mlem_adjoint_step(plan, ctx->g, ctx->adjoint, ck(k),
-                                      gbarT, gl, ctx->stream, &err))
+                                      gbarT, gl, ctx->stream, &err, bk(k)))
       return strip::fail(STRIP_MLEM_ERR_HIP, err);
   // the context keeps rho_N as its image, as after n_iterations iterations
   if ((d_grad &&
@@ -1566,6 +1568,7 @@ struct strip_mlem_batch {
   strip::BandInstance* desc[2] = {nullptr, nullptr};  // image parity 0 / 1
   int4* fwd_tasks = nullptr;
   int4* bwd_tasks = nullptr;
+  int* queue = nullptr;  // task counters of the forward / back-projection
   int n_fwd = 0, n_bwd = 0;
   int H_fwd = 0, H_bwd = 0;
   int max_Wt = 0, bwd_waves = 1;
@@ -1598,7 +1601,9 @@ void batch_drop_merge(strip_mlem_batch* b) {
   }
   if (b->fwd_tasks) (void)hipFree(b->fwd_tasks);
   if (b->bwd_tasks) (void)hipFree(b->bwd_tasks);
+  if (b->queue) (void)hipFree(b->queue);
   b->fwd_tasks = b->bwd_tasks = nullptr;
+  b->queue = nullptr;
   b->n_fwd = b->n_bwd = 0;
   b->generations.clear();
 }
@@ -1672,6 +1677,15 @@ int batch_merge(strip_mlem_batch* b) {
     HIP_TRY(hipMemcpy(b->desc[c], c ? d1.data() : d0.data(),
                       sizeof(strip::BandInstance) * n, hipMemcpyHostToDevice));
   }
+  // the queues hand out the largest tasks first, so the launches end on
+  // small ones
+  auto by_size = [](const int4& a, const int4& c) {
+    return (int64_t)(a.z - a.y) * (a.w >> 16) >
+           (int64_t)(c.z - c.y) * (c.w >> 16);
+  };
+  std::stable_sort(fwd.begin(), fwd.end(), by_size);
+  std::stable_sort(bwd.begin(), bwd.end(), by_size);
+  HIP_TRY(hipMalloc(&b->queue, 2 * sizeof(int)));
   b->n_fwd = (int)fwd.size();
   b->n_bwd = (int)bwd.size();
   if (b->n_fwd) {
@@ -1724,10 +1738,11 @@ int batch_launch_iteration(strip_mlem_batch* b, int c) {
   const int n = (int)b->items.size();
   const int persistent = std::max(1, b->cu_count * b->grid_factor);
   const bool rec = b->tuning.recurrence != 0;
-  if (strip::band_launch_forward(b->desc[c], b->fwd_tasks, b->n_fwd, b->H_fwd,
+  if (strip::band_zero_ints(b->queue, 2, b->stream, &err) ||
+      strip::band_launch_forward(b->desc[c], b->fwd_tasks, b->n_fwd, b->H_fwd,
                                  b->spans, rec, b->fwd_lds,
-                                 std::min(b->n_fwd, persistent), b->stream,
-                                 &err) ||
+                                 std::min(b->n_fwd, persistent), b->queue,
+                                 b->stream, &err) ||
       strip::band_launch_batch_update(b->desc[c], n, b->max_events,
                                       b->max_pixels, b->H_fwd, true,
                                       b->stream, &err) ||
@@ -1735,7 +1750,7 @@ int batch_launch_iteration(strip_mlem_batch* b, int c) {
           b->desc[c], b->bwd_tasks, b->n_bwd, b->H_bwd, b->spans, rec,
           b->max_Wt * b->H_bwd, b->bwd_waves,
           std::min((b->n_bwd + b->bwd_waves - 1) / b->bwd_waves, persistent),
-          b->stream, &err) ||
+          b->queue + 1, b->stream, &err) ||
       strip::band_launch_batch_update(b->desc[c], n, b->max_events,
                                       b->max_pixels, b->H_fwd, false,
                                       b->stream, &err))
