@@ -186,8 +186,9 @@ int strip_mlem_set_backprojection_waves(strip_mlem_ctx* ctx, int waves);
 int strip_mlem_set_span_table(strip_mlem_ctx* ctx, int enable);
 int strip_mlem_get_span_table(strip_mlem_ctx* ctx, int* in_use);
 
-/* The plan in use: band rows of both kernels, task window width (columns)
- * and the number of (event, band) pairs of both kernels. */
+/* The plan in use: band rows of both kernels, the widest back-projection
+ * task window (columns) and the number of (event, band) pairs of both
+ * kernels. */
 int strip_mlem_get_band_tiling(strip_mlem_ctx* ctx, int* forward_rows,
                                int* backward_rows, int* window_columns,
                                int64_t* forward_pairs,

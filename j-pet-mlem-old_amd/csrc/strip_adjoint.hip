@@ -360,7 +360,7 @@ int launch_moments(BandPlan& plan, const DeviceGeometry& g, AdjointWork& wk,
                    std::string* err) {
   const PairList& L = plan.fwd;
   if (L.n_tasks == 0) return 0;
-  const size_t lds = (size_t)plan.Wt * L.H * sizeof(float) * NIMG;
+  const size_t lds = (size_t)L.Wt * L.H * sizeof(float) * NIMG;
   if (lds > 160 * 1024) {
     if (err) *err = "moment windows do not fit in LDS";
     return -1;
@@ -377,7 +377,7 @@ int launch_moments(BandPlan& plan, const DeviceGeometry& g, AdjointWork& wk,
                                (int)lds));
   hipLaunchKernelGGL(kern, dim3(L.n_tasks), dim3(64 * kFwdWaves), lds, stream,
                      g, plan.records, L.pair_event, L.tasks, plan.spans,
-                     plan.span_stride, x0T, x1T, plan.Wt, wk.moments,
+                     plan.span_stride, x0T, x1T, L.Wt, wk.moments,
                      L.max_bands);
   BAND_TRY(hipGetLastError());
   return 0;

@@ -34,19 +34,54 @@ __global__ __launch_bounds__(256) void count_pairs_kernel(
   counts[e] = c;
 }
 
+// Columns [lo, hi) reached by record e in band b of height H: the union of
+// the band rows' runs from the span table, or the bbox columns without one
+// (an empty slice gives the bbox start and one column).
+__device__ inline void band_columns(const BandRecord& r,
+                                    const uint16_t* __restrict__ spans,
+                                    int stride, int64_t e, int b, int H,
+                                    int& lo, int& hi) {
+  int x0, x1, y0, y1;
+  record_box(r, x0, x1, y0, y1);
+  if (!spans) {
+    lo = x0;
+    hi = x1;
+    return;
+  }
+  lo = x1;
+  hi = x0;
+  const int r0 = max(b * H, y0), r1 = min(b * H + H, y1);
+  for (int iy = r0; iy < r1; ++iy) {
+    const uint32_t w = spans[e * stride + (iy - y0)];
+    const int n = (int)(w >> 8);
+    if (n > 0) {
+      lo = min(lo, x0 + (int)(w & 0xff));
+      hi = max(hi, x0 + (int)(w & 0xff) + n);
+    }
+  }
+  if (hi <= lo) {
+    lo = x0;
+    hi = x0 + 1;
+  }
+}
+
+// keys of the event's pairs: band << 16 | first column the band's runs reach
 __global__ __launch_bounds__(256) void emit_pairs_kernel(
-    const BandRecord* __restrict__ rec, int64_t n, int H,
-    const int* __restrict__ counts, const int* __restrict__ offsets,
-    uint32_t* __restrict__ keys, int* __restrict__ values) {
+    const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
+    int stride, int64_t n, int H, const int* __restrict__ counts,
+    const int* __restrict__ offsets, uint32_t* __restrict__ keys,
+    int* __restrict__ values) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int c = counts[e];
   if (!c) return;
-  const uint32_t x0 = rec[e].bx & 0xffff;
-  const int y0 = rec[e].by & 0xffff;
+  const BandRecord r = rec[e];
+  const int b0 = (int)(r.by & 0xffff) / H;
   const int base = offsets[e];
   for (int k = 0; k < c; ++k) {
-    keys[base + k] = ((uint32_t)(y0 / H + k) << 16) | x0;
+    int lo, hi;
+    band_columns(r, spans, stride, e, b0 + k, H, lo, hi);
+    keys[base + k] = ((uint32_t)(b0 + k) << 16) | (uint32_t)lo;
     values[base + k] = (int)e;
   }
 }
@@ -138,15 +173,20 @@ __global__ __launch_bounds__(256) void group_bounds_kernel(
 // task.w |= (columns its pairs reach, counted from its first column) << 16,
 // at most Wt: one wavefront per task
 __global__ __launch_bounds__(256) void task_extent_kernel(
-    const BandRecord* __restrict__ rec, const int* __restrict__ pair_event,
+    const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
+    int stride, int H, const int* __restrict__ pair_event,
     int4* __restrict__ tasks, int n_tasks, int Wt) {
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= n_tasks) return;  // uniform over the wavefront
   const int lane = threadIdx.x & 63;
   const int4 task = tasks[t];
   int reach = 1;
-  for (int p = task.y + lane; p < task.z; p += 64)
-    reach = max(reach, (int)(rec[pair_event[p]].bx >> 16) - task.w);
+  for (int p = task.y + lane; p < task.z; p += 64) {
+    const int e = pair_event[p];
+    int lo, hi;
+    band_columns(rec[e], spans, stride, e, task.x & 0xffff, H, lo, hi);
+    reach = max(reach, hi - task.w);
+  }
   for (int o = 32; o > 0; o >>= 1) reach = max(reach, __shfl_xor(reach, o));
   if (lane == 0) tasks[t].w = task.w | (min(reach, Wt) << 16);
 }
@@ -285,7 +325,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
       const int col = c0 + i / H, iy = y_base + (i & (H - 1));
       win[i] = (col < g.n_z && iy < g.n_y)
-                   ? rhoT[(int64_t)col * g.n_y + iy]
+                   ? gload(rhoT + (int64_t)col * g.n_y + iy)
                    : 0.0f;
     }
     __syncthreads();
@@ -365,7 +405,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
       }
       const float dsum = wave_total(acc * s.pre);
       if (lane == 0)
-        partial[(int64_t)v.e * max_bands + (b - v.y0 / H)] = dsum;
+        gstore(partial + (int64_t)v.e * max_bands + (b - v.y0 / H), dsum);
     }
   }
   }
@@ -525,7 +565,7 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
     const float val = win[c * H + r];
     const int col = c0 + c;
     if (val != 0.0f && iy < g.n_y && col < g.n_z)
-      unsafeAtomicAdd(&bpT[(int64_t)col * g.n_y + iy], val);
+      gatomic_add(bpT + (int64_t)col * g.n_y + iy, val);
   }
   }
 }
@@ -674,7 +714,7 @@ int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
   }
   const bool sp = plan.spans != nullptr;
   const bool rec = plan.tuning.recurrence != 0;
-  const int window_words = plan.Wt * L.H;
+  const int window_words = L.Wt * L.H;
   const int grid = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
   switch (L.H) {
     case 32:
@@ -712,8 +752,9 @@ constexpr size_t kLdsLimit = 160 * 1024;
 // Sorted (event, band) pairs of band height H for the resident events, and
 // the index of the first pair of every (band, x0 >> shift) group.
 int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
-                const BandRecord* rec, int64_t n, hipStream_t stream,
-                std::vector<int>& first, std::string* err) {
+                const BandRecord* rec, const uint16_t* spans, int stride,
+                int64_t n, hipStream_t stream, std::vector<int>& first,
+                std::string* err) {
   L.H = H;
   L.n_bands = (g.n_y + H - 1) / H;
   const unsigned blocks = (unsigned)((n + 255) / 256);
@@ -768,7 +809,8 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
     BAND_TRY(hipMalloc(&L.pair_event, sizeof(int) * n_pairs));
     BAND_TRY(hipMalloc(&vals_alt, sizeof(int) * n_pairs));
     hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       rec, n, H, counts, offsets, keys, L.pair_event);
+                       rec, spans, stride, n, H, counts, offsets, keys,
+                       L.pair_event);
     BAND_TRY(hipGetLastError());
     int end_bit = 16;
     while ((1 << (end_bit - 16)) < L.n_bands + 1) ++end_bit;
@@ -810,10 +852,10 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
 }
 
 // Tasks of at most `chunk` consecutive pairs inside one group, each with the
-// column extent of its pairs' boxes.
+// columns its pairs reach; L.Wt = the widest task.
 int build_tasks(PairList& L, int shift, const std::vector<int>& first,
-                int chunk, const BandRecord* rec, int Wt, hipStream_t stream,
-                std::string* err) {
+                int chunk, const BandRecord* rec, const uint16_t* spans,
+                int stride, int Wt, hipStream_t stream, std::string* err) {
   const int n_groups = (int)first.size() - 1;
   std::vector<int4> tasks;
   std::vector<int> end(n_groups, 0);
@@ -836,13 +878,15 @@ int build_tasks(PairList& L, int shift, const std::vector<int>& first,
     BAND_TRY(hipMemcpyAsync(L.tasks, tasks.data(), sizeof(int4) * tasks.size(),
                             hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(task_extent_kernel, dim3((L.n_tasks + 3) / 4),
-                       dim3(256), 0, stream, rec, L.pair_event, L.tasks,
-                       L.n_tasks, Wt);
+                       dim3(256), 0, stream, rec, spans, stride, L.H,
+                       L.pair_event, L.tasks, L.n_tasks, Wt);
     BAND_TRY(hipGetLastError());
     BAND_TRY(hipMemcpyAsync(tasks.data(), L.tasks, sizeof(int4) * tasks.size(),
                             hipMemcpyDeviceToHost, stream));
     BAND_TRY(hipStreamSynchronize(stream));
   }
+  L.Wt = 1;
+  for (const int4& t : tasks) L.Wt = std::max(L.Wt, t.w >> 16);
   L.tasks_host = tasks;
   return 0;
 }
@@ -978,28 +1022,23 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   const BandTuning& tu = plan.tuning;
   const int bucket = (tu.bucket == 16 || tu.bucket == 64) ? tu.bucket : 32;
   const int shift = bucket == 16 ? 4 : (bucket == 64 ? 6 : 5);
+  // a task reaches at most its bucket plus the widest bbox (the bound used
+  // to pick band heights); its actual window is set by its pairs' runs
   plan.Wt = bucket + max_width;
   const int Hf = tu.fwd_rows == 32 ? 32 : 64;
   int Hb = tu.bwd_rows == 64 ? 64 : (tu.bwd_rows == 16 ? 16 : 32);
-  plan.fwd_lds = (size_t)plan.Wt * Hf * sizeof(float);
-  if (plan.fwd_lds > kLdsLimit) {
+  if ((size_t)plan.Wt * Hf * sizeof(float) > kLdsLimit) {
     if (err) *err = "task window does not fit in LDS";
     return -1;
   }
   if ((size_t)plan.Wt * Hb * sizeof(float) > kLdsLimit) Hb = 32;
-  const size_t window_bytes = (size_t)plan.Wt * Hb * sizeof(float);
-  plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
-  if (tu.bwd_waves > 0) plan.bwd_waves = std::min(plan.bwd_waves, tu.bwd_waves);
-  if (plan.bwd_waves < 1) {
-    if (err) *err = "back-projection window does not fit in LDS";
-    return -1;
-  }
-  plan.bwd_lds = window_bytes * plan.bwd_waves;
+  const uint16_t* sp = plan.spans;
+  const int ss = plan.span_stride;
   std::vector<int> first;
-  if (build_pairs(plan.fwd, Hf, shift, g, plan.records, n, stream, first,
-                  err))
+  if (build_pairs(plan.fwd, Hf, shift, g, plan.records, sp, ss, n, stream,
+                  first, err))
     return -1;
-  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, plan.records,
+  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, plan.records, sp, ss,
                   plan.Wt, stream, err))
     return -1;
   if (Hb == Hf) {
@@ -1009,13 +1048,22 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     plan.bwd.n_pairs = plan.fwd.n_pairs;
     plan.bwd.pair_event = plan.fwd.pair_event;
     plan.bwd.owns_pairs = false;
-  } else if (build_pairs(plan.bwd, Hb, shift, g, plan.records, n, stream,
-                         first, err)) {
+  } else if (build_pairs(plan.bwd, Hb, shift, g, plan.records, sp, ss, n,
+                         stream, first, err)) {
     return -1;
   }
-  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records,
+  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records, sp, ss,
                   plan.Wt, stream, err))
     return -1;
+  plan.fwd_lds = (size_t)plan.fwd.Wt * Hf * sizeof(float);
+  const size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
+  plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
+  if (tu.bwd_waves > 0) plan.bwd_waves = std::min(plan.bwd_waves, tu.bwd_waves);
+  if (plan.bwd_waves < 1) {
+    if (err) *err = "back-projection window does not fit in LDS";
+    return -1;
+  }
+  plan.bwd_lds = window_bytes * plan.bwd_waves;
   if (n > 0) {
     BAND_TRY(hipMalloc(&plan.partial,
                        sizeof(float) * n * plan.fwd.max_bands));

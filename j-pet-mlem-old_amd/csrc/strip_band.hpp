@@ -68,6 +68,7 @@ struct PairList {
   // (band | instance << 16, begin, end, first column | columns << 16)
   int4* tasks = nullptr;
   int n_tasks = 0;
+  int Wt = 1;                 // widest task window (columns)
   bool owns_pairs = true;
   std::vector<int4> tasks_host;  // host copy (instance 0)
 };
@@ -102,7 +103,7 @@ struct BandPlan {
   bool ready = false;
   BandTuning tuning;
   PairList fwd, bwd;
-  int Wt = 0;         // task window width in columns
+  int Wt = 0;         // bound on the task windows: bucket + widest bbox
   int bwd_waves = 4;  // wavefronts per back-projection workgroup
   int max_width = 0;  // widest clamped bbox (x1 - x0)
   int64_t n_events = 0;

@@ -15,6 +15,30 @@ namespace {
 
 constexpr int kFwdWaves = 8;  // wavefronts per forward workgroup
 
+// Pointers that a kernel reads from a descriptor in memory are generic, and
+// accesses through them compile to flat instructions, which count against
+// the LDS wait counter as well: every LDS wait would then also wait for the
+// outstanding global loads.  The hot accesses go through global-space
+// pointers instead (with native vector types: a copy of a HIP vector struct
+// through such a pointer falls back to flat loads).
+template <typename T>
+using global_t = __attribute__((address_space(1))) T;
+typedef float gfloat4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const global_t<T>*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v) {
+  *(global_t<T>*)p = v;
+}
+// relaxed, device scope, no return value
+__device__ __forceinline__ void gatomic_add(float* p, float v) {
+  __hip_atomic_fetch_add((global_t<float>*)p, v, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+}
+
 #define BAND_TRY(expr)                                              \
   do {                                                              \
     hipError_t _e = (expr);                                         \
@@ -135,7 +159,7 @@ __device__ inline uint32_t fetch_span(const uint16_t* __restrict__ spans,
                                       int stride, int e, uint32_t by, int iy) {
   const int y0 = (int)(by & 0xffff);
   const int k = min(max(iy - y0, 0), stride - 1);
-  return spans[(int64_t)e * stride + k];
+  return gload(spans + (int64_t)e * stride + k);
 }
 
 template <int CTRL>
@@ -179,17 +203,22 @@ __device__ inline PairBatch gather_batch(const BandRecord* __restrict__ rec,
                                          int first, int end, int lane) {
   PairBatch pb;
   const int p = first + lane;
-  pb.e = (p < end) ? pair_event[p] : 0;
-  const BandRecord r = rec[pb.e];
-  pb.cz = r.cz;
-  pb.cy = r.cy;
-  pb.A = r.A;
-  pb.B = r.B;
-  pb.P = r.P;
-  pb.Q0 = r.Q0;
-  pb.bx = r.bx;
-  pb.by = r.by;
-  pb.inv = WITH_INV ? inv[pb.e] : 0.0f;
+  pb.e = (p < end) ? gload(pair_event + p) : 0;
+  // the 32-byte record as two 16-byte loads: (cz, cy, A, B), (P, Q0, bx, by)
+  const gfloat4* r = reinterpret_cast<const gfloat4*>(rec + pb.e);
+  const gfloat4 lo = gload(r), hi = gload(r + 1);
+  pb.cz = lo.x;
+  pb.cy = lo.y;
+  pb.A = lo.z;
+  pb.B = lo.w;
+  pb.P = hi.x;
+  pb.Q0 = hi.y;
+  // (the elements are copied out first: a bit cast applied directly to a
+  // vector element expression reads element 0)
+  const float bx = hi.z, by = hi.w;
+  pb.bx = __builtin_bit_cast(uint32_t, bx);
+  pb.by = __builtin_bit_cast(uint32_t, by);
+  pb.inv = WITH_INV ? gload(inv + pb.e) : 0.0f;
   return pb;
 }
 

@@ -1368,7 +1368,7 @@ int strip_mlem_get_band_tiling(strip_mlem_ctx* ctx, int* forward_rows,
   if (rc) return rc;
   if (forward_rows) *forward_rows = ctx->band.fwd.H;
   if (backward_rows) *backward_rows = ctx->band.bwd.H;
-  if (window_columns) *window_columns = ctx->band.Wt;
+  if (window_columns) *window_columns = ctx->band.bwd.Wt;
   if (forward_pairs) *forward_pairs = ctx->band.fwd.n_pairs;
   if (backward_pairs) *backward_pairs = ctx->band.bwd.n_pairs;
   return STRIP_MLEM_OK;
@@ -1571,7 +1571,7 @@ struct strip_mlem_batch {
   int* queue = nullptr;  // task counters of the forward / back-projection
   int n_fwd = 0, n_bwd = 0;
   int H_fwd = 0, H_bwd = 0;
-  int max_Wt = 0, bwd_waves = 1;
+  int fwd_Wt = 0, bwd_Wt = 0, bwd_waves = 1;  // widest member windows
   size_t fwd_lds = 0;
   bool spans = true;
   int64_t max_events = 0, max_pixels = 0;
@@ -1626,7 +1626,7 @@ int batch_merge(strip_mlem_batch* b) {
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "at most 32767 members");
   std::vector<strip::BandInstance> d0(n), d1(n);
   std::vector<int4> fwd, bwd;
-  b->max_Wt = 1;
+  b->fwd_Wt = b->bwd_Wt = 1;
   b->spans = true;
   b->max_events = b->max_pixels = 0;
   for (int i = 0; i < n; ++i) {
@@ -1650,7 +1650,8 @@ int batch_merge(strip_mlem_batch* b) {
       return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
                          "members ended with different band heights");
     }
-    b->max_Wt = std::max(b->max_Wt, P.Wt);
+    b->fwd_Wt = std::max(b->fwd_Wt, P.fwd.Wt);
+    b->bwd_Wt = std::max(b->bwd_Wt, P.bwd.Wt);
     b->spans = b->spans && P.spans != nullptr;
     b->max_events = std::max<int64_t>(b->max_events, P.n_events);
     b->max_pixels = std::max<int64_t>(b->max_pixels, c->n_pixels);
@@ -1666,8 +1667,8 @@ int batch_merge(strip_mlem_batch* b) {
     }
     b->generations.push_back(P.generation);
   }
-  b->fwd_lds = sizeof(float) * (size_t)b->max_Wt * b->H_fwd;
-  const size_t window = sizeof(float) * (size_t)b->max_Wt * b->H_bwd;
+  b->fwd_lds = sizeof(float) * (size_t)b->fwd_Wt * b->H_fwd;
+  const size_t window = sizeof(float) * (size_t)b->bwd_Wt * b->H_bwd;
   if (b->fwd_lds > 160 * 1024 || window > 160 * 1024)
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
                        "a member's task window does not fit in LDS");
@@ -1748,7 +1749,7 @@ int batch_launch_iteration(strip_mlem_batch* b, int c) {
                                       b->stream, &err) ||
       strip::band_launch_backward(
           b->desc[c], b->bwd_tasks, b->n_bwd, b->H_bwd, b->spans, rec,
-          b->max_Wt * b->H_bwd, b->bwd_waves,
+          b->bwd_Wt * b->H_bwd, b->bwd_waves,
           std::min((b->n_bwd + b->bwd_waves - 1) / b->bwd_waves, persistent),
           b->queue + 1, b->stream, &err) ||
       strip::band_launch_batch_update(b->desc[c], n, b->max_events,
