@@ -159,7 +159,7 @@ int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx);
 
 /* Tile shapes of the row-band algorithm (the knobs an autotuner turns in
  * place of -B/-W): forward_rows / backward_rows = image rows per band of the
- * forward-projection (32 or 64) and back-projection (16, 32 or 64) kernels,
+ * forward-projection and back-projection kernels (16, 32 or 64),
  * bucket_columns = x0 bucket width bounding a task's column window (16, 32
  * or 64), forward_pairs / backward_pairs = (event, band) pairs per forward
  * workgroup / per back-projection wavefront.  Takes effect at the next
@@ -167,11 +167,21 @@ int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx);
 int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
                                int backward_rows, int bucket_columns,
                                int forward_pairs, int backward_pairs);
+/* (Event, band) pairs one wavefront of the forward / back-projection kernel
+ * processes together: 1, 2 or 4 (at most 64 / band rows).  Lane groups of
+ * 64 / G lanes then take one pair each, so one instruction stream serves G
+ * pairs (the back-projection keeps one LDS window per group; G is halved
+ * until they fit).  strip_mlem_get_band_groups returns the values in use. */
+int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
+                               int backward_group);
+int strip_mlem_get_band_groups(strip_mlem_ctx* ctx, int* forward_group,
+                               int* backward_group);
 /* How the row-band kernels evaluate exp(-Q/2) along an image row: 0 = one
  * hardware exp2 per pixel; 1 = the recurrence e[k+1] = e[k] r[k],
  * r[k+1] = r[k] G (exact for exp2 of a quadratic in the column), restarted
- * from exp2 every 8 pixels (4 in a back-projection tail).  Both stay within
- * the FP32 parity tolerance of the oracle. */
+ * from exp2 every 8 pixels (4 in a tail); the forward projection sums each
+ * block in Horner form, e[0] (rho[0] + r[0] (rho[1] + ...)).  Both stay
+ * within the FP32 parity tolerance of the oracle. */
 int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode);
 /* Wavefronts per back-projection workgroup (each owns an LDS window);
  * 0 = as many as fit, at most 4. */
@@ -272,6 +282,12 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* batch, int forward_rows,
                                 int forward_pairs, int backward_pairs,
                                 int grid_factor);
 int strip_mlem_batch_get_tiling(const strip_mlem_batch* batch, int* tiling);
+/* Shared pairs per wavefront step of the members' plans (as
+ * strip_mlem_set_band_groups). */
+int strip_mlem_batch_set_groups(strip_mlem_batch* batch, int forward_group,
+                                int backward_group);
+int strip_mlem_batch_get_groups(const strip_mlem_batch* batch,
+                                int* forward_group, int* backward_group);
 /* 1 (default) = replay graph-recorded iterations, 0 = launch each. */
 int strip_mlem_batch_set_graph(strip_mlem_batch* batch, int enable);
 /* Device time (ms) of the last strip_mlem_batch_iterate call. */

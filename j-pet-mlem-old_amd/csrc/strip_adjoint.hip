@@ -370,8 +370,9 @@ int launch_moments(BandPlan& plan, const DeviceGeometry& g, AdjointWork& wk,
     return plan.spans ? band_moments_kernel<H, NIMG, true>
                       : band_moments_kernel<H, NIMG, false>;
   };
-  auto kern = L.H == 32 ? pick_kernel(std::integral_constant<int, 32>{})
-                        : pick_kernel(std::integral_constant<int, 64>{});
+  auto kern = L.H == 16   ? pick_kernel(std::integral_constant<int, 16>{})
+              : L.H == 32 ? pick_kernel(std::integral_constant<int, 32>{})
+                          : pick_kernel(std::integral_constant<int, 64>{});
   BAND_TRY(hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)lds));

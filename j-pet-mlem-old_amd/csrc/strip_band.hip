@@ -281,17 +281,28 @@ __global__ __launch_bounds__(256) void band_inverse_kernel(
   if (inv) inv[e] = guard ? ((d > 0.0f) ? 1.0f / d : 0.0f) : 1.0f / d;
 }
 
-template <int H, int MODE, bool REC, bool SPANS>
+// Forward projection.  Workgroup = one task (pairs of one band and one x0
+// bucket); its rho window of Wt columns from c0 lives in LDS column-major
+// (word = (col - c0) * H + row; lanes of a column touch distinct rows, hence
+// distinct banks).  Wave w takes a contiguous share of the task's pairs, G
+// pairs per step: lane = g*(H*LR) + q*H + r covers row r of pair g of the
+// step and every LR-th column of the row's span from offset q (G = 64 /
+// (H*LR); with G > 1 the lane groups read their pairs' fields through the
+// LDS crossbar, so one instruction stream serves G pairs).
+// MODE 0: partial D per pair; MODE 1: count the span pixels instead.
+template <int H, int LR, int MODE, bool REC, bool SPANS>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
     int n_tasks, int* __restrict__ queue,
     unsigned long long* __restrict__ count) {
+  constexpr int G = 64 / (H * LR);
+  static_assert(G * H * LR == 64, "lane layout");
   extern __shared__ float win[];
   __shared__ int queued;
-  constexpr int LR = 64 / H;  // lanes per image row
   const int lane = threadIdx.x & 63;
   const int r = lane & (H - 1);
-  const int q = lane / H;
+  const int q = (lane / H) % LR;
+  const int grp = lane / (H * LR);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   unsigned long long counted = 0;
   // Resident workgroups walk the task list: from a queue counter (tasks
@@ -317,27 +328,29 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   const int max_bands = d.fwd_max_bands;
   const int c0 = task.w & 0xffff;
   const int y_base = b * H;
+  const int iy = y_base + r;
   if (MODE == 0) {
     const float* __restrict__ rhoT = d.rhoT;
     // the columns the task's pairs reach
     const int n_words = (task.w >> 16) * H;
     __syncthreads();  // the previous task's window is no longer read
     for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
-      const int col = c0 + i / H, iy = y_base + (i & (H - 1));
-      win[i] = (col < g.n_z && iy < g.n_y)
-                   ? gload(rhoT + (int64_t)col * g.n_y + iy)
+      const int col = c0 + i / H, row = y_base + (i & (H - 1));
+      win[i] = (col < g.n_z && row < g.n_y)
+                   ? gload(rhoT + (int64_t)col * g.n_y + row)
                    : 0.0f;
     }
     __syncthreads();
   }
   const float step = (float)LR * g.pixel_width;
   const int total = task.z - task.y;
-  const int share = (total + kFwdWaves - 1) / kFwdWaves;
+  // shares of whole steps
+  const int share = ((total + kFwdWaves * G - 1) / (kFwdWaves * G)) * G;
   const int begin = task.y + wave * share;
   const int end = min(task.z, begin + share);
   const float qc = g.C * kExp2Scale;
   const float two_qc_step = 2.0f * qc * step;
-  const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
+  const float G2 = __builtin_amdgcn_exp2f(two_qc_step * step);
   PairBatch next = gather_batch<false>(rec, pair_event, nullptr, begin, end,
                                        lane);
   for (int base = begin; base < end; base += 64) {
@@ -345,31 +358,37 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     if (base + 64 < end)
       next = gather_batch<false>(rec, pair_event, nullptr, base + 64, end, lane);
     const int n_here = min(64, end - base);
-    // the next pair's span word is in flight while one pair is processed
+    // the next step's pair and span word are in flight while one step is
+    // processed
+    PairView nv = step_pair<G, false>(cur, 0, grp);
     uint32_t sp = 0;
-    if (SPANS)
-      sp = fetch_span(spans, span_stride, lane_int(cur.e, 0),
-                      (uint32_t)lane_int((int)cur.by, 0), y_base + r);
-    for (int j = 0; j < n_here; ++j) {
-      const PairView v = pick(cur, j);
+    if (SPANS) sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
+    for (int j = 0; j < n_here; j += G) {
+      const PairView v = nv;
+      const bool valid = G == 1 || j + grp < n_here;
       RowSpan s;
       if (SPANS) {
-        const int iy = y_base + r;
-        const uint32_t word = (iy >= v.y0 && iy < v.y1) ? sp : 0u;
+        const uint32_t word = (valid && iy >= v.y0 && iy < v.y1) ? sp : 0u;
         // the word is consumed before the next load reuses its register, so
-        // that load stays in flight through this pair
+        // that load stays in flight through this step
         __builtin_amdgcn_sched_barrier(0);
-        const int j1 = min(j + 1, n_here - 1);
-        sp = fetch_span(spans, span_stride, lane_int(cur.e, j1),
-                        (uint32_t)lane_int((int)cur.by, j1), y_base + r);
+        const int jn = G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G);
+        nv = step_pair<G, false>(cur, jn, grp);
+        sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
         s = stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy);
       } else {
+        if (G > 1) nv = step_pair<G, false>(cur, min(j + G, 64 - G), grp);
         s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0, v.x1, v.y0,
-                     v.y1, y_base + r);
+                     v.y1, iy);
+        if (!valid) s.n = 0;
       }
+      if (G == 1 && !SPANS) nv = pick(cur, min(j + 1, n_here - 1));
       const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
       if (MODE == 1) {
-        counted += (unsigned long long)wave_sum_int(q == 0 ? s.n : 0);
+        if (G == 1)
+          counted += (unsigned long long)wave_sum_int(q == 0 ? s.n : 0);
+        else
+          counted += (unsigned long long)(q == 0 ? s.n : 0);
         continue;
       }
       const int col0 = s.lo + q;
@@ -381,20 +400,47 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
       float acc = 0.0f;
       int k = 0;
       if (REC) {
-        // exp2 of a quadratic along the row: e[k+1] = e[k] r[k],
-        // r[k+1] = r[k] G, restarted from exp2 every 8 columns
+        // exp2 of a quadratic along the row: e[j+1] = e[j] r[j] with
+        // r[j] = r[0] G2^j, so a block's sum is Horner's scheme
+        //   sum_j rho[j] e[j] = e[0] (rho[0] + r[0] (rho[1] + ... r[6] rho[7]))
+        // with r[j] taken downward from r[6] (r[j-1] = r[j] / G2): two
+        // operations per column; e[0] and r[6] from exp2 in every block of 8
+        // (4 in a tail)
         const float db = __builtin_fmaf(qc * step, step, qb * step);
+        const float Ginv = __builtin_amdgcn_exp2f(-two_qc_step * step);
+        const float db6 = __builtin_fmaf(6.0f * two_qc_step, step, db);
+        const float db2 = __builtin_fmaf(2.0f * two_qc_step, step, db);
+        constexpr int S = LR * H;
         for (; k + 8 <= nq; k += 8) {
-          float e = __builtin_amdgcn_exp2f(
+          const float e0 = __builtin_amdgcn_exp2f(
               __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
-          float rr = __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db));
+          float rr =
+              __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db6));
+          const float* p = src + k * S;
+          float t = p[7 * S];
 #pragma unroll
-          for (int kk = 0; kk < 8; ++kk) {
-            acc = __builtin_fmaf(e, src[(k + kk) * LR * H], acc);
-            e *= rr;
-            rr *= G;
+          for (int i = 6; i >= 0; --i) {
+            t = __builtin_fmaf(rr, t, p[i * S]);
+            if (i) rr *= Ginv;
           }
+          acc = __builtin_fmaf(e0, t, acc);
           dz += 8.0f * step;
+        }
+        if (k + 4 <= nq) {
+          const float e0 = __builtin_amdgcn_exp2f(
+              __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
+          float rr =
+              __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db2));
+          const float* p = src + k * S;
+          float t = p[3 * S];
+#pragma unroll
+          for (int i = 2; i >= 0; --i) {
+            t = __builtin_fmaf(rr, t, p[i * S]);
+            if (i) rr *= Ginv;
+          }
+          acc = __builtin_fmaf(e0, t, acc);
+          dz += 4.0f * step;
+          k += 4;
         }
       }
       for (; k < nq; ++k) {
@@ -403,30 +449,47 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
                              acc);
         dz += step;
       }
-      const float dsum = wave_total(acc * s.pre);
-      if (lane == 0)
-        gstore(partial + (int64_t)v.e * max_bands + (b - v.y0 / H), dsum);
+      if constexpr (G == 1) {
+        const float dsum = wave_total(acc * s.pre);
+        if (lane == 0)
+          gstore(partial + (int64_t)v.e * max_bands + (b - v.y0 / H), dsum);
+      } else {
+        const float dsum = group_total<H * LR>(acc * s.pre);
+        if ((lane & (H * LR - 1)) == 0 && valid)
+          gstore(partial + (int64_t)v.e * max_bands + (b - v.y0 / H), dsum);
+      }
     }
   }
   }
-  if (MODE == 1 && lane == 0 && counted) atomicAdd(count, counted);
+  if (MODE == 1) {
+    if (G > 1)
+      for (int o = 32; o > 0; o >>= 1) counted += __shfl_xor(counted, o);
+    if (lane == 0 && counted) atomicAdd(count, counted);
+  }
 }
 
-// Back-projection.  Each wavefront takes one task and owns a private window
-// of Wt columns x H rows in LDS (word = (col - c0) * H + row): every lane
-// writes only its own (row, column) words, and no other wave sees the
-// window, so plain read-modify-write suffices.  Four columns are read before
-// any is written (distinct words).  The window is then added to the
+// Back-projection.  Each wavefront takes one task and owns G private
+// windows of Wt columns x H rows in LDS (word = (col - c0) * H + row), one
+// per lane group: every lane writes only its own (row, column) words of its
+// group's window, and no other wave sees them, so plain read-modify-write
+// suffices.  A step takes G pairs, lane = g*(H*LR) + q*H + r as in the
+// forward projection.  The windows are then summed and added to the
 // column-major image.
-template <int H, bool REC, bool SPANS>
+template <int H, int LR, bool REC, bool SPANS>
 __global__ __launch_bounds__(256) void band_backward_kernel(
     const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
     int n_tasks, int* __restrict__ queue, int window_words) {
+  constexpr int G = 64 / (H * LR);
+  static_assert(G * H * LR == 64, "lane layout");
   extern __shared__ float windows[];
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int n_waves = blockDim.x >> 6;
-  float* win = windows + wave * window_words;
+  const int r = lane & (H - 1);
+  const int q = (lane / H) % LR;
+  const int grp = lane / (H * LR);
+  float* const wave_win = windows + wave * G * window_words;
+  float* const win = wave_win + grp * window_words;
   // each wavefront walks the task list: from a queue counter or in grid
   // strides of the launch's wavefronts
   for (int t = blockIdx.x * n_waves + wave;;) {
@@ -448,17 +511,16 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
   const float* __restrict__ inv = d.inv;
   float* __restrict__ bpT = d.bpT;
   const int Wt = task.w >> 16;  // the columns the task's pairs reach
-  for (int i = lane; i < Wt * H; i += 64) win[i] = 0.0f;
+  // each lane group clears its own window
+  for (int i = lane & (H * LR - 1); i < Wt * H; i += H * LR) win[i] = 0.0f;
   const int c0 = task.w & 0xffff;
   const int y_base = b * H;
-  constexpr int LR = 64 / H;  // lanes per image row
-  constexpr int S = LR * H;   // word stride between a lane's columns
-  const int r = lane & (H - 1);
-  const int q = lane / H;
+  const int iy = y_base + r;
+  constexpr int S = LR * H;  // word stride between a lane's columns
   const float step = (float)LR * g.pixel_width;
   const float qc = g.C * kExp2Scale;
   const float two_qc_step = 2.0f * qc * step;
-  const float G = __builtin_amdgcn_exp2f(two_qc_step * step);
+  const float G2 = __builtin_amdgcn_exp2f(two_qc_step * step);
   PairBatch next = gather_batch<true>(rec, pair_event, inv, task.y, task.z,
                                       lane);
   for (int base = task.y; base < task.z; base += 64) {
@@ -466,29 +528,31 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
     if (base + 64 < task.z)
       next = gather_batch<true>(rec, pair_event, inv, base + 64, task.z, lane);
     const int n_here = min(64, task.z - base);
+    PairView nv = step_pair<G, true>(cur, 0, grp);
     uint32_t sp = 0;
-    if (SPANS)
-      sp = fetch_span(spans, span_stride, lane_int(cur.e, 0),
-                      (uint32_t)lane_int((int)cur.by, 0), y_base + r);
-    for (int j = 0; j < n_here; ++j) {
-      const PairView v = pick(cur, j);
-      const int iy = y_base + r;
+    if (SPANS) sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
+    for (int j = 0; j < n_here; j += G) {
+      const PairView v = nv;
+      const bool valid = G == 1 || j + grp < n_here;
       uint32_t word = 0;
       if (SPANS) {
         // an event with inv = 0 adds nothing: empty run; the word is
         // consumed before the next load reuses its register
-        word = (iy >= v.y0 && iy < v.y1 && v.inv != 0.0f) ? sp : 0u;
+        word = (valid && iy >= v.y0 && iy < v.y1 && v.inv != 0.0f) ? sp : 0u;
         __builtin_amdgcn_sched_barrier(0);
-        const int j1 = min(j + 1, n_here - 1);
-        sp = fetch_span(spans, span_stride, lane_int(cur.e, j1),
-                        (uint32_t)lane_int((int)cur.by, j1), y_base + r);
-      } else if (v.inv == 0.0f) {
-        continue;
+        const int jn = G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G);
+        nv = step_pair<G, true>(cur, jn, grp);
+        sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
+      } else {
+        nv = step_pair<G, true>(
+            cur, G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G), grp);
+        if (G == 1 && v.inv == 0.0f) continue;
       }
-      const RowSpan s =
+      RowSpan s =
           SPANS ? stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy)
                 : row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0, v.x1,
-                           v.y0, v.y1, y_base + r);
+                           v.y0, v.y1, iy);
+      if (!SPANS && G > 1 && (!valid || v.inv == 0.0f)) s.n = 0;
       const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
       const int col0 = s.lo + q;
       float dz = __builtin_fmaf((float)col0, g.pixel_width,
@@ -499,7 +563,7 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
       float* w = win + (col0 - c0) * H + r;
       int k = 0;
       if (REC) {
-        // e[k+1] = e[k] r[k], r[k+1] = r[k] G, restarted every 8 columns
+        // e[k+1] = e[k] r[k], r[k+1] = r[k] G2, restarted every 8 columns
         // (4 for a tail): the block's words are all read before any is
         // written, so eight reads are in flight together
         const float db = __builtin_fmaf(qc * step, step, qb * step);
@@ -515,7 +579,7 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
           for (int i = 0; i < 8; ++i) {
             w[i * S] = __builtin_fmaf(e, scale, a[i]);
             e *= rr;
-            rr *= G;
+            rr *= G2;
           }
           dz += 8.0f * step;
           w += 8 * S;
@@ -526,8 +590,8 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
               __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
           const float r0 =
               __builtin_amdgcn_exp2f(__builtin_fmaf(two_qc_step, dz, db));
-          const float e1 = e0 * r0, r1 = r0 * G;
-          const float e2 = e1 * r1, r2 = r1 * G;
+          const float e1 = e0 * r0, r1 = r0 * G2;
+          const float e2 = e1 * r1, r2 = r1 * G2;
           const float e3 = e2 * r2;
           w[0] = __builtin_fmaf(e0, scale, a0);
           w[S] = __builtin_fmaf(e1, scale, a1);
@@ -559,10 +623,14 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
       }
     }
   }
-  // flush: lane = q*H + r adds row r of columns c0+q, c0+q+64/H, ...
-  const int iy = y_base + r;
-  for (int c = q; c < Wt; c += LR) {
-    const float val = win[c * H + r];
+  // flush: lane = q'*H + r adds row r of columns c0+q', c0+q'+64/H, ...,
+  // summed over the wave's G windows
+  constexpr int LC = 64 / H;
+  const int qc0 = lane / H;
+  for (int c = qc0; c < Wt; c += LC) {
+    float val = wave_win[c * H + r];
+#pragma unroll
+    for (int k = 1; k < G; ++k) val += wave_win[k * window_words + c * H + r];
     const int col = c0 + c;
     if (val != 0.0f && iy < g.n_y && col < g.n_z)
       gatomic_add(bpT + (int64_t)col * g.n_y + iy, val);
@@ -621,23 +689,23 @@ __global__ __launch_bounds__(256) void batch_finalize_kernel(
   }
 }
 
-template <int H>
-int launch_forward_h(const BandInstance* desc, const int4* tasks, int n_tasks,
+template <int H, int LR>
+int launch_forward_t(const BandInstance* desc, const int4* tasks, int n_tasks,
                      bool sp, bool recurrence, int mode, size_t lds, int grid,
                      int* queue, unsigned long long* count,
                      hipStream_t stream, std::string* err) {
-  if (n_tasks == 0) return 0;
   if (mode == 0) {
-    auto kern = recurrence ? (sp ? band_forward_kernel<H, 0, true, true>
-                                 : band_forward_kernel<H, 0, true, false>)
-                           : (sp ? band_forward_kernel<H, 0, false, true>
-                                 : band_forward_kernel<H, 0, false, false>);
+    auto kern =
+        recurrence ? (sp ? band_forward_kernel<H, LR, 0, true, true>
+                         : band_forward_kernel<H, LR, 0, true, false>)
+                   : (sp ? band_forward_kernel<H, LR, 0, false, true>
+                         : band_forward_kernel<H, LR, 0, false, false>);
     if (set_lds(kern, lds, err)) return -1;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kFwdWaves), lds, stream,
                        desc, tasks, n_tasks, queue, count);
   } else {
-    auto kern = sp ? band_forward_kernel<H, 1, false, true>
-                   : band_forward_kernel<H, 1, false, false>;
+    auto kern = sp ? band_forward_kernel<H, LR, 1, false, true>
+                   : band_forward_kernel<H, LR, 1, false, false>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * kFwdWaves), 0, stream,
                        desc, tasks, n_tasks, queue, count);
   }
@@ -645,22 +713,68 @@ int launch_forward_h(const BandInstance* desc, const int4* tasks, int n_tasks,
   return 0;
 }
 
-template <int H>
-int launch_backward_h(const BandInstance* desc, const int4* tasks,
+template <int H, int LR>
+int launch_backward_t(const BandInstance* desc, const int4* tasks,
                       int n_tasks, bool sp, bool recurrence, int window_words,
                       int waves, int grid, int* queue, hipStream_t stream,
                       std::string* err) {
-  if (n_tasks == 0) return 0;
-  auto kern = recurrence ? (sp ? band_backward_kernel<H, true, true>
-                               : band_backward_kernel<H, true, false>)
-                         : (sp ? band_backward_kernel<H, false, true>
-                               : band_backward_kernel<H, false, false>);
-  const size_t lds = sizeof(float) * (size_t)window_words * waves;
+  auto kern = recurrence ? (sp ? band_backward_kernel<H, LR, true, true>
+                               : band_backward_kernel<H, LR, true, false>)
+                         : (sp ? band_backward_kernel<H, LR, false, true>
+                               : band_backward_kernel<H, LR, false, false>);
+  constexpr int G = 64 / (H * LR);
+  const size_t lds = sizeof(float) * (size_t)window_words * G * waves;
   if (set_lds(kern, lds, err)) return -1;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), lds, stream, desc,
                      tasks, n_tasks, queue, window_words);
   BAND_TRY(hipGetLastError());
   return 0;
+}
+
+// (band rows, pairs per step) of the forward projection
+int launch_forward_any(int H, int G, const BandInstance* desc,
+                       const int4* tasks, int n_tasks, bool sp,
+                       bool recurrence, int mode, size_t lds, int grid,
+                       int* queue, unsigned long long* count,
+                       hipStream_t stream, std::string* err) {
+  if (n_tasks == 0) return 0;
+#define STRIP_FWD(HH, GG)                                                  \
+  if (H == HH && G == GG)                                                  \
+    return launch_forward_t<HH, 64 / (HH * GG)>(desc, tasks, n_tasks, sp,  \
+                                                recurrence, mode, lds, grid, \
+                                                queue, count, stream, err);
+  STRIP_FWD(64, 1)
+  STRIP_FWD(32, 1)
+  STRIP_FWD(32, 2)
+  STRIP_FWD(16, 1)
+  STRIP_FWD(16, 2)
+  STRIP_FWD(16, 4)
+#undef STRIP_FWD
+  if (err) *err = "unsupported forward band rows / pairs per step";
+  return -1;
+}
+
+// (band rows, pairs per step) of the back-projection
+int launch_backward_any(int H, int G, const BandInstance* desc,
+                        const int4* tasks, int n_tasks, bool sp,
+                        bool recurrence, int window_words, int waves,
+                        int grid, int* queue, hipStream_t stream,
+                        std::string* err) {
+  if (n_tasks == 0) return 0;
+#define STRIP_BWD(HH, GG)                                                 \
+  if (H == HH && G == GG)                                                 \
+    return launch_backward_t<HH, 64 / (HH * GG)>(                         \
+        desc, tasks, n_tasks, sp, recurrence, window_words, waves, grid,  \
+        queue, stream, err);
+  STRIP_BWD(16, 1)
+  STRIP_BWD(16, 2)
+  STRIP_BWD(16, 4)
+  STRIP_BWD(32, 1)
+  STRIP_BWD(32, 2)
+  STRIP_BWD(64, 1)
+#undef STRIP_BWD
+  if (err) *err = "unsupported back-projection band rows / pairs per step";
+  return -1;
 }
 
 int upload_slot(BandPlan& plan, int slot, hipStream_t stream,
@@ -679,18 +793,9 @@ int dispatch_forward(BandPlan& plan, const DeviceGeometry& g, int mode,
   const bool sp = plan.spans != nullptr;
   const bool rec = plan.tuning.recurrence != 0;
   const BandInstance* desc = plan.desc + plan.curT;
-  switch (L.H) {
-    case 32:
-      return launch_forward_h<32>(desc, L.tasks, L.n_tasks, sp, rec, mode,
-                                  plan.fwd_lds, L.n_tasks, nullptr, count,
-                                  stream, err);
-    case 64:
-      return launch_forward_h<64>(desc, L.tasks, L.n_tasks, sp, rec, mode,
-                                  plan.fwd_lds, L.n_tasks, nullptr, count,
-                                  stream, err);
-  }
-  if (err) *err = "unsupported band height";
-  return -1;
+  return launch_forward_any(L.H, plan.fwd_group, desc, L.tasks, L.n_tasks, sp,
+                            rec, mode, plan.fwd_lds, L.n_tasks, nullptr,
+                            count, stream, err);
 }
 
 int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
@@ -716,22 +821,9 @@ int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
   const bool rec = plan.tuning.recurrence != 0;
   const int window_words = L.Wt * L.H;
   const int grid = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
-  switch (L.H) {
-    case 32:
-      return launch_backward_h<32>(plan.desc + slot, L.tasks, L.n_tasks, sp,
-                                   rec, window_words, plan.bwd_waves, grid,
-                                   nullptr, stream, err);
-    case 64:
-      return launch_backward_h<64>(plan.desc + slot, L.tasks, L.n_tasks, sp,
-                                   rec, window_words, plan.bwd_waves, grid,
-                                   nullptr, stream, err);
-    case 16:
-      return launch_backward_h<16>(plan.desc + slot, L.tasks, L.n_tasks, sp,
-                                   rec, window_words, plan.bwd_waves, grid,
-                                   nullptr, stream, err);
-  }
-  if (err) *err = "unsupported band height";
-  return -1;
+  return launch_backward_any(L.H, plan.bwd_group, plan.desc + slot, L.tasks,
+                             L.n_tasks, sp, rec, window_words, plan.bwd_waves,
+                             grid, nullptr, stream, err);
 }
 
 template <typename T>
@@ -1025,7 +1117,7 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   // a task reaches at most its bucket plus the widest bbox (the bound used
   // to pick band heights); its actual window is set by its pairs' runs
   plan.Wt = bucket + max_width;
-  const int Hf = tu.fwd_rows == 32 ? 32 : 64;
+  const int Hf = tu.fwd_rows == 16 ? 16 : (tu.fwd_rows == 32 ? 32 : 64);
   int Hb = tu.bwd_rows == 64 ? 64 : (tu.bwd_rows == 16 ? 16 : 32);
   if ((size_t)plan.Wt * Hf * sizeof(float) > kLdsLimit) {
     if (err) *err = "task window does not fit in LDS";
@@ -1055,8 +1147,18 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records, sp, ss,
                   plan.Wt, stream, err))
     return -1;
+  // pairs per wavefront step: as asked where the lane layout allows it
+  // (G * H <= 64)
+  plan.fwd_group = std::max(1, std::min(tu.fwd_group, 64 / Hf));
+  plan.bwd_group = std::max(1, std::min(tu.bwd_group, 64 / Hb));
+  if (plan.fwd_group == 3) plan.fwd_group = 2;
+  if (plan.bwd_group == 3) plan.bwd_group = 2;
   plan.fwd_lds = (size_t)plan.fwd.Wt * Hf * sizeof(float);
-  const size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
+  // back-projection: G windows per wavefront
+  size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
+  while (plan.bwd_group > 1 && window_bytes * plan.bwd_group > kLdsLimit)
+    plan.bwd_group /= 2;
+  window_bytes *= plan.bwd_group;
   plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
   if (tu.bwd_waves > 0) plan.bwd_waves = std::min(plan.bwd_waves, tu.bwd_waves);
   if (plan.bwd_waves < 1) {
@@ -1116,19 +1218,11 @@ BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
 }
 
 int band_launch_forward(const BandInstance* desc, const int4* tasks,
-                        int n_tasks, int H, bool spans, bool recurrence,
+                        int n_tasks, int H, int G, bool spans, bool recurrence,
                         size_t lds, int grid, int* queue, hipStream_t stream,
                         std::string* err) {
-  switch (H) {
-    case 32:
-      return launch_forward_h<32>(desc, tasks, n_tasks, spans, recurrence, 0,
-                                  lds, grid, queue, nullptr, stream, err);
-    case 64:
-      return launch_forward_h<64>(desc, tasks, n_tasks, spans, recurrence, 0,
-                                  lds, grid, queue, nullptr, stream, err);
-  }
-  if (err) *err = "unsupported band height";
-  return -1;
+  return launch_forward_any(H, G, desc, tasks, n_tasks, spans, recurrence, 0,
+                            lds, grid, queue, nullptr, stream, err);
 }
 
 int band_launch_batch_update(const BandInstance* desc, int n_instances,
@@ -1152,25 +1246,12 @@ int band_launch_batch_update(const BandInstance* desc, int n_instances,
 }
 
 int band_launch_backward(const BandInstance* desc, const int4* tasks,
-                         int n_tasks, int H, bool spans, bool recurrence,
-                         int window_words, int waves, int grid, int* queue,
-                         hipStream_t stream, std::string* err) {
-  switch (H) {
-    case 32:
-      return launch_backward_h<32>(desc, tasks, n_tasks, spans, recurrence,
-                                   window_words, waves, grid, queue, stream,
-                                   err);
-    case 64:
-      return launch_backward_h<64>(desc, tasks, n_tasks, spans, recurrence,
-                                   window_words, waves, grid, queue, stream,
-                                   err);
-    case 16:
-      return launch_backward_h<16>(desc, tasks, n_tasks, spans, recurrence,
-                                   window_words, waves, grid, queue, stream,
-                                   err);
-  }
-  if (err) *err = "unsupported band height";
-  return -1;
+                         int n_tasks, int H, int G, bool spans,
+                         bool recurrence, int window_words, int waves,
+                         int grid, int* queue, hipStream_t stream,
+                         std::string* err) {
+  return launch_backward_any(H, G, desc, tasks, n_tasks, spans, recurrence,
+                             window_words, waves, grid, queue, stream, err);
 }
 
 int band_load_images(BandPlan& plan, const DeviceGeometry& g,

@@ -31,17 +31,22 @@
 namespace strip {
 
 struct BandTuning {
-  int fwd_rows = 64;     // forward band height: 64 or 32
+  int fwd_rows = 32;     // forward band height: 64, 32 or 16
   int bwd_rows = 16;     // back-projection band height: 64, 32 or 16
   int bwd_waves = 0;     // wavefronts per back-projection workgroup (0 = as
                          // many as fit, at most 4)
   int bucket = 16;       // x0 bucket width in columns: 16, 32 or 64
   int fwd_pairs = 2048;  // pairs per forward workgroup task
   int bwd_pairs = 512;   // pairs per back-projection wavefront task
+  // pairs a wavefront processes together (1, 2 or 4, at most 64 / rows):
+  // lane groups of 64 / G lanes take one pair each, with 64 / (G * rows)
+  // lanes per image row
+  int fwd_group = 2;
+  int bwd_group = 2;
   // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
   // two-term recurrence e[k+1] = e[k] r[k], r[k+1] = r[k] G of exp2 of a
-  // quadratic, restarted from exp2 every 8 pixels (4 in a back-projection
-  // tail)
+  // quadratic, restarted from exp2 every 8 pixels (4 in a tail); the forward
+  // projection sums each block in Horner form with r[k] taken downward
   int recurrence = 1;
   // 1 = build the span table when it fits, 0 = solve every row's run in the
   // iteration kernels
@@ -105,6 +110,8 @@ struct BandPlan {
   PairList fwd, bwd;
   int Wt = 0;         // bound on the task windows: bucket + widest bbox
   int bwd_waves = 4;  // wavefronts per back-projection workgroup
+  int fwd_group = 1;  // pairs per wavefront step in use
+  int bwd_group = 1;
   int max_width = 0;  // widest clamped bbox (x1 - x0)
   int64_t n_events = 0;
   // events renumbered in (bbox y0, x0) order so neighbouring pairs of a task
@@ -143,14 +150,17 @@ BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
 // wavefronts per workgroup.
 // queue (may be null): a device counter, zero at launch, from which the
 // resident workgroups (forward) / wavefronts (back-projection) take tasks.
+// H = band rows, G = pairs per wavefront step; the back-projection uses G
+// windows of window_words per wavefront.
 int band_launch_forward(const BandInstance* desc, const int4* tasks,
-                        int n_tasks, int H, bool spans, bool recurrence,
+                        int n_tasks, int H, int G, bool spans, bool recurrence,
                         size_t lds, int grid, int* queue, hipStream_t stream,
                         std::string* err);
 int band_launch_backward(const BandInstance* desc, const int4* tasks,
-                         int n_tasks, int H, bool spans, bool recurrence,
-                         int window_words, int waves, int grid, int* queue,
-                         hipStream_t stream, std::string* err);
+                         int n_tasks, int H, int G, bool spans,
+                         bool recurrence, int window_words, int waves,
+                         int grid, int* queue, hipStream_t stream,
+                         std::string* err);
 
 // Zeroes n ints on the stream with a kernel (graph-recordable).
 int band_zero_ints(int* p, int n, hipStream_t stream, std::string* err);

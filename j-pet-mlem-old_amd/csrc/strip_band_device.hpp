@@ -255,12 +255,54 @@ __device__ inline PairView pick(const PairBatch& pb, int j) {
   return v;
 }
 
-// Forward projection.  Workgroup = one task (pairs of one band and one x0
-// bucket); its rho window of Wt columns from c0 lives in LDS column-major
-// (word = (col - c0) * H + row; lanes of a column touch distinct rows, hence
-// distinct banks).  Wave w takes a contiguous share of the task's pairs;
-// lane = q*H + r covers row r of the band and every (64/H)-th column of the
-// row's span from offset q.
-// MODE 0: partial D per pair; MODE 1: count the span pixels instead.
+// The fields of batch lane src (which may differ between lanes) for this
+// lane, through the LDS crossbar (ds_bpermute; no LDS memory is touched).
+template <bool WITH_INV>
+__device__ inline PairView pick_lane(const PairBatch& pb, int src) {
+  const int a = src << 2;
+  auto mv = [a](float x) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_ds_bpermute(a, __builtin_bit_cast(int, x)));
+  };
+  PairView v;
+  v.e = __builtin_amdgcn_ds_bpermute(a, pb.e);
+  v.cz = mv(pb.cz);
+  v.cy = mv(pb.cy);
+  v.A = mv(pb.A);
+  v.B = mv(pb.B);
+  v.P = mv(pb.P);
+  v.Q0 = mv(pb.Q0);
+  v.inv = WITH_INV ? mv(pb.inv) : 0.0f;
+  const uint32_t bx = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)pb.bx);
+  const uint32_t by = (uint32_t)__builtin_amdgcn_ds_bpermute(a, (int)pb.by);
+  v.x0 = bx & 0xffff;
+  v.x1 = bx >> 16;
+  v.y0 = by & 0xffff;
+  v.y1 = by >> 16;
+  return v;
+}
+
+// Pair j + group of a batch for this lane: wave-uniform scalars when a step
+// holds one pair (G = 1), else the lane group's own pair.
+template <int G, bool WITH_INV>
+__device__ inline PairView step_pair(const PairBatch& pb, int j, int group) {
+  if constexpr (G == 1)
+    return pick(pb, j);
+  else
+    return pick_lane<WITH_INV>(pb, j + group);
+}
+
+// Sum over each group of N consecutive lanes (N = 16, 32 or 64), returned
+// in every lane of the group.
+template <int N>
+__device__ inline float group_total(float v) {
+  v += dpp_move<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_move<0x124>(v);  // row_ror:4
+  v += dpp_move<0x128>(v);  // row_ror:8
+  if constexpr (N >= 32) v += __shfl_xor(v, 16);
+  if constexpr (N >= 64) v += __shfl_xor(v, 32);
+  return v;
+}
 }  // namespace
 }  // namespace strip

@@ -1300,19 +1300,45 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
                                int backward_rows, int bucket_columns,
                                int forward_pairs, int backward_pairs) {
   if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
-  if ((forward_rows != 32 && forward_rows != 64) ||
+  if ((forward_rows != 16 && forward_rows != 32 && forward_rows != 64) ||
       (backward_rows != 16 && backward_rows != 32 && backward_rows != 64) ||
       (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
       forward_pairs < 1 || backward_pairs < 1)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "forward rows 32/64, back-projection rows 16/32/64, "
-                       "bucket 16/32/64, pairs >= 1");
+                       "band rows 16/32/64, bucket 16/32/64, pairs >= 1");
   ctx->band.tuning.fwd_rows = forward_rows;
   ctx->band.tuning.bwd_rows = backward_rows;
   ctx->band.tuning.bucket = bucket_columns;
   ctx->band.tuning.fwd_pairs = forward_pairs;
   ctx->band.tuning.bwd_pairs = backward_pairs;
   ctx->band_dirty = true;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
+                               int backward_group) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
+  if (!ok(forward_group) || !ok(backward_group))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "groups must be 1, 2 or 4");
+  if (ctx->band.tuning.fwd_group != forward_group ||
+      ctx->band.tuning.bwd_group != backward_group) {
+    ctx->band.tuning.fwd_group = forward_group;
+    ctx->band.tuning.bwd_group = backward_group;
+    ctx->band_dirty = true;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_get_band_groups(strip_mlem_ctx* ctx, int* forward_group,
+                               int* backward_group) {
+  CTX_CHECK(ctx);
+  if (effective_algorithm(ctx) != 2)
+    return strip::fail(STRIP_MLEM_ERR_STATE, "row-band algorithm not in use");
+  int rc = ensure_band(ctx);
+  if (rc) return rc;
+  if (forward_group) *forward_group = ctx->band.fwd_group;
+  if (backward_group) *backward_group = ctx->band.bwd_group;
   return STRIP_MLEM_OK;
 }
 
@@ -1571,6 +1597,7 @@ struct strip_mlem_batch {
   int* queue = nullptr;  // task counters of the forward / back-projection
   int n_fwd = 0, n_bwd = 0;
   int H_fwd = 0, H_bwd = 0;
+  int G_fwd = 1, G_bwd = 1;  // pairs per wavefront step
   int fwd_Wt = 0, bwd_Wt = 0, bwd_waves = 1;  // widest member windows
   size_t fwd_lds = 0;
   bool spans = true;
@@ -1636,6 +1663,8 @@ int batch_merge(strip_mlem_batch* b) {
         c->band.tuning.bucket != b->tuning.bucket ||
         c->band.tuning.fwd_pairs != b->tuning.fwd_pairs ||
         c->band.tuning.bwd_pairs != b->tuning.bwd_pairs ||
+        c->band.tuning.fwd_group != b->tuning.fwd_group ||
+        c->band.tuning.bwd_group != b->tuning.bwd_group ||
         c->band.tuning.spans != b->tuning.spans) {
       c->band.tuning = b->tuning;
       c->band_dirty = true;
@@ -1646,9 +1675,12 @@ int batch_merge(strip_mlem_batch* b) {
     if (i == 0) {
       b->H_fwd = P.fwd.H;
       b->H_bwd = P.bwd.H;
-    } else if (P.fwd.H != b->H_fwd || P.bwd.H != b->H_bwd) {
+      b->G_fwd = P.fwd_group;
+      b->G_bwd = P.bwd_group;
+    } else if (P.fwd.H != b->H_fwd || P.bwd.H != b->H_bwd ||
+               P.fwd_group != b->G_fwd || P.bwd_group != b->G_bwd) {
       return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
-                         "members ended with different band heights");
+                         "members ended with different band layouts");
     }
     b->fwd_Wt = std::max(b->fwd_Wt, P.fwd.Wt);
     b->bwd_Wt = std::max(b->bwd_Wt, P.bwd.Wt);
@@ -1668,7 +1700,8 @@ int batch_merge(strip_mlem_batch* b) {
     b->generations.push_back(P.generation);
   }
   b->fwd_lds = sizeof(float) * (size_t)b->fwd_Wt * b->H_fwd;
-  const size_t window = sizeof(float) * (size_t)b->bwd_Wt * b->H_bwd;
+  const size_t window =
+      sizeof(float) * (size_t)b->bwd_Wt * b->H_bwd * b->G_bwd;
   if (b->fwd_lds > 160 * 1024 || window > 160 * 1024)
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
                        "a member's task window does not fit in LDS");
@@ -1741,15 +1774,15 @@ int batch_launch_iteration(strip_mlem_batch* b, int c) {
   const bool rec = b->tuning.recurrence != 0;
   if (strip::band_zero_ints(b->queue, 2, b->stream, &err) ||
       strip::band_launch_forward(b->desc[c], b->fwd_tasks, b->n_fwd, b->H_fwd,
-                                 b->spans, rec, b->fwd_lds,
+                                 b->G_fwd, b->spans, rec, b->fwd_lds,
                                  std::min(b->n_fwd, persistent), b->queue,
                                  b->stream, &err) ||
       strip::band_launch_batch_update(b->desc[c], n, b->max_events,
                                       b->max_pixels, b->H_fwd, true,
                                       b->stream, &err) ||
       strip::band_launch_backward(
-          b->desc[c], b->bwd_tasks, b->n_bwd, b->H_bwd, b->spans, rec,
-          b->bwd_Wt * b->H_bwd, b->bwd_waves,
+          b->desc[c], b->bwd_tasks, b->n_bwd, b->H_bwd, b->G_bwd, b->spans,
+          rec, b->bwd_Wt * b->H_bwd, b->bwd_waves,
           std::min((b->n_bwd + b->bwd_waves - 1) / b->bwd_waves, persistent),
           b->queue + 1, b->stream, &err) ||
       strip::band_launch_batch_update(b->desc[c], n, b->max_events,
@@ -1902,13 +1935,13 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
                                 int forward_pairs, int backward_pairs,
                                 int grid_factor) {
   BATCH_CHECK(b);
-  if ((forward_rows != 32 && forward_rows != 64) ||
+  if ((forward_rows != 16 && forward_rows != 32 && forward_rows != 64) ||
       (backward_rows != 16 && backward_rows != 32 && backward_rows != 64) ||
       (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
       forward_pairs < 1 || backward_pairs < 1 || grid_factor < 1 ||
       grid_factor > 64)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "band rows 32/64, bucket 16/32/64, pairs >= 1, "
+                       "band rows 16/32/64, bucket 16/32/64, pairs >= 1, "
                        "grid factor 1..64");
   int rc = batch_store_images(b);
   if (rc) return rc;
@@ -1922,6 +1955,27 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
     b->graph = nullptr;
   }
   b->grid_factor = grid_factor;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_set_groups(strip_mlem_batch* b, int forward_group,
+                                int backward_group) {
+  BATCH_CHECK(b);
+  auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
+  if (!ok(forward_group) || !ok(backward_group))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "groups must be 1, 2 or 4");
+  int rc = batch_store_images(b);
+  if (rc) return rc;
+  b->tuning.fwd_group = forward_group;
+  b->tuning.bwd_group = backward_group;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_get_groups(const strip_mlem_batch* b, int* forward_group,
+                                int* backward_group) {
+  if (!b) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "batch is NULL");
+  if (forward_group) *forward_group = b->tuning.fwd_group;
+  if (backward_group) *backward_group = b->tuning.bwd_group;
   return STRIP_MLEM_OK;
 }
 
@@ -1980,10 +2034,12 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
     saved[i].resize(b->items[i]->n_pixels);
     if ((rc = strip_mlem_get_rho(b->items[i], saved[i].data()))) return rc;
   }
-  int cur[6];
+  int cur[8];
   strip_mlem_batch_get_tiling(b, cur);
+  strip_mlem_batch_get_groups(b, cur + 6, cur + 7);
   auto time_it = [&](const int* t, double* ms) -> int {
     int r = strip_mlem_batch_set_tiling(b, t[0], t[1], t[2], t[3], t[4], t[5]);
+    if (!r) r = strip_mlem_batch_set_groups(b, t[6], t[7]);
     if (r) return r;
     const bool graph = b->use_graph;
     b->use_graph = false;
@@ -1996,24 +2052,25 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
   double best = 0.0;
   if ((rc = time_it(cur, &best))) return rc;
   const std::vector<std::vector<int>> choices = {
-      {64, 32}, {16, 32, 64}, {16, 32, 64}, {1024, 2048, 4096},
-      {256, 512, 1024}, {2, 4, 8}};
-  for (int k = 0; k < 6; ++k) {
+      {64, 32, 16}, {16, 32, 64}, {16, 32, 64}, {1024, 2048, 4096},
+      {256, 512, 1024}, {2, 4, 8}, {1, 2, 4}, {1, 2, 4}};
+  for (int k = 0; k < 8; ++k) {
     for (int v : choices[k]) {
       if (v == cur[k]) continue;
-      int cand[6];
-      std::copy(cur, cur + 6, cand);
+      int cand[8];
+      std::copy(cur, cur + 8, cand);
       cand[k] = v;
       double ms = 0.0;
       if (time_it(cand, &ms)) continue;  // a shape that does not fit
       if (ms < best) {
         best = ms;
-        std::copy(cand, cand + 6, cur);
+        std::copy(cand, cand + 8, cur);
       }
     }
   }
   if ((rc = strip_mlem_batch_set_tiling(b, cur[0], cur[1], cur[2], cur[3],
-                                        cur[4], cur[5])))
+                                        cur[4], cur[5])) ||
+      (rc = strip_mlem_batch_set_groups(b, cur[6], cur[7])))
     return rc;
   for (size_t i = 0; i < b->items.size(); ++i)
     if ((rc = strip_mlem_set_rho(b->items[i], saved[i].data()))) return rc;

@@ -159,6 +159,12 @@ def load_library():
             ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]),
         "strip_phantom_last_error": (ctypes.c_char_p, []),
         "strip_mlem_set_backprojection_waves": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_set_band_groups": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+        "strip_mlem_get_band_groups": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int),
+                                                      ctypes.POINTER(ctypes.c_int)]),
+        "strip_mlem_batch_set_groups": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
+        "strip_mlem_batch_get_groups": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int),
+                                                       ctypes.POINTER(ctypes.c_int)]),
         "strip_mlem_reserve_events": (ctypes.c_int, [P, i64]),
         "strip_mlem_append_events": (ctypes.c_int, [P, fptr, i64]),
         "strip_mlem_set_subsets": (ctypes.c_int, [P, ctypes.c_int]),
@@ -232,6 +238,8 @@ def exported_symbols():
         "strip_mlem_set_band_tiling", "strip_mlem_get_band_tiling",
         "strip_mlem_set_exp_mode", "strip_mlem_set_span_table",
         "strip_mlem_set_backprojection_waves",
+        "strip_mlem_set_band_groups", "strip_mlem_get_band_groups",
+        "strip_mlem_batch_set_groups", "strip_mlem_batch_get_groups",
         "strip_mlem_get_span_table", "strip_mlem_loglik_gradient",
         "strip_mlem_loglik_gradient_device", "strip_mlem_unrolled_gradient",
         "strip_mlem_unrolled_gradient_device", "strip_mlem_batch_create",
@@ -359,7 +367,7 @@ class Reconstruction:
     def set_algorithm(self, algorithm: int):
         _check(self._lib.strip_mlem_set_algorithm(self._ctx, algorithm))
 
-    def set_band_tiling(self, forward_rows=64, backward_rows=16,
+    def set_band_tiling(self, forward_rows=32, backward_rows=16,
                         bucket_columns=16, forward_pairs=2048,
                         backward_pairs=512):
         _check(self._lib.strip_mlem_set_band_tiling(
@@ -369,6 +377,19 @@ class Reconstruction:
     def set_exp_mode(self, mode: int):
         """0 = exp2 per pixel, 1 = restarted exp recurrence along rows."""
         _check(self._lib.strip_mlem_set_exp_mode(self._ctx, int(mode)))
+
+    def set_band_groups(self, forward_group: int = 2, backward_group: int = 2):
+        """(Event, band) pairs one wavefront processes together in the
+        forward / back-projection kernels (1, 2 or 4)."""
+        _check(self._lib.strip_mlem_set_band_groups(
+            self._ctx, int(forward_group), int(backward_group)))
+
+    def band_groups(self):
+        """(forward, back-projection) pairs per wavefront step in use."""
+        f, b = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.strip_mlem_get_band_groups(
+            self._ctx, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
 
     def set_backprojection_waves(self, waves: int):
         """Wavefronts per back-projection workgroup (0 = automatic)."""
@@ -630,7 +651,7 @@ class Batch:
                                                   ctypes.byref(c)))
         return _MemberView(self._lib, c, self.geometries[index])
 
-    def set_tiling(self, forward_rows=64, backward_rows=16, bucket_columns=16,
+    def set_tiling(self, forward_rows=32, backward_rows=16, bucket_columns=16,
                    forward_pairs=2048, backward_pairs=512, grid_factor=4):
         _check(self._lib.strip_mlem_batch_set_tiling(
             self._b, forward_rows, backward_rows, bucket_columns,
@@ -640,6 +661,16 @@ class Batch:
         t = (ctypes.c_int * 6)()
         _check(self._lib.strip_mlem_batch_get_tiling(self._b, t))
         return tuple(t)
+
+    def set_groups(self, forward_group=2, backward_group=2):
+        _check(self._lib.strip_mlem_batch_set_groups(
+            self._b, int(forward_group), int(backward_group)))
+
+    def groups(self):
+        f, b = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.strip_mlem_batch_get_groups(
+            self._b, ctypes.byref(f), ctypes.byref(b)))
+        return f.value, b.value
 
     def set_graph(self, enable: bool):
         _check(self._lib.strip_mlem_batch_set_graph(self._b, int(bool(enable))))

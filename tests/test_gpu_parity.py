@@ -264,3 +264,37 @@ def test_band_tilings(pystrip_mod, phantom_mod, oracle_mod, tiling, waves):
     e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=3))
     print("rel-L2", e, "tiling", rec.band_tiling())
     assert e <= 1e-4
+
+
+@pytest.mark.parametrize("rows,groups,spans,exp_mode", [
+    ((16, 16), (4, 2), True, 1), ((16, 16), (2, 4), True, 0),
+    ((32, 32), (2, 2), True, 1), ((16, 16), (4, 4), False, 1),
+    ((32, 16), (2, 2), False, 0)])
+def test_band_groups(pystrip_mod, phantom_mod, oracle_mod, rows, groups,
+                     spans, exp_mode):
+    # several pairs per wavefront step (lane groups with their own pairs and
+    # back-projection windows): forward D per event, support count and the
+    # reconstruction against the oracle
+    print_case("band groups", rows, groups, "spans", spans, "exp", exp_mode)
+    w, ev, rec, og = _setup(pystrip_mod, phantom_mod, oracle_mod, CFG2,
+                            60_000, algorithm=2)
+    rec.set_span_table(spans)
+    rec.set_exp_mode(exp_mode)
+    rec.set_band_tiling(rows[0], rows[1], 16, 2048, 512)
+    rec.set_band_groups(*groups)
+    nb, ne = oracle_mod.support_counts(og, ev)
+    assert rec.kernel_evaluations() == int(ne.sum())
+    assert rec.band_groups() == groups
+    rng = np.random.default_rng(9)
+    rho = rng.uniform(0.5, 2.0, size=rec.shape).astype(np.float32)
+    rec.rho = rho
+    d_gpu = rec.forward().astype(np.float64)
+    d_ref = oracle_mod.forward(og, ev, rho).astype(np.float64)
+    nz = d_ref > 0
+    assert np.array_equal(d_gpu > 0, nz)
+    assert np.max(np.abs(d_gpu[nz] - d_ref[nz]) / d_ref[nz]) < 2e-5
+    rec.rho = np.ones(rec.shape, np.float32)
+    rec.iterate(3)
+    e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=3))
+    print("rel-L2", e, "tiling", rec.band_tiling())
+    assert e <= 1e-4
