@@ -360,21 +360,23 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     const int n_here = min(64, end - base);
     // the next step's pair and span word are in flight while one step is
     // processed
-    PairView nv = step_pair<G, false>(cur, 0, grp);
-    uint32_t sp = 0;
-    if (SPANS) sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
-    for (int j = 0; j < n_here; j += G) {
-      const PairView v = nv;
+    // two register sets for a step's pair and span word, used alternately:
+    // a step reads one set and fills the other with the next step's
+    PairView va = step_pair<G, false>(cur, 0, grp), vb = va;
+    uint32_t spa = 0, spb = 0;
+    if (SPANS) spa = fetch_span(spans, span_stride, va.e, (uint32_t)va.y0, iy);
+    auto run_step = [&](int j, const PairView& v, uint32_t sp, PairView& nv,
+                        uint32_t& sp_next) {
       const bool valid = G == 1 || j + grp < n_here;
       RowSpan s;
       if (SPANS) {
         const uint32_t word = (valid && iy >= v.y0 && iy < v.y1) ? sp : 0u;
-        // the word is consumed before the next load reuses its register, so
-        // that load stays in flight through this step
+        // the word is consumed before the next load is issued, so that load
+        // stays in flight through this step
         __builtin_amdgcn_sched_barrier(0);
         const int jn = G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G);
         nv = step_pair<G, false>(cur, jn, grp);
-        sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
+        sp_next = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
         s = stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy);
       } else {
         if (G > 1) nv = step_pair<G, false>(cur, min(j + G, 64 - G), grp);
@@ -389,7 +391,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
           counted += (unsigned long long)wave_sum_int(q == 0 ? s.n : 0);
         else
           counted += (unsigned long long)(q == 0 ? s.n : 0);
-        continue;
+        return;
       }
       const int col0 = s.lo + q;
       float dz = __builtin_fmaf((float)col0, g.pixel_width,
@@ -458,6 +460,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
         if ((lane & (H * LR - 1)) == 0 && valid)
           gstore(partial + (int64_t)v.e * max_bands + (b - v.y0 / H), dsum);
       }
+    };
+    for (int j = 0; j < n_here; j += 2 * G) {
+      run_step(j, va, spa, vb, spb);
+      if (j + G >= n_here) break;
+      run_step(j + G, vb, spb, va, spa);
     }
   }
   }
@@ -528,25 +535,26 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
     if (base + 64 < task.z)
       next = gather_batch<true>(rec, pair_event, inv, base + 64, task.z, lane);
     const int n_here = min(64, task.z - base);
-    PairView nv = step_pair<G, true>(cur, 0, grp);
-    uint32_t sp = 0;
-    if (SPANS) sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
-    for (int j = 0; j < n_here; j += G) {
-      const PairView v = nv;
+    // two register sets for a step's pair and span word, used alternately
+    PairView va = step_pair<G, true>(cur, 0, grp), vb = va;
+    uint32_t spa = 0, spb = 0;
+    if (SPANS) spa = fetch_span(spans, span_stride, va.e, (uint32_t)va.y0, iy);
+    auto run_step = [&](int j, const PairView& v, uint32_t sp, PairView& nv,
+                        uint32_t& sp_next) {
       const bool valid = G == 1 || j + grp < n_here;
       uint32_t word = 0;
       if (SPANS) {
         // an event with inv = 0 adds nothing: empty run; the word is
-        // consumed before the next load reuses its register
+        // consumed before the next load is issued
         word = (valid && iy >= v.y0 && iy < v.y1 && v.inv != 0.0f) ? sp : 0u;
         __builtin_amdgcn_sched_barrier(0);
         const int jn = G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G);
         nv = step_pair<G, true>(cur, jn, grp);
-        sp = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
+        sp_next = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
       } else {
         nv = step_pair<G, true>(
             cur, G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G), grp);
-        if (G == 1 && v.inv == 0.0f) continue;
+        if (G == 1 && v.inv == 0.0f) return;
       }
       RowSpan s =
           SPANS ? stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy)
@@ -621,6 +629,11 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
         dz += step;
         w += S;
       }
+    };
+    for (int j = 0; j < n_here; j += 2 * G) {
+      run_step(j, va, spa, vb, spb);
+      if (j + G >= n_here) break;
+      run_step(j + G, vb, spb, va, spa);
     }
   }
   // flush: lane = q'*H + r adds row r of columns c0+q', c0+q'+64/H, ...,
