@@ -34,54 +34,67 @@ __global__ __launch_bounds__(256) void count_pairs_kernel(
   counts[e] = c;
 }
 
-// Columns [lo, hi) reached by record e in band b of height H: the union of
-// the band rows' runs from the span table, or the bbox columns without one
-// (an empty slice gives the bbox start and one column).
-__device__ inline void band_columns(const BandRecord& r,
-                                    const uint16_t* __restrict__ spans,
-                                    int stride, int64_t e, int b, int H,
-                                    int& lo, int& hi) {
+// bounds[e * max_bands + k] = lo | hi << 16: the columns [lo, hi) reached
+// by the runs of band y0 / H + k of record e (the span table's runs, or the
+// bbox columns without one; an empty slice gives [x0, x0 + 1)).  One
+// wavefront per record, lane = row offset from the start of its first band,
+// so a band's rows are H consecutive lanes (H divides 64).
+__global__ __launch_bounds__(256) void band_bounds_kernel(
+    const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
+    int stride, int64_t n, int H, int max_bands, int* __restrict__ bounds) {
+  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (e >= n) return;  // uniform over the wavefront
+  const int lane = threadIdx.x & 63;
   int x0, x1, y0, y1;
-  record_box(r, x0, x1, y0, y1);
-  if (!spans) {
-    lo = x0;
-    hi = x1;
-    return;
-  }
-  lo = x1;
-  hi = x0;
-  const int r0 = max(b * H, y0), r1 = min(b * H + H, y1);
-  for (int iy = r0; iy < r1; ++iy) {
-    const uint32_t w = spans[e * stride + (iy - y0)];
-    const int n = (int)(w >> 8);
-    if (n > 0) {
-      lo = min(lo, x0 + (int)(w & 0xff));
-      hi = max(hi, x0 + (int)(w & 0xff) + n);
+  record_box(rec[e], x0, x1, y0, y1);
+  if (x1 <= x0 || y1 <= y0) return;
+  const int b0 = y0 / H;
+  const int rows = ((y1 - 1) / H - b0 + 1) * H;
+  for (int base = 0; base < rows; base += 64) {
+    const int i = base + lane;
+    const int iy = b0 * H + i;
+    int lo = 0x7fff, hi = 0;
+    if (i < rows && iy >= y0 && iy < y1) {
+      if (spans) {
+        const uint32_t w = spans[e * stride + (iy - y0)];
+        if (w >> 8) {
+          lo = x0 + (int)(w & 0xff);
+          hi = lo + (int)(w >> 8);
+        }
+      } else {
+        lo = x0;
+        hi = x1;
+      }
     }
-  }
-  if (hi <= lo) {
-    lo = x0;
-    hi = x0 + 1;
+    for (int o = 1; o < H; o <<= 1) {
+      lo = min(lo, __shfl_xor(lo, o));
+      hi = max(hi, __shfl_xor(hi, o));
+    }
+    if ((lane & (H - 1)) == 0 && i < rows) {
+      if (hi <= lo) {
+        lo = x0;
+        hi = x0 + 1;
+      }
+      bounds[e * max_bands + i / H] = lo | (hi << 16);
+    }
   }
 }
 
 // keys of the event's pairs: band << 16 | first column the band's runs reach
 __global__ __launch_bounds__(256) void emit_pairs_kernel(
-    const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
-    int stride, int64_t n, int H, const int* __restrict__ counts,
+    const BandRecord* __restrict__ rec, const int* __restrict__ bounds,
+    int max_bands, int64_t n, int H, const int* __restrict__ counts,
     const int* __restrict__ offsets, uint32_t* __restrict__ keys,
     int* __restrict__ values) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int c = counts[e];
   if (!c) return;
-  const BandRecord r = rec[e];
-  const int b0 = (int)(r.by & 0xffff) / H;
+  const int b0 = (int)(rec[e].by & 0xffff) / H;
   const int base = offsets[e];
   for (int k = 0; k < c; ++k) {
-    int lo, hi;
-    band_columns(r, spans, stride, e, b0 + k, H, lo, hi);
-    keys[base + k] = ((uint32_t)(b0 + k) << 16) | (uint32_t)lo;
+    const uint32_t lo = (uint32_t)bounds[e * max_bands + k] & 0xffff;
+    keys[base + k] = ((uint32_t)(b0 + k) << 16) | lo;
     values[base + k] = (int)e;
   }
 }
@@ -173,8 +186,8 @@ __global__ __launch_bounds__(256) void group_bounds_kernel(
 // task.w |= (columns its pairs reach, counted from its first column) << 16,
 // at most Wt: one wavefront per task
 __global__ __launch_bounds__(256) void task_extent_kernel(
-    const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
-    int stride, int H, const int* __restrict__ pair_event,
+    const BandRecord* __restrict__ rec, const int* __restrict__ bounds,
+    int max_bands, int H, const int* __restrict__ pair_event,
     int4* __restrict__ tasks, int n_tasks, int Wt) {
   const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= n_tasks) return;  // uniform over the wavefront
@@ -183,8 +196,8 @@ __global__ __launch_bounds__(256) void task_extent_kernel(
   int reach = 1;
   for (int p = task.y + lane; p < task.z; p += 64) {
     const int e = pair_event[p];
-    int lo, hi;
-    band_columns(rec[e], spans, stride, e, task.x & 0xffff, H, lo, hi);
+    const int k = (task.x & 0xffff) - (int)(rec[e].by & 0xffff) / H;
+    const int hi = (int)((uint32_t)bounds[(int64_t)e * max_bands + k] >> 16);
     reach = max(reach, hi - task.w);
   }
   for (int o = 32; o > 0; o >>= 1) reach = max(reach, __shfl_xor(reach, o));
@@ -846,7 +859,10 @@ void free_ptr(T*& p) {
 }
 
 void free_list(PairList& L) {
-  if (L.owns_pairs) free_ptr(L.pair_event);
+  if (L.owns_pairs) {
+    free_ptr(L.pair_event);
+    free_ptr(L.bounds);
+  }
   L.pair_event = nullptr;
   free_ptr(L.tasks);
   L = PairList{};
@@ -913,8 +929,13 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
     BAND_TRY(hipMalloc(&keys_alt, sizeof(uint32_t) * n_pairs));
     BAND_TRY(hipMalloc(&L.pair_event, sizeof(int) * n_pairs));
     BAND_TRY(hipMalloc(&vals_alt, sizeof(int) * n_pairs));
+    BAND_TRY(hipMalloc(&L.bounds, sizeof(int) * n * L.max_bands));
+    hipLaunchKernelGGL(band_bounds_kernel, dim3((unsigned)((n + 3) / 4)),
+                       dim3(256), 0, stream, rec, spans, stride, n, H,
+                       L.max_bands, L.bounds);
+    BAND_TRY(hipGetLastError());
     hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       rec, spans, stride, n, H, counts, offsets, keys,
+                       rec, L.bounds, L.max_bands, n, H, counts, offsets, keys,
                        L.pair_event);
     BAND_TRY(hipGetLastError());
     int end_bit = 16;
@@ -959,8 +980,8 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
 // Tasks of at most `chunk` consecutive pairs inside one group, each with the
 // columns its pairs reach; L.Wt = the widest task.
 int build_tasks(PairList& L, int shift, const std::vector<int>& first,
-                int chunk, const BandRecord* rec, const uint16_t* spans,
-                int stride, int Wt, hipStream_t stream, std::string* err) {
+                int chunk, const BandRecord* rec, int Wt, hipStream_t stream,
+                std::string* err) {
   const int n_groups = (int)first.size() - 1;
   std::vector<int4> tasks;
   std::vector<int> end(n_groups, 0);
@@ -983,8 +1004,8 @@ int build_tasks(PairList& L, int shift, const std::vector<int>& first,
     BAND_TRY(hipMemcpyAsync(L.tasks, tasks.data(), sizeof(int4) * tasks.size(),
                             hipMemcpyHostToDevice, stream));
     hipLaunchKernelGGL(task_extent_kernel, dim3((L.n_tasks + 3) / 4),
-                       dim3(256), 0, stream, rec, spans, stride, L.H,
-                       L.pair_event, L.tasks, L.n_tasks, Wt);
+                       dim3(256), 0, stream, rec, L.bounds, L.max_bands,
+                       L.H, L.pair_event, L.tasks, L.n_tasks, Wt);
     BAND_TRY(hipGetLastError());
     BAND_TRY(hipMemcpyAsync(tasks.data(), L.tasks, sizeof(int4) * tasks.size(),
                             hipMemcpyDeviceToHost, stream));
@@ -1143,7 +1164,7 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if (build_pairs(plan.fwd, Hf, shift, g, plan.records, sp, ss, n, stream,
                   first, err))
     return -1;
-  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, plan.records, sp, ss,
+  if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, plan.records,
                   plan.Wt, stream, err))
     return -1;
   if (Hb == Hf) {
@@ -1152,14 +1173,19 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     plan.bwd.max_bands = plan.fwd.max_bands;
     plan.bwd.n_pairs = plan.fwd.n_pairs;
     plan.bwd.pair_event = plan.fwd.pair_event;
+    plan.bwd.bounds = plan.fwd.bounds;
     plan.bwd.owns_pairs = false;
   } else if (build_pairs(plan.bwd, Hb, shift, g, plan.records, sp, ss, n,
                          stream, first, err)) {
     return -1;
   }
-  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records, sp, ss,
+  if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records,
                   plan.Wt, stream, err))
     return -1;
+  // the per-(record, band) column bounds are needed only to cut the tasks
+  if (plan.bwd.owns_pairs) free_ptr(plan.bwd.bounds);
+  plan.bwd.bounds = nullptr;
+  free_ptr(plan.fwd.bounds);
   // pairs per wavefront step: as asked where the lane layout allows it
   // (G * H <= 64)
   plan.fwd_group = std::max(1, std::min(tu.fwd_group, 64 / Hf));

@@ -74,6 +74,9 @@ struct PairList {
   int4* tasks = nullptr;
   int n_tasks = 0;
   int Wt = 1;                 // widest task window (columns)
+  // while the plan is built: columns [lo, hi) each (record, band) reaches,
+  // lo | hi << 16 at [record * max_bands + band offset]
+  int* bounds = nullptr;
   bool owns_pairs = true;
   std::vector<int4> tasks_host;  // host copy (instance 0)
 };
