@@ -56,7 +56,7 @@ __global__ __launch_bounds__(256) void band_bounds_kernel(
     int lo = 0x7fff, hi = 0;
     if (i < rows && iy >= y0 && iy < y1) {
       if (spans) {
-        const uint32_t w = spans[e * stride + (iy - y0)];
+        const uint32_t w = spans[e * stride + (iy - (y0 & ~(kSpanAlign - 1)))];
         if (w >> 8) {
           lo = x0 + (int)(w & 0xff);
           hi = lo + (int)(w >> 8);
@@ -99,7 +99,8 @@ __global__ __launch_bounds__(256) void emit_pairs_kernel(
   }
 }
 
-// extent[e] = clamped bbox width (rows = 0) or height (rows = 1)
+// extent[e] = clamped bbox width (rows = 0) or its rows rounded out to
+// multiples of kSpanAlign (rows = 1: the record's span-table slot)
 __global__ __launch_bounds__(256) void box_extent_kernel(
     const uint2* __restrict__ box, int64_t n, int rows,
     int* __restrict__ extent) {
@@ -107,7 +108,12 @@ __global__ __launch_bounds__(256) void box_extent_kernel(
   if (e >= n) return;
   const uint2 b = box[e];
   const uint32_t v = rows ? b.y : b.x;
-  extent[e] = max((int)(v >> 16) - (int)(v & 0xffff), 0);
+  const int lo = (int)(v & 0xffff), hi = (int)(v >> 16);
+  if (!rows || hi <= lo)
+    extent[e] = max(hi - lo, 0);
+  else
+    extent[e] = ((hi + kSpanAlign - 1) & ~(kSpanAlign - 1)) -
+                (lo & ~(kSpanAlign - 1));
 }
 
 // spatial order of the events: key = (bbox y0, bbox x0)
@@ -153,9 +159,10 @@ __global__ __launch_bounds__(256) void unpermute_kernel(
   out[order[i]] = in[i];
 }
 
-// Span table: one wavefront per record, lanes over the rows of its bbox;
-// spans[i * stride + k] = (lo - x0) | n << 8 of row y0 + k (stride >= bbox
-// height, lo - x0 and n below 256).
+// Span table: one wavefront per record, lanes over the rows of its slot;
+// spans[i * stride + k] = (lo - x0) | n << 8 of row y0a + k, y0a = y0
+// rounded down to a multiple of kSpanAlign (0 outside [y0, y1); stride >=
+// every slot, lo - x0 and n below 256).
 __global__ __launch_bounds__(256) void span_table_kernel(
     DeviceGeometry g, const BandRecord* __restrict__ rec, int64_t n,
     int stride, uint16_t* __restrict__ spans) {
@@ -166,10 +173,16 @@ __global__ __launch_bounds__(256) void span_table_kernel(
   int x0, x1, y0, y1;
   record_box(r, x0, x1, y0, y1);
   uint16_t* out = spans + i * stride;
-  for (int k = lane; k < y1 - y0; k += 64) {
-    const RowSpan s =
-        row_span(g, r.cz, r.cy, r.A, r.B, r.P, r.Q0, x0, x1, y0, y1, y0 + k);
-    out[k] = (uint16_t)(s.n > 0 ? ((s.lo - x0) | (s.n << 8)) : 0);
+  const int y0a = y0 & ~(kSpanAlign - 1);
+  for (int k = lane; k < stride; k += 64) {
+    const int iy = y0a + k;
+    uint16_t word = 0;
+    if (iy >= y0 && iy < y1) {
+      const RowSpan s =
+          row_span(g, r.cz, r.cy, r.A, r.B, r.P, r.Q0, x0, x1, y0, y1, iy);
+      if (s.n > 0) word = (uint16_t)((s.lo - x0) | (s.n << 8));
+    }
+    out[k] = word;
   }
 }
 
@@ -383,7 +396,8 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
       const bool valid = G == 1 || j + grp < n_here;
       RowSpan s;
       if (SPANS) {
-        const uint32_t word = (valid && iy >= v.y0 && iy < v.y1) ? sp : 0u;
+        // rows outside the bbox read 0 (an empty run)
+        const uint32_t word = valid ? sp : 0u;
         // the word is consumed before the next load is issued, so that load
         // stays in flight through this step
         __builtin_amdgcn_sched_barrier(0);
@@ -559,7 +573,7 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
       if (SPANS) {
         // an event with inv = 0 adds nothing: empty run; the word is
         // consumed before the next load is issued
-        word = (valid && iy >= v.y0 && iy < v.y1 && v.inv != 0.0f) ? sp : 0u;
+        word = (valid && v.inv != 0.0f) ? sp : 0u;
         __builtin_amdgcn_sched_barrier(0);
         const int jn = G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G);
         nv = step_pair<G, true>(cur, jn, grp);

@@ -122,12 +122,13 @@ struct BandPlan {
   BandRecord* records = nullptr;
   int* order = nullptr;
   // exact support of every bbox row of every record, computed once per
-  // upload: spans[i * span_stride + (iy - y0)] = (lo - x0) | n << 8 (first
-  // column and length of row iy's in-ellipse run); null when a bbox is wider
-  // than 255 columns or the table does not fit, then the kernels solve each
-  // row's span themselves
+  // upload: spans[i * span_stride + (iy - y0a)] = (lo - x0) | n << 8 (first
+  // column and length of row iy's in-ellipse run, 0 outside the bbox), y0a =
+  // y0 rounded down to a multiple of 64 so the rows of a band are one aligned
+  // piece; null when a bbox is wider than 255 columns or the table does not
+  // fit, then the kernels solve each row's span themselves
   uint16_t* spans = nullptr;
-  int span_stride = 0;  // tallest clamped bbox (y1 - y0)
+  int span_stride = 0;  // largest slot: bbox rows rounded out to 64
   float* partial = nullptr;   // n_events * fwd.max_bands
   float* inv = nullptr;       // n_events
   float* rhoT[2] = {nullptr, nullptr};  // column-major images (n_z * n_y)

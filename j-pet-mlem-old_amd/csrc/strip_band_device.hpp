@@ -14,6 +14,7 @@ namespace strip {
 namespace {
 
 constexpr int kFwdWaves = 8;  // wavefronts per forward workgroup
+constexpr int kSpanAlign = 64;  // span-table slots start on such rows
 
 // Pointers that a kernel reads from a descriptor in memory are generic, and
 // accesses through them compile to flat instructions, which count against
@@ -151,14 +152,17 @@ __device__ inline RowSpan stored_span(const DeviceGeometry& g, uint32_t word,
   return s;
 }
 
-// Span-table word of row iy of record e.  The index is clamped into the
-// record's slot so the load is unconditional and always in bounds; the
-// caller drops the word for rows outside [y0, y1) when it uses it (keeping
-// the load free of conditions lets it stay in flight across pairs).
+// Span-table word of row iy of record e.  A record's slot holds the rows
+// from its bbox top rounded down to a multiple of kSpanAlign (rows outside
+// the bbox hold 0, an empty run), so every band of the record (H divides
+// kSpanAlign) reads within the slot and a 16-row band is one aligned 32-byte
+// piece.  The index is clamped into the slot so the load is unconditional
+// and always in bounds (keeping the load free of conditions lets it stay in
+// flight across pairs).
 __device__ inline uint32_t fetch_span(const uint16_t* __restrict__ spans,
                                       int stride, int e, uint32_t by, int iy) {
-  const int y0 = (int)(by & 0xffff);
-  const int k = min(max(iy - y0, 0), stride - 1);
+  const int y0a = (int)(by & 0xffff) & ~(kSpanAlign - 1);
+  const int k = min(max(iy - y0a, 0), stride - 1);
   return gload(spans + (int64_t)e * stride + k);
 }
 
