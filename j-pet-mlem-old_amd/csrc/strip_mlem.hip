@@ -1945,11 +1945,16 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
                        "grid factor 1..64");
   int rc = batch_store_images(b);
   if (rc) return rc;
-  b->tuning.fwd_rows = forward_rows;
-  b->tuning.bwd_rows = backward_rows;
-  b->tuning.bucket = bucket_columns;
-  b->tuning.fwd_pairs = forward_pairs;
-  b->tuning.bwd_pairs = backward_pairs;
+  strip::BandTuning& t = b->tuning;
+  if (t.fwd_rows != forward_rows || t.bwd_rows != backward_rows ||
+      t.bucket != bucket_columns || t.fwd_pairs != forward_pairs ||
+      t.bwd_pairs != backward_pairs)
+    b->generations.clear();  // the members' plans are rebuilt and re-merged
+  t.fwd_rows = forward_rows;
+  t.bwd_rows = backward_rows;
+  t.bucket = bucket_columns;
+  t.fwd_pairs = forward_pairs;
+  t.bwd_pairs = backward_pairs;
   if (b->grid_factor != grid_factor && b->graph) {
     (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
@@ -1966,6 +1971,9 @@ int strip_mlem_batch_set_groups(strip_mlem_batch* b, int forward_group,
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "groups must be 1, 2 or 4");
   int rc = batch_store_images(b);
   if (rc) return rc;
+  if (b->tuning.fwd_group != forward_group ||
+      b->tuning.bwd_group != backward_group)
+    b->generations.clear();  // the members' plans are rebuilt and re-merged
   b->tuning.fwd_group = forward_group;
   b->tuning.bwd_group = backward_group;
   return STRIP_MLEM_OK;

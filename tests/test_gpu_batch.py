@@ -73,6 +73,27 @@ def test_batch_and_plain_agree(pystrip_mod, phantom_mod, oracle_mod):
     assert oracle_mod.rel_l2(b.rho(1), ref) <= 1e-4
 
 
+@pytest.mark.parametrize("tiling,groups", [((64, 32, 32, 1024, 256, 2), (1, 1)),
+                                           ((16, 16, 16, 2048, 512, 4), (4, 2))])
+def test_batch_tiling_takes_effect(pystrip_mod, phantom_mod, oracle_mod,
+                                   tiling, groups):
+    # a new tiling after the first iterations rebuilds every member's plan
+    members = _members(phantom_mod, n=3, seed=13)
+    print_case("batch re-tiling", tiling, groups)
+    b, evs = _build(pystrip_mod, members)
+    b.iterate(1)
+    b.set_tiling(*tiling)
+    b.set_groups(*groups)
+    b.iterate(2)
+    for i, w in enumerate(members):
+        m = b.member(i)
+        assert m.band_tiling()[:2] == tiling[:2]
+        assert m.band_groups() == groups
+        og = oracle_mod.geometry(*w.oracle_geometry_tuple())
+        ref = oracle_mod.mlem(og, evs[i], iterations=3)
+        assert oracle_mod.rel_l2(b.rho(i), ref) <= 1e-4
+
+
 def test_batch_autotune_keeps_images(pystrip_mod, phantom_mod, oracle_mod):
     members = _members(phantom_mod, n=3, seed=11)
     print_case("batch autotune")
