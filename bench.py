@@ -121,11 +121,16 @@ def batched_timing(pystrip, phantom, n_members, iterations):
         b.add(w.geometry(), w.events())
     b.iterate(1)
     tuned_ms = b.autotune()
+    b.iterate(4)  # untimed: records the iteration graph of the chosen tiling
     t = time.perf_counter()
     b.iterate(iterations)
     batch_s = time.perf_counter() - t
+    # the plain form at the library's default tiling (its own best; the
+    # members' plans otherwise carry the batch's tuned tiling)
     views = [b.member(i) for i in range(len(members))]
     for m in views:
+        m.set_band_tiling()
+        m.set_band_groups()
         m.iterate(1)
         m.synchronize()
     t = time.perf_counter()
@@ -139,6 +144,7 @@ def batched_timing(pystrip, phantom, n_members, iterations):
         "grids": sorted({f"{w.n_z}x{w.n_y}@{w.pixel:g}mm" for w in members}),
         "iterations": iterations,
         "tiling": list(b.tiling()),
+        "groups": list(b.groups()),
         "autotuned_iteration_ms": tuned_ms,
         "batched_ms_per_iteration": batch_s * 1e3 / iterations,
         "plain_ms_per_iteration": plain_s * 1e3 / iterations,

@@ -2076,9 +2076,11 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
       }
     }
   }
+  // the chosen tiling, its plans built and merged before returning
   if ((rc = strip_mlem_batch_set_tiling(b, cur[0], cur[1], cur[2], cur[3],
                                         cur[4], cur[5])) ||
-      (rc = strip_mlem_batch_set_groups(b, cur[6], cur[7])))
+      (rc = strip_mlem_batch_set_groups(b, cur[6], cur[7])) ||
+      (rc = batch_merge(b)))
     return rc;
   for (size_t i = 0; i < b->items.size(); ++i)
     if ((rc = strip_mlem_set_rho(b->items[i], saved[i].data()))) return rc;
