@@ -115,6 +115,13 @@ int strip_mlem_append_events(strip_mlem_ctx* ctx, const float* responses,
  * semantics on the row-band algorithm only. */
 int strip_mlem_set_subsets(strip_mlem_ctx* ctx, int n_subsets);
 
+/* Events per row-band plan of plain MLEM (default 2^25): a context holding
+ * more splits them into contiguous ranges with a plan each, which accumulate
+ * one MLEM update together (the plan's pair indices are 32-bit; a smaller
+ * value also bounds the per-plan memory).  No counterpart in the reference,
+ * whose CPU loop has no plan. */
+int strip_mlem_set_plan_events(strip_mlem_ctx* ctx, int64_t n);
+
 /* Sum of the current image's pixels, accumulated in FP64 on the device (for
  * MLEM it equals the number of events with D_e > 0 after an iteration,
  * reconstruction.h:290-296). */
@@ -171,7 +178,10 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
  * processes together: 1, 2 or 4 (at most 64 / band rows).  Lane groups of
  * 64 / G lanes then take one pair each, so one instruction stream serves G
  * pairs (the back-projection keeps one LDS window per group; G is halved
- * until they fit).  strip_mlem_get_band_groups returns the values in use. */
+ * until they fit).  Until this is called the defaults are (2, 2) with the
+ * back-projection's G also halved while its windows would leave fewer than
+ * 16 wavefronts per CU.  strip_mlem_get_band_groups returns the values in
+ * use. */
 int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
                                int backward_group);
 int strip_mlem_get_band_groups(strip_mlem_ctx* ctx, int* forward_group,

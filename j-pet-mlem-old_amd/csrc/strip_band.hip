@@ -22,6 +22,18 @@ namespace {
 // pair list construction
 // ---------------------------------------------------------------------------
 
+// total += sum of counts, in 64 bits (the pair offsets are 32-bit)
+__global__ __launch_bounds__(256) void sum_counts_kernel(
+    const int* __restrict__ counts, int64_t n,
+    unsigned long long* __restrict__ total) {
+  unsigned long long mine = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    mine += (unsigned long long)counts[i];
+  for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
+  if ((threadIdx.x & 63) == 0 && mine) atomicAdd(total, mine);
+}
+
 __global__ __launch_bounds__(256) void count_pairs_kernel(
     const BandRecord* __restrict__ rec, int64_t n, int H,
     int* __restrict__ counts) {
@@ -883,6 +895,7 @@ void free_list(PairList& L) {
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
+constexpr size_t kMinBackprojectionWaves = 16;  // per CU, see band_prepare
 
 // Sorted (event, band) pairs of band height H for the resident events, and
 // the index of the first pair of every (band, x0 >> shift) group.
@@ -905,6 +918,26 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
     hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
                        rec, n, H, counts);
     BAND_TRY(hipGetLastError());
+    // the pair offsets below are 32-bit: refuse before they could wrap
+    unsigned long long* d_total = nullptr;
+    unsigned long long total = 0;
+    BAND_TRY(hipMalloc(&d_total, sizeof(unsigned long long)));
+    BAND_TRY(hipMemsetAsync(d_total, 0, sizeof(unsigned long long), stream));
+    hipLaunchKernelGGL(sum_counts_kernel,
+                       dim3((unsigned)std::min<int64_t>((n + 255) / 256, 4096)),
+                       dim3(256), 0, stream, counts, n, d_total);
+    BAND_TRY(hipGetLastError());
+    BAND_TRY(hipMemcpyAsync(&total, d_total, sizeof(total),
+                            hipMemcpyDeviceToHost, stream));
+    BAND_TRY(hipStreamSynchronize(stream));
+    BAND_TRY(hipFree(d_total));
+    if (total > 0x7fffffffull) {
+      (void)hipFree(counts);
+      (void)hipFree(offsets);
+      (void)hipFree(d_red);
+      if (err) *err = "more than 2^31 (event, band) pairs in one plan";
+      return -1;
+    }
     size_t t1 = 0, t2 = 0;
     BAND_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, counts, offsets,
                                                (int)n, stream));
@@ -1207,9 +1240,13 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if (plan.fwd_group == 3) plan.fwd_group = 2;
   if (plan.bwd_group == 3) plan.bwd_group = 2;
   plan.fwd_lds = (size_t)plan.fwd.Wt * Hf * sizeof(float);
-  // back-projection: G windows per wavefront
+  // back-projection: G windows per wavefront; with the default groups, fewer
+  // pairs per step when the windows would leave fewer than 16 wavefronts per
+  // CU (the LDS-latency bound then costs more than the shared prologue saves)
   size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
-  while (plan.bwd_group > 1 && window_bytes * plan.bwd_group > kLdsLimit)
+  const size_t min_waves = tu.auto_groups ? kMinBackprojectionWaves : 1;
+  while (plan.bwd_group > 1 &&
+         window_bytes * plan.bwd_group * min_waves > kLdsLimit)
     plan.bwd_group /= 2;
   window_bytes *= plan.bwd_group;
   plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
@@ -1337,17 +1374,29 @@ int band_store_image(BandPlan& plan, const DeviceGeometry& g, float* rho,
   return 0;
 }
 
+int band_finalize(BandPlan& plan, const DeviceGeometry& g,
+                  int sensitivity_mode, hipStream_t stream, std::string* err,
+                  double scale, float* bp_out) {
+  const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
+  const int blocks = (int)std::min<int64_t>((n_pixels + 255) / 256, 4096);
+  hipLaunchKernelGGL(band_finalize_kernel, dim3(blocks), dim3(256), 0, stream,
+                     n_pixels, plan.rhoT[plan.curT], plan.bpT, plan.sensT,
+                     sensitivity_mode == 2 ? 1 : 0, scale,
+                     plan.rhoT[plan.curT ^ 1], bp_out);
+  BAND_TRY(hipGetLastError());
+  plan.curT ^= 1;
+  return 0;
+}
+
 int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                    const EventSoA& ev, int sensitivity_mode, int guard_zero,
                    hipStream_t stream, hipEvent_t t_fwd, hipEvent_t t_bwd,
-                   std::string* err, double scale, float* bp_out) {
+                   std::string* err, double scale, float* bp_out,
+                   bool finalize) {
   if (!plan.ready) {
     if (err) *err = "band plan not prepared";
     return -1;
   }
-  const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
-  float* rho = plan.rhoT[plan.curT];
-  float* next = plan.rhoT[plan.curT ^ 1];
   if (sensitivity_mode == 1) {
     if (err) *err = "the band algorithm does not fold sensitivity into D";
     return -1;
@@ -1364,13 +1413,8 @@ int band_iteration(BandPlan& plan, const DeviceGeometry& g,
   }
   if (dispatch_backward(plan, g, plan.inv, plan.bpT, stream, err)) return -1;
   if (t_bwd) BAND_TRY(hipEventRecord(t_bwd, stream));
-  const int blocks = (int)std::min<int64_t>((n_pixels + 255) / 256, 4096);
-  hipLaunchKernelGGL(band_finalize_kernel, dim3(blocks), dim3(256), 0, stream,
-                     n_pixels, rho, plan.bpT, plan.sensT,
-                     sensitivity_mode == 2 ? 1 : 0, scale, next, bp_out);
-  BAND_TRY(hipGetLastError());
-  plan.curT ^= 1;
-  return 0;
+  if (!finalize) return 0;
+  return band_finalize(plan, g, sensitivity_mode, stream, err, scale, bp_out);
 }
 
 int band_forward_only(BandPlan& plan, const DeviceGeometry& g,

@@ -43,6 +43,9 @@ struct BandTuning {
   // lanes per image row
   int fwd_group = 2;
   int bwd_group = 2;
+  // true until a caller sets the groups: the back-projection's G then also
+  // drops while its windows would leave fewer than 16 wavefronts per CU
+  bool auto_groups = true;
   // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
   // two-term recurrence e[k+1] = e[k] r[k], r[k+1] = r[k] G of exp2 of a
   // quadratic, restarted from exp2 every 8 pixels (4 in a tail); the forward
@@ -198,11 +201,20 @@ int band_store_image(BandPlan& plan, const DeviceGeometry& g, float* rho,
 // scale multiplies the update (in FP64): 1 for MLEM, the ratio of all events
 // to the plan's events for an ordered-subset update.  bp_out (column-major,
 // may be null) receives the iteration's back-projection sum_e K_e / D_e.
+// finalize = false stops after the back-projection (added into plan.bpT), so
+// several plans sharing the images can accumulate one MLEM update before
+// band_finalize.
 int band_iteration(BandPlan& plan, const DeviceGeometry& g,
                    const EventSoA& ev, int sensitivity_mode, int guard_zero,
                    hipStream_t stream, hipEvent_t t_fwd, hipEvent_t t_bwd,
                    std::string* err, double scale = 1.0,
-                   float* bp_out = nullptr);
+                   float* bp_out = nullptr, bool finalize = true);
+
+// rhoT[curT ^ 1] = rhoT[curT] * bpT [/ sensT] (x scale in FP64), bpT = 0
+// (bp_out, if given, keeps bpT), then curT flips.
+int band_finalize(BandPlan& plan, const DeviceGeometry& g,
+                  int sensitivity_mode, hipStream_t stream, std::string* err,
+                  double scale = 1.0, float* bp_out = nullptr);
 
 // Per-event forward projections of the current column-major image.
 int band_forward_only(BandPlan& plan, const DeviceGeometry& g,

@@ -339,6 +339,8 @@ struct strip_mlem_ctx {
   bool subsets_dirty = true;
   float* osemT = nullptr;  // 3 column-major images: rho (2), back-projection
   int osem_cur = 0;
+  // events per row-band plan of plain MLEM (more are split over plans)
+  int64_t plan_events = int64_t(1) << 25;
   // streamed uploads: pinned host and device staging, double-buffered
   hipStream_t copy_stream = nullptr;
   float* pinned[2] = {nullptr, nullptr};
@@ -577,6 +579,17 @@ int effective_algorithm(const strip_mlem_ctx* ctx) {
   return band_supported(ctx) ? 2 : 1;
 }
 
+// Number of row-band plans the events are split into: the ordered subsets,
+// or, for plain MLEM with the CPU semantics, ceil(n / plan_events) (the plans,
+// contiguous event ranges like the subsets', then accumulate one update
+// together, keeping every plan's 32-bit pair indices in range).
+int plan_parts(const strip_mlem_ctx* ctx) {
+  if (ctx->n_subsets > 1) return ctx->n_subsets;
+  if (ctx->mode.sensitivity != 0 || ctx->n_events <= ctx->plan_events)
+    return 1;
+  return (int)((ctx->n_events + ctx->plan_events - 1) / ctx->plan_events);
+}
+
 int record_marks(strip_mlem_ctx* ctx, hipEvent_t* marks) {
   for (int i = 0; i < 4; ++i) {
     marks[i] = next_event(ctx);
@@ -695,7 +708,7 @@ int ensure_images(strip_mlem_ctx* ctx, int64_t n_images) {
 
 int loglik_gradient_device(strip_mlem_ctx* ctx, const float* d_rho,
                            double* value, float* d_grad, double* grad_sigma) {
-  int rc = adjoint_ready(ctx, ctx->n_subsets == 1);
+  int rc = adjoint_ready(ctx, plan_parts(ctx) == 1);
   if (rc) return rc;
   if ((rc = ensure_images(ctx, 2))) return rc;
   std::string err;
@@ -705,7 +718,7 @@ int loglik_gradient_device(strip_mlem_ctx* ctx, const float* d_rho,
   if (strip::band_to_column_major(ctx->g, d_rho ? d_rho : ctx->rho[ctx->cur],
                                   rhoT, ctx->stream, &err))
     return strip::fail(STRIP_MLEM_ERR_HIP, err);
-  if (ctx->n_subsets > 1) {
+  if (plan_parts(ctx) > 1) {
     // the sum over the subsets' plans (each holds a part of the events)
     if ((rc = ensure_subsets(ctx))) return rc;
     if (d_grad && (rc = fill(ctx, gradT, ctx->n_pixels, 0.0f))) return rc;
@@ -801,7 +814,7 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
 
 // Plans of the ordered subsets: contiguous event ranges, shared images.
 int ensure_subsets(strip_mlem_ctx* ctx) {
-  const int S = ctx->n_subsets;
+  const int S = plan_parts(ctx);
   if (!ctx->subsets_dirty && (int)ctx->subsets.size() == S) {
     bool ready = true;
     for (auto& p : ctx->subsets) ready = ready && p.ready;
@@ -871,6 +884,49 @@ int osem_iterations(strip_mlem_ctx* ctx, int n_iterations) {
       HIP_TRY(hipEventRecord(marks[2], ctx->stream));
       HIP_TRY(hipEventRecord(marks[3], ctx->stream));
     }
+  }
+  if (strip::band_to_row_major(ctx->g, ctx->osemT + ctx->osem_cur * np,
+                               ctx->rho[ctx->cur], ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  return STRIP_MLEM_OK;
+}
+
+// n plain MLEM iterations over several plans (more events than one plan
+// holds): every plan adds its back-projection into the shared accumulator,
+// then one update rho' = rho * bp.
+int split_iterations(strip_mlem_ctx* ctx, int n_iterations) {
+  int rc = ensure_subsets(ctx);
+  if (rc) return rc;
+  std::string err;
+  const int64_t np = ctx->n_pixels;
+  if (strip::band_to_column_major(ctx->g, ctx->rho[ctx->cur], ctx->osemT,
+                                  ctx->stream, &err))
+    return strip::fail(STRIP_MLEM_ERR_HIP, err);
+  ctx->osem_cur = 0;
+  for (int it = 0; it < n_iterations; ++it) {
+    hipEvent_t marks[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (ctx->timing) {
+      if ((rc = record_marks(ctx, marks))) return rc;
+      HIP_TRY(hipEventRecord(marks[0], ctx->stream));
+    }
+    for (auto& p : ctx->subsets) {
+      p.curT = ctx->osem_cur;
+      if (p.n_events > 0 &&
+          strip::band_iteration(p, ctx->g, ctx->ev, 0, ctx->mode.guard_zero,
+                                ctx->stream, nullptr, nullptr, &err, 1.0,
+                                nullptr, false))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    }
+    if (ctx->timing) {
+      // the pass over the plans is reported as its forward part
+      HIP_TRY(hipEventRecord(marks[1], ctx->stream));
+      HIP_TRY(hipEventRecord(marks[2], ctx->stream));
+    }
+    strip::BandPlan& p0 = ctx->subsets[0];
+    if (strip::band_finalize(p0, ctx->g, 0, ctx->stream, &err))
+      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    ctx->osem_cur = p0.curT;
+    if (ctx->timing) HIP_TRY(hipEventRecord(marks[3], ctx->stream));
   }
   if (strip::band_to_row_major(ctx->g, ctx->osemT + ctx->osem_cur * np,
                                ctx->rho[ctx->cur], ctx->stream, &err))
@@ -1173,7 +1229,9 @@ int strip_mlem_iterate_async(strip_mlem_ctx* ctx, int n_iterations) {
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "negative iteration count");
   if (n_iterations == 0) return STRIP_MLEM_OK;
   if (ctx->n_subsets > 1) return osem_iterations(ctx, n_iterations);
-  if (effective_algorithm(ctx) == 2) return band_iterations(ctx, n_iterations);
+  if (effective_algorithm(ctx) == 2)
+    return plan_parts(ctx) > 1 ? split_iterations(ctx, n_iterations)
+                               : band_iterations(ctx, n_iterations);
   for (int i = 0; i < n_iterations; ++i) {
     int rc = launch_iteration(ctx);
     if (rc) return rc;
@@ -1240,6 +1298,17 @@ int strip_mlem_set_subsets(strip_mlem_ctx* ctx, int n_subsets) {
                        "row-band algorithm");
   if (n_subsets != ctx->n_subsets) {
     ctx->n_subsets = n_subsets;
+    ctx->subsets_dirty = true;
+  }
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_set_plan_events(strip_mlem_ctx* ctx, int64_t n) {
+  CTX_CHECK(ctx);
+  if (n < 1 || n > (int64_t(1) << 30))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "plan events must be 1..2^30");
+  if (n != ctx->plan_events) {
+    ctx->plan_events = n;
     ctx->subsets_dirty = true;
   }
   return STRIP_MLEM_OK;
@@ -1312,6 +1381,7 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
   ctx->band.tuning.fwd_pairs = forward_pairs;
   ctx->band.tuning.bwd_pairs = backward_pairs;
   ctx->band_dirty = true;
+  ctx->subsets_dirty = true;
   return STRIP_MLEM_OK;
 }
 
@@ -1322,10 +1392,13 @@ int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
   if (!ok(forward_group) || !ok(backward_group))
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "groups must be 1, 2 or 4");
   if (ctx->band.tuning.fwd_group != forward_group ||
-      ctx->band.tuning.bwd_group != backward_group) {
+      ctx->band.tuning.bwd_group != backward_group ||
+      ctx->band.tuning.auto_groups) {
     ctx->band.tuning.fwd_group = forward_group;
     ctx->band.tuning.bwd_group = backward_group;
+    ctx->band.tuning.auto_groups = false;
     ctx->band_dirty = true;
+    ctx->subsets_dirty = true;
   }
   return STRIP_MLEM_OK;
 }
@@ -1349,6 +1422,7 @@ int strip_mlem_set_backprojection_waves(strip_mlem_ctx* ctx, int waves) {
   if (ctx->band.tuning.bwd_waves != waves) {
     ctx->band.tuning.bwd_waves = waves;
     ctx->band_dirty = true;
+    ctx->subsets_dirty = true;
   }
   return STRIP_MLEM_OK;
 }
@@ -1358,6 +1432,7 @@ int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode) {
   if (mode != 0 && mode != 1)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "exp mode must be 0 or 1");
   ctx->band.tuning.recurrence = mode;
+  for (auto& p : ctx->subsets) p.tuning.recurrence = mode;
   return STRIP_MLEM_OK;
 }
 
@@ -1368,6 +1443,7 @@ int strip_mlem_set_span_table(strip_mlem_ctx* ctx, int enable) {
   if (ctx->band.tuning.spans != enable) {
     ctx->band.tuning.spans = enable;
     ctx->band_dirty = true;
+    ctx->subsets_dirty = true;
   }
   return STRIP_MLEM_OK;
 }
@@ -1406,12 +1482,22 @@ int strip_mlem_kernel_evaluations(strip_mlem_ctx* ctx, int64_t* count) {
   HIP_TRY(hipMemsetAsync(ctx->counter, 0, sizeof(unsigned long long),
                          ctx->stream));
   if (ctx->n_events > 0 && effective_algorithm(ctx) == 2) {
-    int rc = ensure_band(ctx);
-    if (rc) return rc;
     std::string err;
-    if (strip::band_count(ctx->band, ctx->g, ctx->ev, ctx->counter,
-                          ctx->stream, &err))
-      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    if (plan_parts(ctx) > 1) {
+      int rc = ensure_subsets(ctx);
+      if (rc) return rc;
+      for (auto& p : ctx->subsets)
+        if (p.n_events > 0 && strip::band_count(p, ctx->g, ctx->ev,
+                                                ctx->counter, ctx->stream,
+                                                &err))
+          return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    } else {
+      int rc = ensure_band(ctx);
+      if (rc) return rc;
+      if (strip::band_count(ctx->band, ctx->g, ctx->ev, ctx->counter,
+                            ctx->stream, &err))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    }
   } else if (ctx->n_events > 0) {
     const int grid = grid_for_events(ctx, ctx->n_events);
     hipLaunchKernelGGL(count_support_kernel, dim3(grid), dim3(ctx->threads), 0,
@@ -1441,14 +1527,35 @@ int strip_mlem_forward(strip_mlem_ctx* ctx, float* denominators) {
     ctx->denominators_cap = ctx->n_events;
   }
   if (effective_algorithm(ctx) == 2) {
-    int rc = ensure_band(ctx);
-    if (rc) return rc;
     std::string err;
-    if (strip::band_load_images(ctx->band, ctx->g, ctx->rho[ctx->cur],
-                                nullptr, ctx->stream, &err) ||
-        strip::band_forward_only(ctx->band, ctx->g, ctx->ev,
-                                 ctx->denominators, ctx->stream, &err))
-      return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    if (plan_parts(ctx) > 1) {
+      // every plan projects its event range into its part of the output
+      int rc = ensure_subsets(ctx);
+      if (rc) return rc;
+      ctx->subsets[0].curT = 0;
+      if (strip::band_load_images(ctx->subsets[0], ctx->g,
+                                  ctx->rho[ctx->cur], nullptr, ctx->stream,
+                                  &err))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+      int64_t lo = 0;
+      for (auto& p : ctx->subsets) {
+        p.curT = 0;
+        if (p.n_events > 0 &&
+            strip::band_forward_only(p, ctx->g, ctx->ev,
+                                     ctx->denominators + lo, ctx->stream,
+                                     &err))
+          return strip::fail(STRIP_MLEM_ERR_HIP, err);
+        lo += p.n_events;
+      }
+    } else {
+      int rc = ensure_band(ctx);
+      if (rc) return rc;
+      if (strip::band_load_images(ctx->band, ctx->g, ctx->rho[ctx->cur],
+                                  nullptr, ctx->stream, &err) ||
+          strip::band_forward_only(ctx->band, ctx->g, ctx->ev,
+                                   ctx->denominators, ctx->stream, &err))
+        return strip::fail(STRIP_MLEM_ERR_HIP, err);
+    }
     HIP_TRY(hipMemcpyAsync(denominators, ctx->denominators,
                            sizeof(float) * ctx->n_events,
                            hipMemcpyDeviceToHost, ctx->stream));
@@ -1665,6 +1772,7 @@ int batch_merge(strip_mlem_batch* b) {
         c->band.tuning.bwd_pairs != b->tuning.bwd_pairs ||
         c->band.tuning.fwd_group != b->tuning.fwd_group ||
         c->band.tuning.bwd_group != b->tuning.bwd_group ||
+        c->band.tuning.auto_groups != b->tuning.auto_groups ||
         c->band.tuning.spans != b->tuning.spans) {
       c->band.tuning = b->tuning;
       c->band_dirty = true;
@@ -1672,16 +1780,19 @@ int batch_merge(strip_mlem_batch* b) {
     int rc = ensure_band(c);
     if (rc) return rc;
     const strip::BandPlan& P = c->band;
+    // the pairs per wavefront step only shape the launches: the batch takes
+    // the smallest of its members' (their windows differ in width)
     if (i == 0) {
       b->H_fwd = P.fwd.H;
       b->H_bwd = P.bwd.H;
       b->G_fwd = P.fwd_group;
       b->G_bwd = P.bwd_group;
-    } else if (P.fwd.H != b->H_fwd || P.bwd.H != b->H_bwd ||
-               P.fwd_group != b->G_fwd || P.bwd_group != b->G_bwd) {
+    } else if (P.fwd.H != b->H_fwd || P.bwd.H != b->H_bwd) {
       return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
-                         "members ended with different band layouts");
+                         "members ended with different band heights");
     }
+    b->G_fwd = std::min(b->G_fwd, P.fwd_group);
+    b->G_bwd = std::min(b->G_bwd, P.bwd_group);
     b->fwd_Wt = std::max(b->fwd_Wt, P.fwd.Wt);
     b->bwd_Wt = std::max(b->bwd_Wt, P.bwd.Wt);
     b->spans = b->spans && P.spans != nullptr;
@@ -1972,10 +2083,11 @@ int strip_mlem_batch_set_groups(strip_mlem_batch* b, int forward_group,
   int rc = batch_store_images(b);
   if (rc) return rc;
   if (b->tuning.fwd_group != forward_group ||
-      b->tuning.bwd_group != backward_group)
+      b->tuning.bwd_group != backward_group || b->tuning.auto_groups)
     b->generations.clear();  // the members' plans are rebuilt and re-merged
   b->tuning.fwd_group = forward_group;
   b->tuning.bwd_group = backward_group;
+  b->tuning.auto_groups = false;
   return STRIP_MLEM_OK;
 }
 

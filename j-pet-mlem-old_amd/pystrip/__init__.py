@@ -168,6 +168,7 @@ def load_library():
         "strip_mlem_reserve_events": (ctypes.c_int, [P, i64]),
         "strip_mlem_append_events": (ctypes.c_int, [P, fptr, i64]),
         "strip_mlem_set_subsets": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_set_plan_events": (ctypes.c_int, [P, i64]),
         "strip_mlem_image_total": (ctypes.c_int, [P, dptr]),
         "strip_mlem_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(P)]),
         "strip_mlem_batch_destroy": (None, [P]),
@@ -253,6 +254,7 @@ def exported_symbols():
         "strip_phantom_generate", "strip_mlem_generate_events",
         "strip_phantom_random_blocks", "strip_phantom_last_error",
         "strip_mlem_set_subsets", "strip_mlem_image_total",
+        "strip_mlem_set_plan_events",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
@@ -445,6 +447,11 @@ class Reconstruction:
     def set_subsets(self, n_subsets: int):
         """Ordered subsets by event index (1 = plain MLEM)."""
         _check(self._lib.strip_mlem_set_subsets(self._ctx, int(n_subsets)))
+
+    def set_plan_events(self, n: int):
+        """Events per row-band plan of plain MLEM; more are split over
+        several plans that accumulate one update."""
+        _check(self._lib.strip_mlem_set_plan_events(self._ctx, int(n)))
 
     def image_total(self) -> float:
         """Sum of the image's pixels, accumulated in FP64 on the device."""

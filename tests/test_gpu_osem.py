@@ -94,3 +94,70 @@ def test_image_total_fp64(pystrip_mod, phantom_mod):
     ref = float(rec.rho.astype(np.float64).sum())
     assert abs(total - ref) <= 1e-9 * ref
     assert abs(total - n_pos) <= 1e-5 * n_pos
+
+
+@pytest.mark.parametrize("plan_events", [20_000, 33_333])
+def test_split_plans_equal_single_plan(pystrip_mod, phantom_mod, oracle_mod,
+                                       plan_events):
+    # plain MLEM over several plans (contiguous event ranges accumulating one
+    # update): same support, forward projections and images as one plan
+    print_case("plain MLEM split over plans of", plan_events, "events")
+    w = phantom_mod.WORKLOADS[CFG1]
+    ev = w.events(70_000, seed=39)
+    og = oracle_mod.geometry(*w.oracle_geometry_tuple())
+    a = pystrip_mod.Reconstruction(w.geometry())
+    a.set_events(ev)
+    b = pystrip_mod.Reconstruction(w.geometry())
+    b.set_plan_events(plan_events)
+    b.set_events(ev)
+    assert a.kernel_evaluations() == b.kernel_evaluations()
+    rng = np.random.default_rng(7)
+    rho = rng.uniform(0.5, 2.0, size=a.shape).astype(np.float32)
+    a.rho = rho
+    b.rho = rho
+    da, db = a.forward(), b.forward()
+    nz = da > 0
+    assert np.array_equal(nz, db > 0)
+    assert np.max(np.abs(da[nz] - db[nz]) / da[nz]) < 1e-6
+    a.rho = np.ones(a.shape, np.float32)
+    b.rho = np.ones(b.shape, np.float32)
+    a.iterate(3)
+    b.iterate(3)
+    assert oracle_mod.rel_l2(b.rho, a.rho) <= 1e-5
+    assert oracle_mod.rel_l2(b.rho, oracle_mod.mlem(og, ev, iterations=3)) \
+        <= 1e-4
+    # mass after one more update: sum rho = #{e : D_e > 0} (FP64 total)
+    n_pos = int((b.forward() > 0).sum())
+    b.iterate(1)
+    assert abs(b.image_total() - n_pos) <= 1e-5 * n_pos
+
+
+def test_osem_largest_grid_invariants(pystrip_mod, phantom_mod):
+    # the 2000 x 800 grid at 0.5 mm with 16 subsets, events generated on the
+    # device in 1e7-event chunks and streamed: after a full iteration (its last
+    # subset update) the FP64 image total equals N / N_s x #{e in s : D_e > 0}
+    # for the last subset s, and the log-likelihood gradient is finite
+    print_case("OSEM 2000x800 @0.5 mm, 4e7 events, 16 subsets")
+    w = phantom_mod.WORKLOADS["strip-2000x800-0.5mm-2e8"]
+    n, chunk, S = 40_000_000, 10_000_000, 16
+    rec = pystrip_mod.Reconstruction(w.geometry())
+    rec.reserve_events(n)
+    for k in range(0, n, chunk):
+        out = pystrip_mod.generate_events(w.phantom, w.radius, w.length,
+                                          w.sigma_z, w.sigma_dl, chunk,
+                                          seed=1000 + k // chunk)
+        rec.append_events(out["responses"][:chunk])
+    assert rec.n_events == n
+    rec.set_subsets(S)
+    rec.iterate(1)
+    total = rec.image_total()
+    # the last subset's events with D > 0 (the update keeps rho > 0 on their
+    # supports, so the set is the one before the update)
+    last = slice(n * (S - 1) // S, n)
+    n_pos = int((rec.forward()[last] > 0).sum())
+    expect = n / (n - n * (S - 1) // S) * n_pos
+    print("image total", total, "expected", expect)
+    assert abs(total - expect) <= 1e-4 * expect
+    v, g, gs = rec.loglik_gradient()
+    print("loglik", v, "d/d sigma", gs)
+    assert np.isfinite(v) and np.all(np.isfinite(gs)) and np.all(np.isfinite(g))
