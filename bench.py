@@ -111,6 +111,59 @@ def adjoint_timing(pystrip, phantom, name, sigma, iterations):
     }
 
 
+def osem_timing(pystrip, phantom, name, n_subsets, iterations, chunk):
+    """Ordered subsets at the largest size: the workload's events generated
+    on the device in `chunk`-event pieces and streamed back in through the
+    pinned double-buffered appends, n_subsets subsets by event index, one
+    untimed and `iterations` - 1 timed full iterations, then the
+    log-likelihood gradient over the final image."""
+    w = phantom.WORKLOADS[name]
+    rec = pystrip.Reconstruction(w.geometry(), device=0)
+    rec.reserve_events(w.n_events)
+    t_gen = t_app = 0.0
+    for k in range(0, w.n_events, chunk):
+        m = min(chunk, w.n_events - k)
+        t = time.perf_counter()
+        out = pystrip.generate_events(w.phantom, w.radius, w.length,
+                                      w.sigma_z, w.sigma_dl, m,
+                                      seed=1000 + k // chunk)
+        t_gen += time.perf_counter() - t
+        t = time.perf_counter()
+        rec.append_events(out["responses"][:m])
+        t_app += time.perf_counter() - t
+    rec.set_subsets(n_subsets)
+    t = time.perf_counter()
+    n_evals = rec.kernel_evaluations()
+    t_plan = time.perf_counter() - t
+    rec.iterate(1)
+    rec.synchronize()
+    t = time.perf_counter()
+    rec.iterate(iterations - 1)
+    rec.synchronize()
+    t_it = (time.perf_counter() - t) / (iterations - 1)
+    t = time.perf_counter()
+    _, _, gs = rec.loglik_gradient(want_grad_rho=False)
+    t_grad = time.perf_counter() - t
+    out = {
+        "workload": name,
+        "events": w.n_events,
+        "subsets": n_subsets,
+        "chunk_events": chunk,
+        "generation_s": t_gen,
+        "streamed_append_s": t_app,
+        "plans_s": t_plan,
+        "kernel_evaluations_per_iteration": n_evals,
+        "full_iteration_s": t_it,
+        "event_updates_per_s": w.n_events / t_it,
+        "kernel_evaluations_per_s": n_evals / t_it,
+        "image_total_fp64": rec.image_total(),
+        "loglik_sigma_gradient_s": t_grad,
+        "d_loglik_d_sigma": [float(x) for x in gs],
+    }
+    rec.close()
+    return out
+
+
 def batched_timing(pystrip, phantom, n_members, iterations):
     """Batched form against the plain form on the same members: n_members
     reconstructions with uneven event counts and grids, `iterations` MLEM
@@ -171,6 +224,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--batch-members", type=int, default=64)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--osem-workload", default="strip-2000x800-0.5mm-2e8")
+    ap.add_argument("--no-osem", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -296,6 +351,10 @@ def main():
         rec.close()
         line["batched"] = batched_timing(pystrip, phantom, args.batch_members,
                                          10)
+    if not args.no_osem:
+        rec.close()
+        line["osem_largest"] = osem_timing(pystrip, phantom, args.osem_workload,
+                                           16, 5, 10_000_000)
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, threads)
