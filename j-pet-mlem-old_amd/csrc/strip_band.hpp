@@ -1,20 +1,23 @@
 // Row-band algorithm of the strip MLEM iteration (host interface).
 //
 // Each event is split into (event, band) pairs, a band being H consecutive
-// image rows (H = 64 or 32).  Pairs are radix-sorted once per event upload by
-// (band, bbox x0) and cut into tasks that never leave one (band, x0 bucket)
-// group, so every task touches at most Wt = bucket + max bbox width columns.
-// One iteration then runs
-//   band_forward   a workgroup stages the task's rho window (Wt columns x H
-//                  rows, column-major: word = column*H + row) in LDS; one
-//                  image row per lane (64/H lanes per row), so lanes never
-//                  share an LDS bank; partial D per pair;
+// image rows (H = 16, 32 or 64).  Pairs are radix-sorted once per event
+// upload by (band, first column the band's row runs reach) and cut into
+// tasks that never leave one (band, column bucket) group; every task's LDS
+// window spans only the columns its pairs reach (at most bucket + widest
+// run slice).  One iteration then runs
+//   band_forward   a workgroup stages the task's rho window (columns x H
+//                  rows, column-major: word = column*H + row) in LDS; a
+//                  wavefront step takes G pairs, one per group of 64/G lanes,
+//                  with 64/(G*H) lanes per image row, so lanes of a group
+//                  never share an LDS bank; partial D per pair;
 //   band_inverse   D_e = sum of the event's partials, inv_e = 1/D_e (guarded);
-//   band_backward  each wavefront owns a private Wt x H window in LDS and
-//                  accumulates K * inv_e of its task's pairs with plain
-//                  read-modify-write (lanes own distinct rows, waves own their
-//                  window: no LDS atomics), then adds the window to the
-//                  column-major back-projection image with coalesced atomics;
+//   band_backward  each wavefront owns G private windows in LDS (one per lane
+//                  group) and accumulates K * inv_e of its task's pairs with
+//                  plain read-modify-write (lanes own distinct words, groups
+//                  and waves own their windows: no LDS atomics), then adds the
+//                  summed windows to the column-major back-projection image
+//                  with coalesced atomics;
 //   band_finalize  rho'(p) = rho(p) * bp(p) [/ s(p)], column-major.
 // The in-ellipse pixels of an event in one image row form one run of
 // columns; the runs of every bbox row of every event are found once per
