@@ -167,8 +167,8 @@ int strip_mlem_get_algorithm(const strip_mlem_ctx* ctx);
 /* Tile shapes of the row-band algorithm (the knobs an autotuner turns in
  * place of -B/-W): forward_rows / backward_rows = image rows per band of the
  * forward-projection and back-projection kernels (16, 32 or 64),
- * bucket_columns = x0 bucket width bounding a task's column window (16, 32
- * or 64), forward_pairs / backward_pairs = (event, band) pairs per forward
+ * bucket_columns = width of the column buckets bounding a task's window (8, 16,
+ * 32 or 64), forward_pairs / backward_pairs = (event, band) pairs per forward
  * workgroup / per back-projection wavefront.  Takes effect at the next
  * iteration. */
 int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,

@@ -1193,8 +1193,10 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
     }
   }
   const BandTuning& tu = plan.tuning;
-  const int bucket = (tu.bucket == 16 || tu.bucket == 64) ? tu.bucket : 32;
-  const int shift = bucket == 16 ? 4 : (bucket == 64 ? 6 : 5);
+  const int bucket = (tu.bucket == 8 || tu.bucket == 16 || tu.bucket == 64)
+                         ? tu.bucket
+                         : 32;
+  const int shift = bucket == 8 ? 3 : bucket == 16 ? 4 : (bucket == 64 ? 6 : 5);
   // a task reaches at most its bucket plus the widest bbox (the bound used
   // to pick band heights); its actual window is set by its pairs' runs
   plan.Wt = bucket + max_width;

@@ -1371,10 +1371,11 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
   if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
   if ((forward_rows != 16 && forward_rows != 32 && forward_rows != 64) ||
       (backward_rows != 16 && backward_rows != 32 && backward_rows != 64) ||
-      (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
+      (bucket_columns != 8 && bucket_columns != 16 && bucket_columns != 32 &&
+       bucket_columns != 64) ||
       forward_pairs < 1 || backward_pairs < 1)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "band rows 16/32/64, bucket 16/32/64, pairs >= 1");
+                       "band rows 16/32/64, bucket 8/16/32/64, pairs >= 1");
   ctx->band.tuning.fwd_rows = forward_rows;
   ctx->band.tuning.bwd_rows = backward_rows;
   ctx->band.tuning.bucket = bucket_columns;
@@ -2048,11 +2049,12 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
   BATCH_CHECK(b);
   if ((forward_rows != 16 && forward_rows != 32 && forward_rows != 64) ||
       (backward_rows != 16 && backward_rows != 32 && backward_rows != 64) ||
-      (bucket_columns != 16 && bucket_columns != 32 && bucket_columns != 64) ||
+      (bucket_columns != 8 && bucket_columns != 16 && bucket_columns != 32 &&
+       bucket_columns != 64) ||
       forward_pairs < 1 || backward_pairs < 1 || grid_factor < 1 ||
       grid_factor > 64)
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "band rows 16/32/64, bucket 16/32/64, pairs >= 1, "
+                       "band rows 16/32/64, bucket 8/16/32/64, pairs >= 1, "
                        "grid factor 1..64");
   int rc = batch_store_images(b);
   if (rc) return rc;
