@@ -92,21 +92,27 @@ __global__ __launch_bounds__(256) void band_bounds_kernel(
   }
 }
 
-// keys of the event's pairs: band << 16 | first column the band's runs reach
+// keys of the event's pairs: band << 16 | (first column the band's runs
+// reach, rounded down to the 2^shift-column bucket) | a width code in the low
+// shift bits, so a bucket's pairs sort by the width of their run slices and
+// the pairs a wavefront step takes together have similar trip counts
 __global__ __launch_bounds__(256) void emit_pairs_kernel(
     const BandRecord* __restrict__ rec, const int* __restrict__ bounds,
-    int max_bands, int64_t n, int H, const int* __restrict__ counts,
-    const int* __restrict__ offsets, uint32_t* __restrict__ keys,
-    int* __restrict__ values) {
+    int max_bands, int64_t n, int H, int shift,
+    const int* __restrict__ counts, const int* __restrict__ offsets,
+    uint32_t* __restrict__ keys, int* __restrict__ values) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int c = counts[e];
   if (!c) return;
   const int b0 = (int)(rec[e].by & 0xffff) / H;
   const int base = offsets[e];
+  const uint32_t code_max = (1u << shift) - 1;
   for (int k = 0; k < c; ++k) {
-    const uint32_t lo = (uint32_t)bounds[e * max_bands + k] & 0xffff;
-    keys[base + k] = ((uint32_t)(b0 + k) << 16) | lo;
+    const uint32_t bd = (uint32_t)bounds[e * max_bands + k];
+    const uint32_t lo = bd & 0xffff, width = (bd >> 16) - lo;
+    const uint32_t code = min(width >> (7 - shift), code_max);
+    keys[base + k] = ((uint32_t)(b0 + k) << 16) | (lo & ~code_max) | code;
     values[base + k] = (int)e;
   }
 }
@@ -982,8 +988,8 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
                        L.max_bands, L.bounds);
     BAND_TRY(hipGetLastError());
     hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       rec, L.bounds, L.max_bands, n, H, counts, offsets, keys,
-                       L.pair_event);
+                       rec, L.bounds, L.max_bands, n, H, shift, counts,
+                       offsets, keys, L.pair_event);
     BAND_TRY(hipGetLastError());
     int end_bit = 16;
     while ((1 << (end_bit - 16)) < L.n_bands + 1) ++end_bit;
