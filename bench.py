@@ -37,8 +37,8 @@ FLOPS_PER_KERNEL_EVAL = 16
 # by the counter's reading for a plain 256 MiB copy kernel (x2.0 and x1.0);
 # recorded in profiles/band_kernels_strip-1000x400-1mm-2e7_counters.md
 MEASURED_TRAFFIC_BYTES = {
-    "back-projection": (50.04 + 0.87) * 1e9,
-    "forward projection": (21.86 + 2.79) * 1e9,
+    "back-projection": (39.91 + 1.00) * 1e9,
+    "forward projection": (20.37 + 2.65) * 1e9,
 }
 TRAFFIC_WORKLOAD = "strip-1000x400-1mm-2e7"
 
