@@ -38,7 +38,7 @@ struct BandTuning {
   int bwd_rows = 16;     // back-projection band height: 64, 32 or 16
   int bwd_waves = 0;     // wavefronts per back-projection workgroup (0 = as
                          // many as fit, at most 4)
-  int bucket = 16;       // x0 bucket width in columns: 16, 32 or 64
+  int bucket = 8;        // column bucket width: 8, 16, 32 or 64
   int fwd_pairs = 2048;  // pairs per forward workgroup task
   int bwd_pairs = 512;   // pairs per back-projection wavefront task
   // pairs a wavefront processes together (1, 2 or 4, at most 64 / rows):

@@ -2174,7 +2174,7 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
   double best = 0.0;
   if ((rc = time_it(cur, &best))) return rc;
   const std::vector<std::vector<int>> choices = {
-      {64, 32, 16}, {16, 32, 64}, {16, 32, 64}, {1024, 2048, 4096},
+      {64, 32, 16}, {16, 32, 64}, {8, 16, 32}, {1024, 2048, 4096},
       {256, 512, 1024}, {2, 4, 8}, {1, 2, 4}, {1, 2, 4}};
   for (int k = 0; k < 8; ++k) {
     for (int v : choices[k]) {

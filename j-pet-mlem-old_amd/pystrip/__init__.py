@@ -370,7 +370,7 @@ class Reconstruction:
         _check(self._lib.strip_mlem_set_algorithm(self._ctx, algorithm))
 
     def set_band_tiling(self, forward_rows=32, backward_rows=16,
-                        bucket_columns=16, forward_pairs=2048,
+                        bucket_columns=8, forward_pairs=2048,
                         backward_pairs=512):
         _check(self._lib.strip_mlem_set_band_tiling(
             self._ctx, forward_rows, backward_rows, bucket_columns,
@@ -658,7 +658,7 @@ class Batch:
                                                   ctypes.byref(c)))
         return _MemberView(self._lib, c, self.geometries[index])
 
-    def set_tiling(self, forward_rows=32, backward_rows=16, bucket_columns=16,
+    def set_tiling(self, forward_rows=32, backward_rows=16, bucket_columns=8,
                    forward_pairs=2048, backward_pairs=512, grid_factor=4):
         _check(self._lib.strip_mlem_batch_set_tiling(
             self._b, forward_rows, backward_rows, bucket_columns,
