@@ -394,7 +394,6 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   const int end = min(task.z, begin + share);
   const float qc = g.C * kExp2Scale;
   const float two_qc_step = 2.0f * qc * step;
-  const float G2 = __builtin_amdgcn_exp2f(two_qc_step * step);
   PairBatch next = gather_batch<false>(rec, pair_event, nullptr, begin, end,
                                        lane);
   for (int base = begin; base < end; base += 64) {
