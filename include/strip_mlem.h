@@ -178,9 +178,9 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
  * processes together: 1, 2 or 4 (at most 64 / band rows).  Lane groups of
  * 64 / G lanes then take one pair each, so one instruction stream serves G
  * pairs (the back-projection keeps one LDS window per group; G is halved
- * until they fit).  Until this is called the defaults are (2, 2) with the
+ * until they fit).  (0, 0) = automatic, the default: (2, 4) with the
  * back-projection's G also halved while its windows would leave fewer than
- * 16 wavefronts per CU.  strip_mlem_get_band_groups returns the values in
+ * 12 wavefronts per CU.  strip_mlem_get_band_groups returns the values in
  * use. */
 int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
                                int backward_group);
@@ -293,7 +293,8 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* batch, int forward_rows,
                                 int grid_factor);
 int strip_mlem_batch_get_tiling(const strip_mlem_batch* batch, int* tiling);
 /* Shared pairs per wavefront step of the members' plans (as
- * strip_mlem_set_band_groups). */
+ * strip_mlem_set_band_groups; with (0, 0) each member's automatic choice,
+ * the batch launching with the smallest). */
 int strip_mlem_batch_set_groups(strip_mlem_batch* batch, int forward_group,
                                 int backward_group);
 int strip_mlem_batch_get_groups(const strip_mlem_batch* batch,

@@ -900,7 +900,7 @@ void free_list(PairList& L) {
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
-constexpr size_t kMinBackprojectionWaves = 16;  // per CU, see band_prepare
+constexpr size_t kMinBackprojectionWaves = 12;  // per CU, see band_prepare
 
 // Sorted (event, band) pairs of band height H for the resident events, and
 // the index of the first pair of every (band, x0 >> shift) group.
@@ -1248,8 +1248,10 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if (plan.bwd_group == 3) plan.bwd_group = 2;
   plan.fwd_lds = (size_t)plan.fwd.Wt * Hf * sizeof(float);
   // back-projection: G windows per wavefront; with the default groups, fewer
-  // pairs per step when the windows would leave fewer than 16 wavefronts per
-  // CU (the LDS-latency bound then costs more than the shared prologue saves)
+  // pairs per step when the windows would leave fewer than 12 wavefronts per
+  // CU (the LDS-latency bound then costs more than the shared prologue saves):
+  // 4 pairs for narrow windows, 2 for the 1 mm head workload's 76 columns,
+  // 1 for the 0.5 mm grid's 119
   size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
   const size_t min_waves = tu.auto_groups ? kMinBackprojectionWaves : 1;
   while (plan.bwd_group > 1 &&

@@ -45,9 +45,9 @@ struct BandTuning {
   // lane groups of 64 / G lanes take one pair each, with 64 / (G * rows)
   // lanes per image row
   int fwd_group = 2;
-  int bwd_group = 2;
+  int bwd_group = 4;
   // true until a caller sets the groups: the back-projection's G then also
-  // drops while its windows would leave fewer than 16 wavefronts per CU
+  // drops while its windows would leave fewer than 12 wavefronts per CU
   bool auto_groups = true;
   // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
   // two-term recurrence e[k+1] = e[k] r[k], r[k+1] = r[k] G of exp2 of a

@@ -1389,15 +1389,22 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
 int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
                                int backward_group) {
   if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  const bool automatic = forward_group == 0 && backward_group == 0;
   auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
-  if (!ok(forward_group) || !ok(backward_group))
-    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "groups must be 1, 2 or 4");
+  if (!automatic && (!ok(forward_group) || !ok(backward_group)))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "groups must be 1, 2 or 4 (or both 0: automatic)");
+  const strip::BandTuning defaults;
+  if (automatic) {
+    forward_group = defaults.fwd_group;
+    backward_group = defaults.bwd_group;
+  }
   if (ctx->band.tuning.fwd_group != forward_group ||
       ctx->band.tuning.bwd_group != backward_group ||
-      ctx->band.tuning.auto_groups) {
+      ctx->band.tuning.auto_groups != automatic) {
     ctx->band.tuning.fwd_group = forward_group;
     ctx->band.tuning.bwd_group = backward_group;
-    ctx->band.tuning.auto_groups = false;
+    ctx->band.tuning.auto_groups = automatic;
     ctx->band_dirty = true;
     ctx->subsets_dirty = true;
   }
@@ -2079,17 +2086,25 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
 int strip_mlem_batch_set_groups(strip_mlem_batch* b, int forward_group,
                                 int backward_group) {
   BATCH_CHECK(b);
+  const bool automatic = forward_group == 0 && backward_group == 0;
   auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
-  if (!ok(forward_group) || !ok(backward_group))
-    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "groups must be 1, 2 or 4");
+  if (!automatic && (!ok(forward_group) || !ok(backward_group)))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "groups must be 1, 2 or 4 (or both 0: automatic)");
+  const strip::BandTuning defaults;
+  if (automatic) {
+    forward_group = defaults.fwd_group;
+    backward_group = defaults.bwd_group;
+  }
   int rc = batch_store_images(b);
   if (rc) return rc;
   if (b->tuning.fwd_group != forward_group ||
-      b->tuning.bwd_group != backward_group || b->tuning.auto_groups)
+      b->tuning.bwd_group != backward_group ||
+      b->tuning.auto_groups != automatic)
     b->generations.clear();  // the members' plans are rebuilt and re-merged
   b->tuning.fwd_group = forward_group;
   b->tuning.bwd_group = backward_group;
-  b->tuning.auto_groups = false;
+  b->tuning.auto_groups = automatic;
   return STRIP_MLEM_OK;
 }
 
@@ -2156,12 +2171,22 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
     saved[i].resize(b->items[i]->n_pixels);
     if ((rc = strip_mlem_get_rho(b->items[i], saved[i].data()))) return rc;
   }
-  int cur[8];
+  // coordinates: the six tiling knobs, then one index into the pairs-per-step
+  // choices ((0, 0) = automatic)
+  static const int kGroups[][2] = {{0, 0}, {1, 1}, {2, 1}, {2, 2},
+                                   {2, 4}, {4, 2}, {4, 4}};
+  constexpr int kNGroups = sizeof(kGroups) / sizeof(kGroups[0]);
+  int cur[7];
   strip_mlem_batch_get_tiling(b, cur);
-  strip_mlem_batch_get_groups(b, cur + 6, cur + 7);
+  cur[6] = 0;
+  if (!b->tuning.auto_groups)
+    for (int g = 1; g < kNGroups; ++g)
+      if (kGroups[g][0] == b->tuning.fwd_group &&
+          kGroups[g][1] == b->tuning.bwd_group)
+        cur[6] = g;
   auto time_it = [&](const int* t, double* ms) -> int {
     int r = strip_mlem_batch_set_tiling(b, t[0], t[1], t[2], t[3], t[4], t[5]);
-    if (!r) r = strip_mlem_batch_set_groups(b, t[6], t[7]);
+    if (!r) r = strip_mlem_batch_set_groups(b, kGroups[t[6]][0], kGroups[t[6]][1]);
     if (r) return r;
     const bool graph = b->use_graph;
     b->use_graph = false;
@@ -2175,25 +2200,26 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
   if ((rc = time_it(cur, &best))) return rc;
   const std::vector<std::vector<int>> choices = {
       {64, 32, 16}, {16, 32, 64}, {8, 16, 32}, {1024, 2048, 4096},
-      {256, 512, 1024}, {2, 4, 8}, {1, 2, 4}, {1, 2, 4}};
-  for (int k = 0; k < 8; ++k) {
+      {256, 512, 1024}, {2, 4, 8}, {0, 1, 2, 3, 4, 5, 6}};
+  for (int k = 0; k < 7; ++k) {
     for (int v : choices[k]) {
       if (v == cur[k]) continue;
-      int cand[8];
-      std::copy(cur, cur + 8, cand);
+      int cand[7];
+      std::copy(cur, cur + 7, cand);
       cand[k] = v;
       double ms = 0.0;
       if (time_it(cand, &ms)) continue;  // a shape that does not fit
       if (ms < best) {
         best = ms;
-        std::copy(cand, cand + 8, cur);
+        std::copy(cand, cand + 7, cur);
       }
     }
   }
   // the chosen tiling, its plans built and merged before returning
   if ((rc = strip_mlem_batch_set_tiling(b, cur[0], cur[1], cur[2], cur[3],
                                         cur[4], cur[5])) ||
-      (rc = strip_mlem_batch_set_groups(b, cur[6], cur[7])) ||
+      (rc = strip_mlem_batch_set_groups(b, kGroups[cur[6]][0],
+                                        kGroups[cur[6]][1])) ||
       (rc = batch_merge(b)))
     return rc;
   for (size_t i = 0; i < b->items.size(); ++i)

@@ -380,9 +380,9 @@ class Reconstruction:
         """0 = exp2 per pixel, 1 = restarted exp recurrence along rows."""
         _check(self._lib.strip_mlem_set_exp_mode(self._ctx, int(mode)))
 
-    def set_band_groups(self, forward_group: int = 2, backward_group: int = 2):
+    def set_band_groups(self, forward_group: int = 0, backward_group: int = 0):
         """(Event, band) pairs one wavefront processes together in the
-        forward / back-projection kernels (1, 2 or 4)."""
+        forward / back-projection kernels (1, 2 or 4; both 0 = automatic)."""
         _check(self._lib.strip_mlem_set_band_groups(
             self._ctx, int(forward_group), int(backward_group)))
 
@@ -669,7 +669,7 @@ class Batch:
         _check(self._lib.strip_mlem_batch_get_tiling(self._b, t))
         return tuple(t)
 
-    def set_groups(self, forward_group=2, backward_group=2):
+    def set_groups(self, forward_group=0, backward_group=0):
         _check(self._lib.strip_mlem_batch_set_groups(
             self._b, int(forward_group), int(backward_group)))
 
