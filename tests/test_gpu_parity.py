@@ -298,3 +298,30 @@ def test_band_groups(pystrip_mod, phantom_mod, oracle_mod, rows, groups,
     e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=3))
     print("rel-L2", e, "tiling", rec.band_tiling())
     assert e <= 1e-4
+
+
+def test_full_size_against_reference_image(pystrip_mod, phantom_mod,
+                                           oracle_mod):
+    # the headline workload at full size (2e7 seeded events, 20 iterations)
+    # against the reference's own CPU reconstruction of the same events
+    # (tests/golden/strip_1000x400_2e7_20it_ref.npz, make_full_image.py)
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                        "strip_1000x400_2e7_20it_ref.npz")
+    gold = np.load(path)
+    print_case("full size", CFG2, int(gold["n_events"]), "events,",
+               int(gold["iterations"]), "iterations")
+    w = phantom_mod.WORKLOADS[CFG2]
+    assert w.seed == int(gold["seed"])
+    ev = w.events(int(gold["n_events"]))
+    rec = pystrip_mod.Reconstruction(w.geometry())
+    rec.set_events(ev)
+    rec.iterate(int(gold["iterations"]))
+    e32 = oracle_mod.rel_l2(rec.rho, gold["rho_f32"])
+    print("rel-L2 vs the reference FP32 image", e32)
+    if "rho_f64" in gold:
+        print("rel-L2 vs the reference FP64 image",
+              oracle_mod.rel_l2(rec.rho, gold["rho_f64"]),
+              "reference FP32 vs FP64",
+              oracle_mod.rel_l2(gold["rho_f32"], gold["rho_f64"]))
+    assert e32 <= 1e-4
