@@ -177,11 +177,14 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
 /* (Event, band) pairs one wavefront of the forward / back-projection kernel
  * processes together: 1, 2 or 4 (at most 64 / band rows).  Lane groups of
  * 64 / G lanes then take one pair each, so one instruction stream serves G
- * pairs (the back-projection keeps one LDS window per group; G is halved
- * until they fit).  (0, 0) = automatic, the default: (2, 4) with the
- * back-projection's G also halved while its windows would leave fewer than
- * 12 wavefronts per CU.  strip_mlem_get_band_groups returns the values in
- * use. */
+ * pairs.  The back-projection keeps one LDS window per group: with set
+ * groups every task takes G (halved until one wavefront's windows fit);
+ * (0, 0) = automatic, the default: forward 2, and every back-projection
+ * wavefront owns the LDS that keeps strip_mlem_set_backprojection_occupancy
+ * wavefronts per CU resident, each task taking the most pairs per step, up
+ * to 4 and halving, whose windows fit there (wide windows fewer, narrow
+ * ones more).  strip_mlem_get_band_groups returns the forward G and the
+ * back-projection's most. */
 int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
                                int backward_group);
 int strip_mlem_get_band_groups(strip_mlem_ctx* ctx, int* forward_group,
@@ -196,6 +199,11 @@ int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode);
 /* Wavefronts per back-projection workgroup (each owns an LDS window);
  * 0 = as many as fit, at most 4. */
 int strip_mlem_set_backprojection_waves(strip_mlem_ctx* ctx, int waves);
+/* Back-projection wavefronts per CU (4..32, default 16) whose LDS windows the
+ * automatic groups keep resident; no counterpart in the reference (it
+ * replaces part of -B/-W). */
+int strip_mlem_set_backprojection_occupancy(strip_mlem_ctx* ctx,
+                                            int cu_waves);
 /* Row runs of the row-band algorithm: 1 (default) = the in-ellipse column run
  * of every bbox row of every event is solved once per upload and stored
  * (2 bytes per row; used when every clamped bbox is at most 255 columns wide
@@ -299,12 +307,21 @@ int strip_mlem_batch_set_groups(strip_mlem_batch* batch, int forward_group,
                                 int backward_group);
 int strip_mlem_batch_get_groups(const strip_mlem_batch* batch,
                                 int* forward_group, int* backward_group);
+/* Shared back-projection occupancy of the members' plans (as
+ * strip_mlem_set_backprojection_occupancy; the batch gives every wavefront
+ * the largest member's LDS budget). */
+int strip_mlem_batch_set_backprojection_occupancy(strip_mlem_batch* batch,
+                                                  int cu_waves);
+int strip_mlem_batch_get_backprojection_occupancy(
+    const strip_mlem_batch* batch, int* cu_waves);
 /* 1 (default) = replay graph-recorded iterations, 0 = launch each. */
 int strip_mlem_batch_set_graph(strip_mlem_batch* batch, int enable);
 /* Device time (ms) of the last strip_mlem_batch_iterate call. */
 int strip_mlem_batch_last_ms(const strip_mlem_batch* batch, double* ms);
-/* Autotuner: coordinate search over the tiling knobs, each candidate timed on
- * batch iterations; keeps the fastest and restores the members' images.
+/* Autotuner: coordinate search over the tiling knobs, the pairs per step and
+ * the back-projection occupancy, each candidate timed on batch iterations
+ * (sweeps repeat until one changes nothing, at most three); keeps the
+ * fastest and restores the members' images.
  * *best_ms (may be NULL) = time of one iteration with the chosen tiling. */
 int strip_mlem_batch_autotune(strip_mlem_batch* batch, double* best_ms);
 

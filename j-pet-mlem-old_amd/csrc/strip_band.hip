@@ -519,41 +519,24 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   }
 }
 
-// Back-projection.  Each wavefront takes one task and owns G private
-// windows of Wt columns x H rows in LDS (word = (col - c0) * H + row), one
-// per lane group: every lane writes only its own (row, column) words of its
-// group's window, and no other wave sees them, so plain read-modify-write
-// suffices.  A step takes G pairs, lane = g*(H*LR) + q*H + r as in the
-// forward projection.  The windows are then summed and added to the
-// column-major image.
+// Back-projection of one task by one wavefront, G pairs per step: G
+// private windows of Wt columns x H rows in LDS at wave_win (word = (col -
+// c0) * H + row, window k at k * Wt * H), one per lane group: every lane
+// writes only its own (row, column) words of its group's window, and no
+// other wave sees them, so plain read-modify-write suffices.  lane =
+// g*(H*LR) + q*H + r as in the forward projection.  The windows are then
+// summed and added to the column-major image.
 template <int H, int LR, bool REC, bool SPANS>
-__global__ __launch_bounds__(256) void band_backward_kernel(
-    const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
-    int n_tasks, int* __restrict__ queue, int window_words) {
+__device__ __forceinline__ void backward_task(const BandInstance& d,
+                                              const int4 task,
+                                              float* const wave_win,
+                                              const int lane) {
   constexpr int G = 64 / (H * LR);
   static_assert(G * H * LR == 64, "lane layout");
-  extern __shared__ float windows[];
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int n_waves = blockDim.x >> 6;
   const int r = lane & (H - 1);
   const int q = (lane / H) % LR;
   const int grp = lane / (H * LR);
-  float* const wave_win = windows + wave * G * window_words;
-  float* const win = wave_win + grp * window_words;
-  // each wavefront walks the task list: from a queue counter or in grid
-  // strides of the launch's wavefronts
-  for (int t = blockIdx.x * n_waves + wave;;) {
-  if (queue) {
-    int got = 0;
-    if (lane == 0) got = atomicAdd(queue, 1);
-    t = __builtin_amdgcn_readfirstlane(got);
-  }
-  if (t >= n_tasks) break;
-  const int4 task = tasks[t];
-  t += gridDim.x * n_waves;
   const int b = task.x & 0xffff;
-  const BandInstance& d = desc[task.x >> 16];
   const DeviceGeometry g = d.g;
   const BandRecord* __restrict__ rec = d.rec;
   const int* __restrict__ pair_event = d.bwd_pairs;
@@ -562,6 +545,8 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
   const float* __restrict__ inv = d.inv;
   float* __restrict__ bpT = d.bpT;
   const int Wt = task.w >> 16;  // the columns the task's pairs reach
+  const int window_words = Wt * H;
+  float* const win = wave_win + grp * window_words;
   // each lane group clears its own window
   for (int i = lane & (H * LR - 1); i < Wt * H; i += H * LR) win[i] = 0.0f;
   const int c0 = task.w & 0xffff;
@@ -692,6 +677,47 @@ __global__ __launch_bounds__(256) void band_backward_kernel(
     if (val != 0.0f && iy < g.n_y && col < g.n_z)
       gatomic_add(bpT + (int64_t)col * g.n_y + iy, val);
   }
+}
+
+// Back-projection.  Each wavefront takes one task at a time and owns
+// wave_words words of LDS; a task runs with the most pairs per step (up to
+// max_group, halving) whose G windows of Wt x H words fit there, so narrow
+// tasks share the per-pair work among more lane groups than wide ones.
+template <int H, bool REC, bool SPANS>
+__global__ __launch_bounds__(256) void band_backward_kernel(
+    const BandInstance* __restrict__ desc, const int4* __restrict__ tasks,
+    int n_tasks, int* __restrict__ queue, int wave_words, int max_group) {
+  extern __shared__ float windows[];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int n_waves = blockDim.x >> 6;
+  float* const wave_win = windows + wave * wave_words;
+  // each wavefront walks the task list: from a queue counter or in grid
+  // strides of the launch's wavefronts
+  for (int t = blockIdx.x * n_waves + wave;;) {
+    if (queue) {
+      int got = 0;
+      if (lane == 0) got = atomicAdd(queue, 1);
+      t = __builtin_amdgcn_readfirstlane(got);
+    }
+    if (t >= n_tasks) break;
+    const int4 task = tasks[t];
+    t += gridDim.x * n_waves;
+    const BandInstance& d = desc[task.x >> 16];
+    const int window = (task.w >> 16) * H;
+    if constexpr (64 / H >= 4) {
+      if (max_group >= 4 && 4 * window <= wave_words) {
+        backward_task<H, 64 / (H * 4), REC, SPANS>(d, task, wave_win, lane);
+        continue;
+      }
+    }
+    if constexpr (64 / H >= 2) {
+      if (max_group >= 2 && 2 * window <= wave_words) {
+        backward_task<H, 64 / (H * 2), REC, SPANS>(d, task, wave_win, lane);
+        continue;
+      }
+    }
+    backward_task<H, 64 / H, REC, SPANS>(d, task, wave_win, lane);
   }
 }
 
@@ -770,20 +796,19 @@ int launch_forward_t(const BandInstance* desc, const int4* tasks, int n_tasks,
   return 0;
 }
 
-template <int H, int LR>
+template <int H>
 int launch_backward_t(const BandInstance* desc, const int4* tasks,
-                      int n_tasks, bool sp, bool recurrence, int window_words,
-                      int waves, int grid, int* queue, hipStream_t stream,
-                      std::string* err) {
-  auto kern = recurrence ? (sp ? band_backward_kernel<H, LR, true, true>
-                               : band_backward_kernel<H, LR, true, false>)
-                         : (sp ? band_backward_kernel<H, LR, false, true>
-                               : band_backward_kernel<H, LR, false, false>);
-  constexpr int G = 64 / (H * LR);
-  const size_t lds = sizeof(float) * (size_t)window_words * G * waves;
+                      int n_tasks, bool sp, bool recurrence, int wave_words,
+                      int max_group, int waves, int grid, int* queue,
+                      hipStream_t stream, std::string* err) {
+  auto kern = recurrence ? (sp ? band_backward_kernel<H, true, true>
+                               : band_backward_kernel<H, true, false>)
+                         : (sp ? band_backward_kernel<H, false, true>
+                               : band_backward_kernel<H, false, false>);
+  const size_t lds = sizeof(float) * (size_t)wave_words * waves;
   if (set_lds(kern, lds, err)) return -1;
   hipLaunchKernelGGL(kern, dim3(grid), dim3(64 * waves), lds, stream, desc,
-                     tasks, n_tasks, queue, window_words);
+                     tasks, n_tasks, queue, wave_words, max_group);
   BAND_TRY(hipGetLastError());
   return 0;
 }
@@ -811,26 +836,27 @@ int launch_forward_any(int H, int G, const BandInstance* desc,
   return -1;
 }
 
-// (band rows, pairs per step) of the back-projection
-int launch_backward_any(int H, int G, const BandInstance* desc,
+// (band rows) of the back-projection; max_group = most pairs per step
+int launch_backward_any(int H, int max_group, const BandInstance* desc,
                         const int4* tasks, int n_tasks, bool sp,
-                        bool recurrence, int window_words, int waves,
+                        bool recurrence, int wave_words, int waves,
                         int grid, int* queue, hipStream_t stream,
                         std::string* err) {
   if (n_tasks == 0) return 0;
-#define STRIP_BWD(HH, GG)                                                 \
-  if (H == HH && G == GG)                                                 \
-    return launch_backward_t<HH, 64 / (HH * GG)>(                         \
-        desc, tasks, n_tasks, sp, recurrence, window_words, waves, grid,  \
-        queue, stream, err);
-  STRIP_BWD(16, 1)
-  STRIP_BWD(16, 2)
-  STRIP_BWD(16, 4)
-  STRIP_BWD(32, 1)
-  STRIP_BWD(32, 2)
-  STRIP_BWD(64, 1)
+  if (max_group < 1 || max_group * H > 64) {
+    if (err) *err = "unsupported back-projection pairs per step";
+    return -1;
+  }
+#define STRIP_BWD(HH)                                                        \
+  if (H == HH)                                                               \
+    return launch_backward_t<HH>(desc, tasks, n_tasks, sp, recurrence,       \
+                                 wave_words, max_group, waves, grid, queue,  \
+                                 stream, err);
+  STRIP_BWD(16)
+  STRIP_BWD(32)
+  STRIP_BWD(64)
 #undef STRIP_BWD
-  if (err) *err = "unsupported back-projection band rows / pairs per step";
+  if (err) *err = "unsupported back-projection band rows";
   return -1;
 }
 
@@ -876,11 +902,10 @@ int dispatch_backward(BandPlan& plan, const DeviceGeometry& g,
   }
   const bool sp = plan.spans != nullptr;
   const bool rec = plan.tuning.recurrence != 0;
-  const int window_words = L.Wt * L.H;
   const int grid = (L.n_tasks + plan.bwd_waves - 1) / plan.bwd_waves;
   return launch_backward_any(L.H, plan.bwd_group, plan.desc + slot, L.tasks,
-                             L.n_tasks, sp, rec, window_words, plan.bwd_waves,
-                             grid, nullptr, stream, err);
+                             L.n_tasks, sp, rec, plan.bwd_wave_words,
+                             plan.bwd_waves, grid, nullptr, stream, err);
 }
 
 template <typename T>
@@ -900,7 +925,18 @@ void free_list(PairList& L) {
 }
 
 constexpr size_t kLdsLimit = 160 * 1024;
-constexpr size_t kMinBackprojectionWaves = 12;  // per CU, see band_prepare
+
+}  // namespace
+
+// LDS bytes per back-projection wavefront that keep cu_waves wavefronts
+// resident per CU in workgroups of four (each workgroup's share a whole
+// number of KiB).
+size_t backprojection_budget(int cu_waves) {
+  const size_t groups = (size_t)std::max(1, (cu_waves + 3) / 4);
+  return ((kLdsLimit / groups) & ~(size_t)1023) / 4;
+}
+
+namespace {
 
 // Sorted (event, band) pairs of band height H for the resident events, and
 // the index of the first pair of every (band, x0 >> shift) group.
@@ -1247,24 +1283,31 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   if (plan.fwd_group == 3) plan.fwd_group = 2;
   if (plan.bwd_group == 3) plan.bwd_group = 2;
   plan.fwd_lds = (size_t)plan.fwd.Wt * Hf * sizeof(float);
-  // back-projection: G windows per wavefront; with the default groups, fewer
-  // pairs per step when the windows would leave fewer than 12 wavefronts per
-  // CU (the LDS-latency bound then costs more than the shared prologue saves):
-  // 4 pairs for narrow windows, 2 for the 1 mm head workload's 76 columns,
-  // 1 for the 0.5 mm grid's 119
-  size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
-  const size_t min_waves = tu.auto_groups ? kMinBackprojectionWaves : 1;
-  while (plan.bwd_group > 1 &&
-         window_bytes * plan.bwd_group * min_waves > kLdsLimit)
-    plan.bwd_group /= 2;
-  window_bytes *= plan.bwd_group;
-  plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / window_bytes);
+  // back-projection: every wavefront owns a fixed LDS budget and runs each
+  // task with the most pairs per step (up to bwd_group, halving) whose
+  // windows fit in it.  Automatic groups: the budget keeps bwd_cu_waves
+  // wavefronts per CU resident (more pairs per step share the per-pair work
+  // but need more LDS, and below ~12 resident wavefronts the LDS latency
+  // costs more than the shared work saves), or holds one widest window
+  // where that is larger.  Explicit groups: bwd_group windows of the widest
+  // task, so every task takes bwd_group pairs per step.
+  const size_t window_bytes = (size_t)plan.bwd.Wt * Hb * sizeof(float);
+  size_t budget;
+  if (tu.auto_groups) {
+    budget = std::max(backprojection_budget(tu.bwd_cu_waves), window_bytes);
+  } else {
+    while (plan.bwd_group > 1 && window_bytes * plan.bwd_group > kLdsLimit)
+      plan.bwd_group /= 2;
+    budget = window_bytes * plan.bwd_group;
+  }
+  plan.bwd_wave_words = (int)(budget / sizeof(float));
+  plan.bwd_waves = (int)std::min<size_t>(4, kLdsLimit / budget);
   if (tu.bwd_waves > 0) plan.bwd_waves = std::min(plan.bwd_waves, tu.bwd_waves);
   if (plan.bwd_waves < 1) {
     if (err) *err = "back-projection window does not fit in LDS";
     return -1;
   }
-  plan.bwd_lds = window_bytes * plan.bwd_waves;
+  plan.bwd_lds = budget * plan.bwd_waves;
   if (n > 0) {
     BAND_TRY(hipMalloc(&plan.partial,
                        sizeof(float) * n * plan.fwd.max_bands));
@@ -1345,12 +1388,13 @@ int band_launch_batch_update(const BandInstance* desc, int n_instances,
 }
 
 int band_launch_backward(const BandInstance* desc, const int4* tasks,
-                         int n_tasks, int H, int G, bool spans,
-                         bool recurrence, int window_words, int waves,
+                         int n_tasks, int H, int max_group, bool spans,
+                         bool recurrence, int wave_words, int waves,
                          int grid, int* queue, hipStream_t stream,
                          std::string* err) {
-  return launch_backward_any(H, G, desc, tasks, n_tasks, spans, recurrence,
-                             window_words, waves, grid, queue, stream, err);
+  return launch_backward_any(H, max_group, desc, tasks, n_tasks, spans,
+                             recurrence, wave_words, waves, grid, queue,
+                             stream, err);
 }
 
 int band_load_images(BandPlan& plan, const DeviceGeometry& g,

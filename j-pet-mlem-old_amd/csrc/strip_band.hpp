@@ -45,10 +45,13 @@ struct BandTuning {
   // lane groups of 64 / G lanes take one pair each, with 64 / (G * rows)
   // lanes per image row
   int fwd_group = 2;
-  int bwd_group = 4;
-  // true until a caller sets the groups: the back-projection's G then also
-  // drops while its windows would leave fewer than 12 wavefronts per CU
+  int bwd_group = 4;  // the most; a task takes fewer when its windows are wide
+  // true until a caller sets the groups: each back-projection wavefront then
+  // owns the LDS that keeps bwd_cu_waves wavefronts per CU resident, and a
+  // task takes the most pairs per step (up to bwd_group) whose windows fit
+  // there; with set groups every task takes bwd_group
   bool auto_groups = true;
+  int bwd_cu_waves = 16;
   // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
   // two-term recurrence e[k+1] = e[k] r[k], r[k+1] = r[k] G of exp2 of a
   // quadratic, restarted from exp2 every 8 pixels (4 in a tail); the forward
@@ -120,7 +123,8 @@ struct BandPlan {
   int Wt = 0;         // bound on the task windows: bucket + widest bbox
   int bwd_waves = 4;  // wavefronts per back-projection workgroup
   int fwd_group = 1;  // pairs per wavefront step in use
-  int bwd_group = 1;
+  int bwd_group = 1;  // the most pairs per back-projection step
+  int bwd_wave_words = 0;  // LDS words per back-projection wavefront
   int max_width = 0;  // widest clamped bbox (x1 - x0)
   int64_t n_events = 0;
   // events renumbered in (bbox y0, x0) order so neighbouring pairs of a task
@@ -156,21 +160,26 @@ BandInstance band_descriptor(const BandPlan& plan, const DeviceGeometry& g,
 
 // Launches of the band kernels over a task list of one or many instances.
 // grid = workgroups (tasks are taken in grid strides); lds = dynamic LDS
-// bytes; backward: window words per wavefront (largest Wt x H) and
-// wavefronts per workgroup.
+// bytes of the forward projection.
 // queue (may be null): a device counter, zero at launch, from which the
 // resident workgroups (forward) / wavefronts (back-projection) take tasks.
-// H = band rows, G = pairs per wavefront step; the back-projection uses G
-// windows of window_words per wavefront.
+// H = band rows, G = pairs per forward wavefront step.  The back-projection
+// gives every wavefront wave_words words of LDS (at least one widest task
+// window, Wt x H) in workgroups of `waves`; a task takes the most pairs per
+// step, up to max_group and halving, whose windows fit in them.
 int band_launch_forward(const BandInstance* desc, const int4* tasks,
                         int n_tasks, int H, int G, bool spans, bool recurrence,
                         size_t lds, int grid, int* queue, hipStream_t stream,
                         std::string* err);
 int band_launch_backward(const BandInstance* desc, const int4* tasks,
-                         int n_tasks, int H, int G, bool spans,
-                         bool recurrence, int window_words, int waves,
+                         int n_tasks, int H, int max_group, bool spans,
+                         bool recurrence, int wave_words, int waves,
                          int grid, int* queue, hipStream_t stream,
                          std::string* err);
+
+// LDS bytes per back-projection wavefront that keep cu_waves wavefronts
+// resident per CU in workgroups of four.
+size_t backprojection_budget(int cu_waves);
 
 // Zeroes n ints on the stream with a kernel (graph-recordable).
 int band_zero_ints(int* p, int n, hipStream_t stream, std::string* err);

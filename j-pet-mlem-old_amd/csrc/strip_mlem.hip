@@ -1435,6 +1435,19 @@ int strip_mlem_set_backprojection_waves(strip_mlem_ctx* ctx, int waves) {
   return STRIP_MLEM_OK;
 }
 
+int strip_mlem_set_backprojection_occupancy(strip_mlem_ctx* ctx,
+                                            int cu_waves) {
+  if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (cu_waves < 4 || cu_waves > 32)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "cu_waves must be 4..32");
+  if (ctx->band.tuning.bwd_cu_waves != cu_waves) {
+    ctx->band.tuning.bwd_cu_waves = cu_waves;
+    ctx->band_dirty = true;
+    ctx->subsets_dirty = true;
+  }
+  return STRIP_MLEM_OK;
+}
+
 int strip_mlem_set_exp_mode(strip_mlem_ctx* ctx, int mode) {
   if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
   if (mode != 0 && mode != 1)
@@ -1713,7 +1726,8 @@ struct strip_mlem_batch {
   int n_fwd = 0, n_bwd = 0;
   int H_fwd = 0, H_bwd = 0;
   int G_fwd = 1, G_bwd = 1;  // pairs per wavefront step
-  int fwd_Wt = 0, bwd_Wt = 0, bwd_waves = 1;  // widest member windows
+  int fwd_Wt = 0, bwd_waves = 1;  // widest member forward window
+  int bwd_wave_words = 0;  // LDS words per back-projection wavefront
   size_t fwd_lds = 0;
   bool spans = true;
   int64_t max_events = 0, max_pixels = 0;
@@ -1768,7 +1782,8 @@ int batch_merge(strip_mlem_batch* b) {
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "at most 32767 members");
   std::vector<strip::BandInstance> d0(n), d1(n);
   std::vector<int4> fwd, bwd;
-  b->fwd_Wt = b->bwd_Wt = 1;
+  b->fwd_Wt = 1;
+  b->bwd_wave_words = 0;
   b->spans = true;
   b->max_events = b->max_pixels = 0;
   for (int i = 0; i < n; ++i) {
@@ -1781,6 +1796,7 @@ int batch_merge(strip_mlem_batch* b) {
         c->band.tuning.fwd_group != b->tuning.fwd_group ||
         c->band.tuning.bwd_group != b->tuning.bwd_group ||
         c->band.tuning.auto_groups != b->tuning.auto_groups ||
+        c->band.tuning.bwd_cu_waves != b->tuning.bwd_cu_waves ||
         c->band.tuning.spans != b->tuning.spans) {
       c->band.tuning = b->tuning;
       c->band_dirty = true;
@@ -1789,7 +1805,8 @@ int batch_merge(strip_mlem_batch* b) {
     if (rc) return rc;
     const strip::BandPlan& P = c->band;
     // the pairs per wavefront step only shape the launches: the batch takes
-    // the smallest of its members' (their windows differ in width)
+    // the smallest of its members' forward G and back-projection limit, and
+    // the largest back-projection LDS budget (the widest member's windows)
     if (i == 0) {
       b->H_fwd = P.fwd.H;
       b->H_bwd = P.bwd.H;
@@ -1802,7 +1819,7 @@ int batch_merge(strip_mlem_batch* b) {
     b->G_fwd = std::min(b->G_fwd, P.fwd_group);
     b->G_bwd = std::min(b->G_bwd, P.bwd_group);
     b->fwd_Wt = std::max(b->fwd_Wt, P.fwd.Wt);
-    b->bwd_Wt = std::max(b->bwd_Wt, P.bwd.Wt);
+    b->bwd_wave_words = std::max(b->bwd_wave_words, P.bwd_wave_words);
     b->spans = b->spans && P.spans != nullptr;
     b->max_events = std::max<int64_t>(b->max_events, P.n_events);
     b->max_pixels = std::max<int64_t>(b->max_pixels, c->n_pixels);
@@ -1819,8 +1836,7 @@ int batch_merge(strip_mlem_batch* b) {
     b->generations.push_back(P.generation);
   }
   b->fwd_lds = sizeof(float) * (size_t)b->fwd_Wt * b->H_fwd;
-  const size_t window =
-      sizeof(float) * (size_t)b->bwd_Wt * b->H_bwd * b->G_bwd;
+  const size_t window = sizeof(float) * (size_t)b->bwd_wave_words;
   if (b->fwd_lds > 160 * 1024 || window > 160 * 1024)
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
                        "a member's task window does not fit in LDS");
@@ -1901,7 +1917,7 @@ int batch_launch_iteration(strip_mlem_batch* b, int c) {
                                       b->stream, &err) ||
       strip::band_launch_backward(
           b->desc[c], b->bwd_tasks, b->n_bwd, b->H_bwd, b->G_bwd, b->spans,
-          rec, b->bwd_Wt * b->H_bwd, b->bwd_waves,
+          rec, b->bwd_wave_words, b->bwd_waves,
           std::min((b->n_bwd + b->bwd_waves - 1) / b->bwd_waves, persistent),
           b->queue + 1, b->stream, &err) ||
       strip::band_launch_batch_update(b->desc[c], n, b->max_events,
@@ -2116,6 +2132,26 @@ int strip_mlem_batch_get_groups(const strip_mlem_batch* b, int* forward_group,
   return STRIP_MLEM_OK;
 }
 
+int strip_mlem_batch_set_backprojection_occupancy(strip_mlem_batch* b,
+                                                  int cu_waves) {
+  BATCH_CHECK(b);
+  if (cu_waves < 4 || cu_waves > 32)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "cu_waves must be 4..32");
+  int rc = batch_store_images(b);
+  if (rc) return rc;
+  if (b->tuning.bwd_cu_waves != cu_waves)
+    b->generations.clear();  // the members' plans are rebuilt and re-merged
+  b->tuning.bwd_cu_waves = cu_waves;
+  return STRIP_MLEM_OK;
+}
+
+int strip_mlem_batch_get_backprojection_occupancy(const strip_mlem_batch* b,
+                                                  int* cu_waves) {
+  if (!b || !cu_waves) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL");
+  *cu_waves = b->tuning.bwd_cu_waves;
+  return STRIP_MLEM_OK;
+}
+
 int strip_mlem_batch_get_tiling(const strip_mlem_batch* b, int* tiling) {
   if (!b || !tiling) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL");
   tiling[0] = b->tuning.fwd_rows;
@@ -2171,55 +2207,75 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
     saved[i].resize(b->items[i]->n_pixels);
     if ((rc = strip_mlem_get_rho(b->items[i], saved[i].data()))) return rc;
   }
-  // coordinates: the six tiling knobs, then one index into the pairs-per-step
-  // choices ((0, 0) = automatic)
+  // coordinates: the six tiling knobs, one index into the pairs-per-step
+  // choices ((0, 0) = automatic), the back-projection wavefronts per CU that
+  // automatic groups keep resident
   static const int kGroups[][2] = {{0, 0}, {1, 1}, {2, 1}, {2, 2},
                                    {2, 4}, {4, 2}, {4, 4}};
   constexpr int kNGroups = sizeof(kGroups) / sizeof(kGroups[0]);
-  int cur[7];
+  constexpr int kKnobs = 8;
+  int cur[kKnobs];
   strip_mlem_batch_get_tiling(b, cur);
   cur[6] = 0;
+  cur[7] = b->tuning.bwd_cu_waves;
   if (!b->tuning.auto_groups)
     for (int g = 1; g < kNGroups; ++g)
       if (kGroups[g][0] == b->tuning.fwd_group &&
           kGroups[g][1] == b->tuning.bwd_group)
         cur[6] = g;
+  // a candidate's time is the faster of two iterations after the one that
+  // builds its plans and sets its LDS limits
   auto time_it = [&](const int* t, double* ms) -> int {
     int r = strip_mlem_batch_set_tiling(b, t[0], t[1], t[2], t[3], t[4], t[5]);
     if (!r) r = strip_mlem_batch_set_groups(b, kGroups[t[6]][0], kGroups[t[6]][1]);
+    if (!r) r = strip_mlem_batch_set_backprojection_occupancy(b, t[7]);
     if (r) return r;
     const bool graph = b->use_graph;
     b->use_graph = false;
-    r = batch_iterate(b, 1);  // plans + LDS limits
-    if (!r) r = batch_iterate(b, 1);
+    r = batch_iterate(b, 1);
+    double fastest = 0.0;
+    for (int rep = 0; rep < 2 && !r; ++rep) {
+      r = batch_iterate(b, 1);
+      if (!r && (rep == 0 || b->last_ms < fastest)) fastest = b->last_ms;
+    }
     b->use_graph = graph;
-    *ms = b->last_ms;
+    *ms = fastest;
     return r;
   };
   double best = 0.0;
   if ((rc = time_it(cur, &best))) return rc;
   const std::vector<std::vector<int>> choices = {
       {64, 32, 16}, {16, 32, 64}, {8, 16, 32}, {1024, 2048, 4096},
-      {256, 512, 1024}, {2, 4, 8}, {0, 1, 2, 3, 4, 5, 6}};
-  for (int k = 0; k < 7; ++k) {
-    for (int v : choices[k]) {
-      if (v == cur[k]) continue;
-      int cand[7];
-      std::copy(cur, cur + 7, cand);
-      cand[k] = v;
-      double ms = 0.0;
-      if (time_it(cand, &ms)) continue;  // a shape that does not fit
-      if (ms < best) {
-        best = ms;
-        std::copy(cand, cand + 7, cur);
+      {256, 512, 1024}, {2, 4, 8}, {0, 1, 2, 3, 4, 5, 6}, {8, 12, 16, 20}};
+  // the knobs interact (the bucket width sets the windows that bound the
+  // pairs per step), so sweeps repeat until one changes nothing
+  for (int pass = 0; pass < 3; ++pass) {
+    bool changed = false;
+    for (int k = 0; k < kKnobs; ++k) {
+      // the occupancy only matters with automatic groups
+      if (k == 7 && cur[6] != 0) continue;
+      for (int v : choices[k]) {
+        if (v == cur[k]) continue;
+        int cand[kKnobs];
+        std::copy(cur, cur + kKnobs, cand);
+        cand[k] = v;
+        double ms = 0.0;
+        if (time_it(cand, &ms)) continue;  // a shape that does not fit
+        if (ms < best) {
+          best = ms;
+          std::copy(cand, cand + kKnobs, cur);
+          changed = true;
+        }
       }
     }
+    if (!changed) break;
   }
   // the chosen tiling, its plans built and merged before returning
   if ((rc = strip_mlem_batch_set_tiling(b, cur[0], cur[1], cur[2], cur[3],
                                         cur[4], cur[5])) ||
       (rc = strip_mlem_batch_set_groups(b, kGroups[cur[6]][0],
                                         kGroups[cur[6]][1])) ||
+      (rc = strip_mlem_batch_set_backprojection_occupancy(b, cur[7])) ||
       (rc = batch_merge(b)))
     return rc;
   for (size_t i = 0; i < b->items.size(); ++i)

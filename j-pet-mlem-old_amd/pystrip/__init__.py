@@ -159,6 +159,12 @@ def load_library():
             ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_void_p]),
         "strip_phantom_last_error": (ctypes.c_char_p, []),
         "strip_mlem_set_backprojection_waves": (ctypes.c_int, [P, ctypes.c_int]),
+        "strip_mlem_set_backprojection_occupancy": (ctypes.c_int,
+                                                    [P, ctypes.c_int]),
+        "strip_mlem_batch_set_backprojection_occupancy": (ctypes.c_int,
+                                                          [P, ctypes.c_int]),
+        "strip_mlem_batch_get_backprojection_occupancy": (
+            ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int)]),
         "strip_mlem_set_band_groups": (ctypes.c_int, [P, ctypes.c_int, ctypes.c_int]),
         "strip_mlem_get_band_groups": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_int),
                                                       ctypes.POINTER(ctypes.c_int)]),
@@ -239,6 +245,9 @@ def exported_symbols():
         "strip_mlem_set_band_tiling", "strip_mlem_get_band_tiling",
         "strip_mlem_set_exp_mode", "strip_mlem_set_span_table",
         "strip_mlem_set_backprojection_waves",
+        "strip_mlem_set_backprojection_occupancy",
+        "strip_mlem_batch_set_backprojection_occupancy",
+        "strip_mlem_batch_get_backprojection_occupancy",
         "strip_mlem_set_band_groups", "strip_mlem_get_band_groups",
         "strip_mlem_batch_set_groups", "strip_mlem_batch_get_groups",
         "strip_mlem_get_span_table", "strip_mlem_loglik_gradient",
@@ -392,6 +401,12 @@ class Reconstruction:
         _check(self._lib.strip_mlem_get_band_groups(
             self._ctx, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+    def set_backprojection_occupancy(self, cu_waves: int):
+        """Back-projection wavefronts per CU (4..32) whose LDS windows the
+        automatic groups keep resident."""
+        _check(self._lib.strip_mlem_set_backprojection_occupancy(
+            self._ctx, int(cu_waves)))
 
     def set_backprojection_waves(self, waves: int):
         """Wavefronts per back-projection workgroup (0 = automatic)."""
@@ -678,6 +693,16 @@ class Batch:
         _check(self._lib.strip_mlem_batch_get_groups(
             self._b, ctypes.byref(f), ctypes.byref(b)))
         return f.value, b.value
+
+    def set_backprojection_occupancy(self, cu_waves: int):
+        _check(self._lib.strip_mlem_batch_set_backprojection_occupancy(
+            self._b, int(cu_waves)))
+
+    def backprojection_occupancy(self) -> int:
+        v = ctypes.c_int()
+        _check(self._lib.strip_mlem_batch_get_backprojection_occupancy(
+            self._b, ctypes.byref(v)))
+        return v.value
 
     def set_graph(self, enable: bool):
         _check(self._lib.strip_mlem_batch_set_graph(self._b, int(bool(enable))))
