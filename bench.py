@@ -37,8 +37,8 @@ FLOPS_PER_KERNEL_EVAL = 16
 # by the counter's reading for a plain 256 MiB copy kernel (x2.0 and x1.0);
 # recorded in profiles/band_kernels_strip-1000x400-1mm-2e7_counters.md
 MEASURED_TRAFFIC_BYTES = {
-    "back-projection": (39.91 + 1.00) * 1e9,
-    "forward projection": (20.37 + 2.65) * 1e9,
+    "back-projection": (41.50 + 0.90) * 1e9,
+    "forward projection": (20.50 + 2.66) * 1e9,
 }
 TRAFFIC_WORKLOAD = "strip-1000x400-1mm-2e7"
 
@@ -198,6 +198,7 @@ def batched_timing(pystrip, phantom, n_members, iterations):
         "iterations": iterations,
         "tiling": list(b.tiling()),
         "groups": list(b.groups()),
+        "backprojection_cu_waves": b.backprojection_occupancy(),
         "autotuned_iteration_ms": tuned_ms,
         "batched_ms_per_iteration": batch_s * 1e3 / iterations,
         "plain_ms_per_iteration": plain_s * 1e3 / iterations,

@@ -177,9 +177,9 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
 /* (Event, band) pairs one wavefront of the forward / back-projection kernel
  * processes together: 1, 2 or 4 (at most 64 / band rows).  Lane groups of
  * 64 / G lanes then take one pair each, so one instruction stream serves G
- * pairs.  The back-projection keeps one LDS window per group: with set
- * groups every task takes G (halved until one wavefront's windows fit);
- * (0, 0) = automatic, the default: forward 2, and every back-projection
+ * pairs.  The back-projection keeps one LDS window per group: with a set
+ * backward_group every task takes it (halved until one wavefront's windows
+ * fit).  0 = automatic, the default: forward 2; back-projection: every
  * wavefront owns the LDS that keeps strip_mlem_set_backprojection_occupancy
  * wavefronts per CU resident, each task taking the most pairs per step, up
  * to 4 and halving, whose windows fit there (wide windows fewer, narrow
@@ -301,8 +301,8 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* batch, int forward_rows,
                                 int grid_factor);
 int strip_mlem_batch_get_tiling(const strip_mlem_batch* batch, int* tiling);
 /* Shared pairs per wavefront step of the members' plans (as
- * strip_mlem_set_band_groups; with (0, 0) each member's automatic choice,
- * the batch launching with the smallest). */
+ * strip_mlem_set_band_groups, 0 = automatic; the batch launches with the
+ * smallest of its members' forward G and back-projection limit). */
 int strip_mlem_batch_set_groups(strip_mlem_batch* batch, int forward_group,
                                 int backward_group);
 int strip_mlem_batch_get_groups(const strip_mlem_batch* batch,

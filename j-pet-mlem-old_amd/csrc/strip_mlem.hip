@@ -1389,16 +1389,14 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
 int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
                                int backward_group) {
   if (!ctx) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
-  const bool automatic = forward_group == 0 && backward_group == 0;
-  auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
-  if (!automatic && (!ok(forward_group) || !ok(backward_group)))
+  auto ok = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
+  if (!ok(forward_group) || !ok(backward_group))
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "groups must be 1, 2 or 4 (or both 0: automatic)");
+                       "groups must be 1, 2 or 4 (0: automatic)");
+  const bool automatic = backward_group == 0;
   const strip::BandTuning defaults;
-  if (automatic) {
-    forward_group = defaults.fwd_group;
-    backward_group = defaults.bwd_group;
-  }
+  if (forward_group == 0) forward_group = defaults.fwd_group;
+  if (automatic) backward_group = defaults.bwd_group;
   if (ctx->band.tuning.fwd_group != forward_group ||
       ctx->band.tuning.bwd_group != backward_group ||
       ctx->band.tuning.auto_groups != automatic) {
@@ -2102,16 +2100,14 @@ int strip_mlem_batch_set_tiling(strip_mlem_batch* b, int forward_rows,
 int strip_mlem_batch_set_groups(strip_mlem_batch* b, int forward_group,
                                 int backward_group) {
   BATCH_CHECK(b);
-  const bool automatic = forward_group == 0 && backward_group == 0;
-  auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
-  if (!automatic && (!ok(forward_group) || !ok(backward_group)))
+  auto ok = [](int v) { return v == 0 || v == 1 || v == 2 || v == 4; };
+  if (!ok(forward_group) || !ok(backward_group))
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
-                       "groups must be 1, 2 or 4 (or both 0: automatic)");
+                       "groups must be 1, 2 or 4 (0: automatic)");
+  const bool automatic = backward_group == 0;
   const strip::BandTuning defaults;
-  if (automatic) {
-    forward_group = defaults.fwd_group;
-    backward_group = defaults.bwd_group;
-  }
+  if (forward_group == 0) forward_group = defaults.fwd_group;
+  if (automatic) backward_group = defaults.bwd_group;
   int rc = batch_store_images(b);
   if (rc) return rc;
   if (b->tuning.fwd_group != forward_group ||
@@ -2208,21 +2204,21 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
     if ((rc = strip_mlem_get_rho(b->items[i], saved[i].data()))) return rc;
   }
   // coordinates: the six tiling knobs, one index into the pairs-per-step
-  // choices ((0, 0) = automatic), the back-projection wavefronts per CU that
+  // choices (0 = automatic), the back-projection wavefronts per CU that
   // automatic groups keep resident
-  static const int kGroups[][2] = {{0, 0}, {1, 1}, {2, 1}, {2, 2},
-                                   {2, 4}, {4, 2}, {4, 4}};
+  static const int kGroups[][2] = {{0, 0}, {1, 0}, {4, 0}, {1, 1}, {2, 1},
+                                   {2, 2}, {2, 4}, {4, 2}, {4, 4}};
   constexpr int kNGroups = sizeof(kGroups) / sizeof(kGroups[0]);
   constexpr int kKnobs = 8;
   int cur[kKnobs];
   strip_mlem_batch_get_tiling(b, cur);
   cur[6] = 0;
   cur[7] = b->tuning.bwd_cu_waves;
-  if (!b->tuning.auto_groups)
-    for (int g = 1; g < kNGroups; ++g)
-      if (kGroups[g][0] == b->tuning.fwd_group &&
-          kGroups[g][1] == b->tuning.bwd_group)
-        cur[6] = g;
+  for (int g = 1; g < kNGroups; ++g) {
+    const int bwd = b->tuning.auto_groups ? 0 : b->tuning.bwd_group;
+    if (kGroups[g][0] == b->tuning.fwd_group && kGroups[g][1] == bwd)
+      cur[6] = g;
+  }
   // a candidate's time is the faster of two iterations after the one that
   // builds its plans and sets its LDS limits
   auto time_it = [&](const int* t, double* ms) -> int {
@@ -2246,14 +2242,15 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
   if ((rc = time_it(cur, &best))) return rc;
   const std::vector<std::vector<int>> choices = {
       {64, 32, 16}, {16, 32, 64}, {8, 16, 32}, {1024, 2048, 4096},
-      {256, 512, 1024}, {2, 4, 8}, {0, 1, 2, 3, 4, 5, 6}, {8, 12, 16, 20}};
+      {256, 512, 1024}, {2, 4, 8}, {0, 1, 2, 3, 4, 5, 6, 7, 8},
+      {8, 12, 16, 20}};
   // the knobs interact (the bucket width sets the windows that bound the
   // pairs per step), so sweeps repeat until one changes nothing
   for (int pass = 0; pass < 3; ++pass) {
     bool changed = false;
     for (int k = 0; k < kKnobs; ++k) {
-      // the occupancy only matters with automatic groups
-      if (k == 7 && cur[6] != 0) continue;
+      // the occupancy only matters with automatic back-projection groups
+      if (k == 7 && kGroups[cur[6]][1] != 0) continue;
       for (int v : choices[k]) {
         if (v == cur[k]) continue;
         int cand[kKnobs];

@@ -391,7 +391,7 @@ class Reconstruction:
 
     def set_band_groups(self, forward_group: int = 0, backward_group: int = 0):
         """(Event, band) pairs one wavefront processes together in the
-        forward / back-projection kernels (1, 2 or 4; both 0 = automatic)."""
+        forward / back-projection kernels (1, 2 or 4; 0 = automatic)."""
         _check(self._lib.strip_mlem_set_band_groups(
             self._ctx, int(forward_group), int(backward_group)))
 

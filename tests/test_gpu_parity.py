@@ -300,6 +300,27 @@ def test_band_groups(pystrip_mod, phantom_mod, oracle_mod, rows, groups,
     assert e <= 1e-4
 
 
+@pytest.mark.parametrize("cu_waves,bucket", [(8, 16), (12, 8), (32, 8),
+                                             (32, 64)])
+def test_backprojection_occupancy(pystrip_mod, phantom_mod, oracle_mod,
+                                  cu_waves, bucket):
+    # automatic back-projection groups: each task takes 4, 2 or 1 pairs per
+    # step as its window fits the per-wavefront LDS budget of the occupancy
+    # (8 wavefronts per CU: mostly 4; 32: a mix down to 1), so one
+    # launch runs tasks of every lane layout
+    print_case("back-projection occupancy", cu_waves, "bucket", bucket)
+    w, ev, rec, og = _setup(pystrip_mod, phantom_mod, oracle_mod, CFG2,
+                            60_000, algorithm=2)
+    rec.set_band_tiling(32, 16, bucket, 2048, 512)
+    rec.set_band_groups(0, 0)
+    rec.set_backprojection_occupancy(cu_waves)
+    assert rec.band_groups() == (2, 4)
+    rec.iterate(3)
+    e = oracle_mod.rel_l2(rec.rho, oracle_mod.mlem(og, ev, iterations=3))
+    print("rel-L2", e)
+    assert e <= 1e-4
+
+
 def test_full_size_against_reference_image(pystrip_mod, phantom_mod,
                                            oracle_mod):
     # the headline workload at full size (2e7 seeded events, 20 iterations)
