@@ -813,6 +813,23 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
 }
 
 // Plans of the ordered subsets: contiguous event ranges, shared images.
+// The plans the iterations run: the subset (or split) plans when there are
+// several (they share one tuning), else the whole-list plan.
+int active_plans(strip_mlem_ctx* ctx,
+                 std::vector<const strip::BandPlan*>& plans) {
+  plans.clear();
+  if (plan_parts(ctx) > 1) {
+    int rc = ensure_subsets(ctx);
+    if (rc) return rc;
+    for (const strip::BandPlan& p : ctx->subsets) plans.push_back(&p);
+  } else {
+    int rc = ensure_band(ctx);
+    if (rc) return rc;
+    plans.push_back(&ctx->band);
+  }
+  return STRIP_MLEM_OK;
+}
+
 int ensure_subsets(strip_mlem_ctx* ctx) {
   const int S = plan_parts(ctx);
   if (!ctx->subsets_dirty && (int)ctx->subsets.size() == S) {
@@ -1414,10 +1431,11 @@ int strip_mlem_get_band_groups(strip_mlem_ctx* ctx, int* forward_group,
   CTX_CHECK(ctx);
   if (effective_algorithm(ctx) != 2)
     return strip::fail(STRIP_MLEM_ERR_STATE, "row-band algorithm not in use");
-  int rc = ensure_band(ctx);
+  std::vector<const strip::BandPlan*> plans;
+  int rc = active_plans(ctx, plans);
   if (rc) return rc;
-  if (forward_group) *forward_group = ctx->band.fwd_group;
-  if (backward_group) *backward_group = ctx->band.bwd_group;
+  if (forward_group) *forward_group = plans[0]->fwd_group;
+  if (backward_group) *backward_group = plans[0]->bwd_group;
   return STRIP_MLEM_OK;
 }
 
@@ -1472,9 +1490,12 @@ int strip_mlem_get_span_table(strip_mlem_ctx* ctx, int* in_use) {
   if (!in_use) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL output");
   *in_use = 0;
   if (effective_algorithm(ctx) != 2) return STRIP_MLEM_OK;
-  int rc = ensure_band(ctx);
+  std::vector<const strip::BandPlan*> plans;
+  int rc = active_plans(ctx, plans);
   if (rc) return rc;
-  *in_use = ctx->band.spans ? 1 : 0;
+  *in_use = 1;
+  for (const strip::BandPlan* p : plans)
+    if (p->n_events > 0 && !p->spans) *in_use = 0;
   return STRIP_MLEM_OK;
 }
 
@@ -1485,13 +1506,21 @@ int strip_mlem_get_band_tiling(strip_mlem_ctx* ctx, int* forward_rows,
   CTX_CHECK(ctx);
   if (effective_algorithm(ctx) != 2)
     return strip::fail(STRIP_MLEM_ERR_STATE, "row-band algorithm not in use");
-  int rc = ensure_band(ctx);
+  std::vector<const strip::BandPlan*> plans;
+  int rc = active_plans(ctx, plans);
   if (rc) return rc;
-  if (forward_rows) *forward_rows = ctx->band.fwd.H;
-  if (backward_rows) *backward_rows = ctx->band.bwd.H;
-  if (window_columns) *window_columns = ctx->band.bwd.Wt;
-  if (forward_pairs) *forward_pairs = ctx->band.fwd.n_pairs;
-  if (backward_pairs) *backward_pairs = ctx->band.bwd.n_pairs;
+  if (forward_rows) *forward_rows = plans[0]->fwd.H;
+  if (backward_rows) *backward_rows = plans[0]->bwd.H;
+  int wt = 0;
+  int64_t nf = 0, nb = 0;
+  for (const strip::BandPlan* p : plans) {
+    wt = std::max(wt, p->bwd.Wt);
+    nf += p->fwd.n_pairs;
+    nb += p->bwd.n_pairs;
+  }
+  if (window_columns) *window_columns = wt;
+  if (forward_pairs) *forward_pairs = nf;
+  if (backward_pairs) *backward_pairs = nb;
   return STRIP_MLEM_OK;
 }
 

@@ -111,6 +111,12 @@ def test_split_plans_equal_single_plan(pystrip_mod, phantom_mod, oracle_mod,
     b.set_plan_events(plan_events)
     b.set_events(ev)
     assert a.kernel_evaluations() == b.kernel_evaluations()
+    # the plan queries read the plans the iterations run (several here)
+    assert a.band_groups() == b.band_groups()
+    assert a.span_table() == b.span_table()
+    ta, tb = a.band_tiling(), b.band_tiling()
+    print("tiling one plan", ta, "split", tb)
+    assert ta[:2] == tb[:2] and ta[3:] == tb[3:]
     rng = np.random.default_rng(7)
     rho = rng.uniform(0.5, 2.0, size=a.shape).astype(np.float32)
     a.rho = rho
