@@ -181,9 +181,10 @@ int strip_mlem_set_band_tiling(strip_mlem_ctx* ctx, int forward_rows,
  * backward_group every task takes it (halved until one wavefront's windows
  * fit).  0 = automatic, the default: forward 2; back-projection: every
  * wavefront owns the LDS that keeps strip_mlem_set_backprojection_occupancy
- * wavefronts per CU resident, each task taking the most pairs per step, up
- * to 4 and halving, whose windows fit there (wide windows fewer, narrow
- * ones more).  strip_mlem_get_band_groups returns the forward G and the
+ * wavefronts per CU resident (or one widest task window, where that share
+ * would leave the widest tasks one pair per step), each task taking the
+ * most pairs per step, up to 4 and halving, whose windows fit there (wide
+ * windows fewer, narrow ones more).  strip_mlem_get_band_groups returns the forward G and the
  * back-projection's most. */
 int strip_mlem_set_band_groups(strip_mlem_ctx* ctx, int forward_group,
                                int backward_group);

@@ -47,9 +47,10 @@ struct BandTuning {
   int fwd_group = 2;
   int bwd_group = 4;  // the most; a task takes fewer when its windows are wide
   // true until a caller sets the groups: each back-projection wavefront then
-  // owns the LDS that keeps bwd_cu_waves wavefronts per CU resident, and a
-  // task takes the most pairs per step (up to bwd_group) whose windows fit
-  // there; with set groups every task takes bwd_group
+  // owns the LDS that keeps bwd_cu_waves wavefronts per CU resident (or one
+  // widest window, where that share would leave the widest tasks one pair),
+  // and a task takes the most pairs per step (up to bwd_group) whose windows
+  // fit there; with set groups every task takes bwd_group
   bool auto_groups = true;
   int bwd_cu_waves = 16;
   // kernel values along a row: 0 = one exp2 per pixel; 1 = the exact
