@@ -812,7 +812,6 @@ int unrolled_gradient_device(strip_mlem_ctx* ctx, const float* d_rho0,
   return STRIP_MLEM_OK;
 }
 
-// Plans of the ordered subsets: contiguous event ranges, shared images.
 // The plans the iterations run: the subset (or split) plans when there are
 // several (they share one tuning), else the whole-list plan.
 int active_plans(strip_mlem_ctx* ctx,
@@ -830,6 +829,7 @@ int active_plans(strip_mlem_ctx* ctx,
   return STRIP_MLEM_OK;
 }
 
+// Plans of the ordered subsets: contiguous event ranges, shared images.
 int ensure_subsets(strip_mlem_ctx* ctx) {
   const int S = plan_parts(ctx);
   if (!ctx->subsets_dirty && (int)ctx->subsets.size() == S) {
@@ -1493,9 +1493,13 @@ int strip_mlem_get_span_table(strip_mlem_ctx* ctx, int* in_use) {
   std::vector<const strip::BandPlan*> plans;
   int rc = active_plans(ctx, plans);
   if (rc) return rc;
-  *in_use = 1;
+  bool any = false, all = true;
   for (const strip::BandPlan* p : plans)
-    if (p->n_events > 0 && !p->spans) *in_use = 0;
+    if (p->n_events > 0) {
+      any = true;
+      all = all && p->spans != nullptr;
+    }
+  *in_use = any && all ? 1 : 0;
   return STRIP_MLEM_OK;
 }
 

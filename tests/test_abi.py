@@ -75,3 +75,8 @@ def test_argument_checks_without_gpu(pystrip_mod):
                                  ctypes.byref(ctx)) != 0
     assert lib.strip_mlem_set_launch(None, 0, 0) != 0
     assert lib.strip_mlem_n_events(None) == -1
+    # tiling knobs on a NULL context or batch are refused, not dereferenced
+    assert lib.strip_mlem_set_band_groups(None, 0, 0) != 0
+    assert lib.strip_mlem_set_backprojection_occupancy(None, 16) != 0
+    assert lib.strip_mlem_batch_set_backprojection_occupancy(None, 16) != 0
+    assert lib.strip_mlem_batch_get_backprojection_occupancy(None, None) != 0
