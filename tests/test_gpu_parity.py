@@ -300,13 +300,14 @@ def test_band_groups(pystrip_mod, phantom_mod, oracle_mod, rows, groups,
     assert e <= 1e-4
 
 
-@pytest.mark.parametrize("cu_waves,bucket", [(8, 16), (12, 8), (32, 8),
-                                             (32, 64)])
+@pytest.mark.parametrize("cu_waves,bucket", [(4, 8), (8, 16), (12, 8),
+                                             (32, 8), (32, 64)])
 def test_backprojection_occupancy(pystrip_mod, phantom_mod, oracle_mod,
                                   cu_waves, bucket):
     # automatic back-projection groups: each task takes 4, 2 or 1 pairs per
     # step as its window fits the per-wavefront LDS budget of the occupancy
-    # (8 wavefronts per CU: mostly 4; 32: a mix down to 1), so one
+    # (4 wavefronts per CU: one 160 KiB workgroup, 4 everywhere; 8: mostly
+    # 4; 32: a mix down to 1), so one
     # launch runs tasks of every lane layout
     print_case("back-projection occupancy", cu_waves, "bucket", bucket)
     w, ev, rec, og = _setup(pystrip_mod, phantom_mod, oracle_mod, CFG2,
