@@ -43,26 +43,104 @@ MEASURED_TRAFFIC_BYTES = {
 TRAFFIC_WORKLOAD = "strip-1000x400-1mm-2e7"
 
 
-def cpu_baseline(workload, n_sample, threads):
+def usable_cores():
+    """CPUs this process may run on: its affinity set, reduced to the
+    cgroup's CPU quota where one is set (cgroup v2 cpu.max)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = max(1, min(n, int(float(quota) / float(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def cpu_baseline(workload, n_sample):
     """Times the reference CPU reconstruction (oracle/_ref, the reference's own
-    Reconstruction<float, GaussianKernel<float>>) -- or the oracle port where
-    the reference build is absent -- on a bounded sample of the workload."""
+    Reconstruction<float, GaussianKernel<float>>, in the widest ISA build the
+    host CPU runs) -- or the oracle port where the reference build is absent
+    -- on a bounded sample of the workload, one iteration, on every usable
+    core and on 16 threads."""
     import oracle
     events = workload.events(n_sample, seed=workload.seed + 1)
     g = oracle.geometry(*workload.oracle_geometry_tuple())
     kind = "reference" if oracle.ref_available() else "port"
-    fn = oracle.ref_mlem if kind == "reference" else oracle.mlem
-    fn(g, events[: min(2000, n_sample)], iterations=1, n_threads=threads)
-    t0 = time.perf_counter()
-    fn(g, events, iterations=1, n_threads=threads)
-    dt = time.perf_counter() - t0
-    return {
+    isa = oracle.ref_native_isa() if kind == "reference" else "x86-64-v3"
+    model, _ = oracle.host_cpu()
+
+    def timed(threads):
+        if kind == "reference":
+            def fn(ev):
+                oracle.ref_mlem(g, ev, iterations=1, n_threads=threads, isa=isa)
+        else:
+            def fn(ev):
+                oracle.mlem(g, ev, iterations=1, n_threads=threads)
+        fn(events[: min(2000, n_sample)])
+        t0 = time.perf_counter()
+        fn(events)
+        return time.perf_counter() - t0
+
+    cores = usable_cores()
+    dt = timed(cores)
+    out = {
         "value": n_sample / dt,
         "unit": "event-updates/s",
-        "cores": threads,
+        "cores": cores,
         "kind": kind,
         "sample": f"{n_sample} events of the same workload (seed "
                   f"{workload.seed + 1}), 1 iteration, {dt:.2f} s wall",
+        "cpu_model": model,
+        "host_logical_cpus": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
+        "isa": isa,
+    }
+    if cores != 16:
+        dt16 = timed(16)
+        out["value_16_threads"] = n_sample / dt16
+    else:
+        out["value_16_threads"] = out["value"]
+    return out
+
+
+def reference_parity(rec, workload, events):
+    """Error of the GPU image after the fixture's iteration count from
+    rho = 1 on the fixture's events against the reference's own FP32 and
+    FP64 CPU images of the same events (tests/golden/
+    strip_1000x400_2e7_20it_ref.npz, made by make_full_image.py); None where
+    the bench workload is not the fixture's."""
+    path = os.path.join(ROOT, "tests", "golden",
+                        "strip_1000x400_2e7_20it_ref.npz")
+    if not os.path.exists(path):
+        return None
+    from pystrip import phantom
+    gold = np.load(path)
+    if (str(gold["workload"]) != workload.name or
+            int(gold["n_events"]) != len(events) or
+            int(gold["seed"]) != workload.seed):
+        return None
+
+    def rel_l2(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    iterations = int(gold["iterations"])
+    rec.rho = np.ones(rec.shape, np.float32)
+    rec.iterate(iterations)
+    img = rec.rho
+    return {
+        "iterations": iterations,
+        "fixture": "tests/golden/strip_1000x400_2e7_20it_ref.npz",
+        "events_sha256_match": (phantom.events_digest(events) ==
+                                str(gold["events_sha256"])),
+        "vs_reference_fp32_rel_l2": rel_l2(img, gold["rho_f32"]),
+        "vs_reference_fp64_rel_l2": rel_l2(img, gold["rho_f64"]),
+        "reference_fp32_vs_fp64_rel_l2": rel_l2(gold["rho_f32"],
+                                                gold["rho_f64"]),
+        "bar": 1e-4,
+        "image_total_fp64": rec.image_total(),
     }
 
 
@@ -93,6 +171,7 @@ def adjoint_timing(pystrip, phantom, name, sigma, iterations):
     _, _, gl = rec.loglik_gradient()
     t_loglik = time.perf_counter() - t
     reverse = (t_unrolled - iterations * t_iter) / iterations
+    spans = rec.span_table()
     rec.close()
     return {
         "workload": name,
@@ -108,6 +187,7 @@ def adjoint_timing(pystrip, phantom, name, sigma, iterations):
         "loss": loss,
         "d_loss_d_sigma": [float(x) for x in gs],
         "d_loglik_d_sigma": [float(x) for x in gl],
+        "span_table": spans,
     }
 
 
@@ -159,6 +239,7 @@ def osem_timing(pystrip, phantom, name, n_subsets, iterations, chunk):
         "image_total_fp64": rec.image_total(),
         "loglik_sigma_gradient_s": t_grad,
         "d_loglik_d_sigma": [float(x) for x in gs],
+        "span_table": rec.span_table(),
     }
     rec.close()
     return out
@@ -205,6 +286,7 @@ def batched_timing(pystrip, phantom, n_members, iterations):
         "batched_event_updates_per_s": n_events * iterations / batch_s,
         "plain_event_updates_per_s": n_events * iterations / plain_s,
         "speedup_over_plain": plain_s / batch_s,
+        "members_with_span_table": sum(bool(m.span_table()) for m in views),
     }
     b.close()
     return out
@@ -289,6 +371,8 @@ def main():
 
     rho = rec.rho
     mass = float(rho.astype(np.float64).sum())
+    spans = rec.span_table()
+    parity = reference_parity(rec, w, events)
 
     line = {
         "metric": "MLEM event-updates/s on 1 GPU vs host CPU",
@@ -313,6 +397,7 @@ def main():
             "sigma_z_mm": w.sigma_z,
             "sigma_dl_mm": w.sigma_dl,
             "algorithm": rec.algorithm,
+            "span_table": spans,
             "kernel_evaluations_per_iteration": n_evals,
             "mean_kernel_evaluations_per_event": n_evals / n_events,
             "device": rec.device_name(),
@@ -342,6 +427,7 @@ def main():
         "timing_s": {"event_generation": t_gen, "upload_and_setup": t_up,
                      "plan_and_support_count": t_plan},
         "check": {"image_mass": mass},
+        "parity": parity,
     }
     if not args.no_adjoint:
         rec.close()
@@ -357,8 +443,7 @@ def main():
         line["osem_largest"] = osem_timing(pystrip, phantom, args.osem_workload,
                                            16, 5, 10_000_000)
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample, threads)
+        line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample)
     rec.close()
     print(json.dumps(line), flush=True)
 

@@ -16,7 +16,6 @@
 // that of the plan: the derivative is the almost-everywhere derivative of the
 // pixel sums.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -272,13 +271,14 @@ __global__ __launch_bounds__(256) void mse_seed_kernel(
   }
 }
 
-__global__ void zero_double_kernel(double* __restrict__ out) {
-  if (threadIdx.x == 0) *out = 0.0;
-}
+// Deterministic FP64 sum of n floats: kSumBlocks block partials (each block
+// a fixed grid-stride share, summed in LDS in a fixed order), then one block
+// adds the partials in index order into *out.  No atomics, so repeated calls
+// give the same bits.
+constexpr int kSumBlocks = 512;
 
-// *out += sum of in[0..n) in FP64 (block sums in LDS, one atomic per block)
-__global__ __launch_bounds__(256) void sum_double_kernel(
-    int64_t n, const float* __restrict__ in, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void sum_partials_kernel(
+    int64_t n, const float* __restrict__ in, double* __restrict__ partials) {
   __shared__ double part[256];
   double acc = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -290,7 +290,22 @@ __global__ __launch_bounds__(256) void sum_double_kernel(
     if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(out, part[0]);
+  if (threadIdx.x == 0) partials[blockIdx.x] = part[0];
+}
+
+__global__ __launch_bounds__(256) void sum_final_kernel(
+    const double* __restrict__ partials, int n_partials,
+    double* __restrict__ out) {
+  __shared__ double part[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n_partials; i += blockDim.x) acc += partials[i];
+  part[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = part[0];
 }
 
 __global__ __launch_bounds__(256) void zero_kernel(int64_t n,
@@ -323,34 +338,30 @@ int ensure_work(AdjointWork& wk, const BandPlan& plan, int64_t n_pixels,
     free_dev(wk.ev_buf);
     BAND_TRY(hipMalloc(&wk.ev_buf, sizeof(float) * 5 * n));
     wk.ev_cap = n;
-    free_dev(wk.tmp);
-    wk.tmp_bytes = 0;
   }
   if (n_pixels > wk.img_cap) {
     free_dev(wk.img);
     BAND_TRY(hipMalloc(&wk.img, sizeof(float) * 3 * n_pixels));
     wk.img_cap = n_pixels;
-    free_dev(wk.tmp);
-    wk.tmp_bytes = 0;
   }
-  if (!wk.sums) BAND_TRY(hipMalloc(&wk.sums, sizeof(float) * 8));
-  if (!wk.tmp) {
-    size_t bytes = 0;
-    const int64_t longest = std::max<int64_t>(n, n_pixels);
-    BAND_TRY(hipcub::DeviceReduce::Sum(nullptr, bytes, wk.ev_buf, wk.sums,
-                                       (int)longest));
-    BAND_TRY(hipMalloc(&wk.tmp, bytes));
-    wk.tmp_bytes = bytes;
-  }
+  if (!wk.sums)
+    BAND_TRY(hipMalloc(&wk.sums, sizeof(double) * (kSumSlots + kSumBlocks)));
   return 0;
 }
 
-// Device sum of n floats into wk.sums[slot].
+// Device FP64 sum of n floats into wk.sums[slot] (deterministic).
 int reduce_sum(AdjointWork& wk, const float* in, int64_t n, int slot,
                hipStream_t stream, std::string* err) {
-  size_t bytes = wk.tmp_bytes;
-  BAND_TRY(hipcub::DeviceReduce::Sum(wk.tmp, bytes, in, wk.sums + slot,
-                                     (int)n, stream));
+  if (!wk.sums || slot < 0 || slot >= kSumSlots) {
+    if (err) *err = "adjoint reduction buffer not prepared";
+    return -1;
+  }
+  double* partials = wk.sums + kSumSlots;
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(kSumBlocks), dim3(256), 0,
+                     stream, n, in, partials);
+  hipLaunchKernelGGL(sum_final_kernel, dim3(1), dim3(256), 0, stream,
+                     (const double*)partials, kSumBlocks, wk.sums + slot);
+  BAND_TRY(hipGetLastError());
   return 0;
 }
 
@@ -391,7 +402,6 @@ void adjoint_release(AdjointWork& wk) {
   free_dev(wk.ev_buf);
   free_dev(wk.img);
   free_dev(wk.sums);
-  free_dev(wk.tmp);
   wk = AdjointWork{};
 }
 
@@ -424,8 +434,8 @@ int band_loglik_terms(BandPlan& plan, const DeviceGeometry& g,
     return -1;
   if (bp_accum && band_backproject(plan, g, w1, bp_accum, stream, err))
     return -1;
-  float sums[3] = {0.f, 0.f, 0.f};
-  BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(float) * 3,
+  double sums[3] = {0.0, 0.0, 0.0};
+  BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(double) * 3,
                           hipMemcpyDeviceToHost, stream));
   BAND_TRY(hipStreamSynchronize(stream));
   *sum_log = sums[0];
@@ -438,17 +448,12 @@ int band_image_total(const DeviceGeometry& g, AdjointWork& wk,
                      const float* imgT, double* total, hipStream_t stream,
                      std::string* err) {
   const int64_t n = (int64_t)g.n_z * g.n_y;
-  if (!wk.sums) BAND_TRY(hipMalloc(&wk.sums, sizeof(float) * 8));
-  double* acc = reinterpret_cast<double*>(wk.sums) + 2;  // bytes 16..23
-  hipLaunchKernelGGL(zero_double_kernel, dim3(1), dim3(64), 0, stream, acc);
-  const unsigned blocks =
-      (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, 1024));
-  hipLaunchKernelGGL(sum_double_kernel, dim3(blocks), dim3(256), 0, stream,
-                     n, imgT, acc);
-  BAND_TRY(hipGetLastError());
+  if (!wk.sums)
+    BAND_TRY(hipMalloc(&wk.sums, sizeof(double) * (kSumSlots + kSumBlocks)));
+  if (reduce_sum(wk, imgT, n, 4, stream, err)) return -1;
   double host = 0.0;
-  BAND_TRY(hipMemcpyAsync(&host, acc, sizeof(double), hipMemcpyDeviceToHost,
-                          stream));
+  BAND_TRY(hipMemcpyAsync(&host, wk.sums + 4, sizeof(double),
+                          hipMemcpyDeviceToHost, stream));
   BAND_TRY(hipStreamSynchronize(stream));
   *total = host;
   return 0;
@@ -485,7 +490,7 @@ int band_mse_seed(const DeviceGeometry& g, AdjointWork& wk, const float* rhoT,
                   const float* targetT, float* gbarT, double* loss,
                   hipStream_t stream, std::string* err) {
   const int64_t n_pixels = (int64_t)g.n_z * g.n_y;
-  if (n_pixels > wk.img_cap || !wk.tmp || !wk.sums) {
+  if (n_pixels > wk.img_cap || !wk.sums) {
     if (err) *err = "adjoint work buffers not prepared";
     return -1;
   }
@@ -495,11 +500,11 @@ int band_mse_seed(const DeviceGeometry& g, AdjointWork& wk, const float* rhoT,
                      gbarT, sq);
   BAND_TRY(hipGetLastError());
   if (reduce_sum(wk, sq, n_pixels, 0, stream, err)) return -1;
-  float total = 0.f;
-  BAND_TRY(hipMemcpyAsync(&total, wk.sums, sizeof(float),
+  double total = 0.0;
+  BAND_TRY(hipMemcpyAsync(&total, wk.sums, sizeof(double),
                           hipMemcpyDeviceToHost, stream));
   BAND_TRY(hipStreamSynchronize(stream));
-  if (loss) *loss = (double)total / (double)n_pixels;
+  if (loss) *loss = total / (double)n_pixels;
   return 0;
 }
 
@@ -536,7 +541,7 @@ int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
   hipLaunchKernelGGL(zero_kernel, dim3(pg), dim3(256), 0, stream, n_pixels,
                      bp2);
   BAND_TRY(hipGetLastError());
-  float sums[2] = {0.f, 0.f};
+  double sums[2] = {0.0, 0.0};
   if (n > 0) {
     if (launch_moments<2>(plan, g, wk, rhoT_k, hT, stream, err)) return -1;
     const unsigned blocks = (unsigned)((n + 255) / 256);
@@ -551,7 +556,7 @@ int band_mlem_adjoint_step(BandPlan& plan, const DeviceGeometry& g,
     if ((!bpT_k && band_backproject(plan, g, w1, bp1, stream, err)) ||
         band_backproject(plan, g, w2, bp2, stream, err))
       return -1;
-    BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(float) * 2,
+    BAND_TRY(hipMemcpyAsync(sums, wk.sums, sizeof(double) * 2,
                             hipMemcpyDeviceToHost, stream));
   }
   hipLaunchKernelGGL(reverse_combine_kernel, dim3(pg), dim3(256), 0, stream,

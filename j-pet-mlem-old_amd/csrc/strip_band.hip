@@ -1112,6 +1112,26 @@ bool band_supported_grid(const DeviceGeometry& g) {
   return g.n_z <= 32767 && g.n_y <= 32767;
 }
 
+// Upper bound of the clamped bbox width in columns for any event.  The bbox
+// half width is int(bb_z / w + 0.5) with bb_z = 3 / sqrt(C - B^2 / (4A))
+// (gaussian_kernel.h:108-128); with u = tan^2, C - B^2 / (4A) =
+// 2 iz - 4 u iz^2 / (4 (1 + u) id + 2 u iz) falls monotonically in u towards
+// 8 iz id / (4 id + 2 iz), so that limit bounds bb_z for every angle (one
+// more column of margin covers the FP32 evaluation).
+int band_max_box_width(const DeviceGeometry& g) {
+  const double iz = g.inv_pow_sigma_z, id = g.inv_pow_sigma_dl;
+  const double k = 8.0 * iz * id / (4.0 * id + 2.0 * iz);
+  if (!(k > 0.0)) return g.n_z;
+  const double hw = std::floor(3.0 / std::sqrt(k) / g.pixel_width + 0.5) + 1.0;
+  return (int)std::min<double>(2.0 * hw, (double)g.n_z);
+}
+
+bool band_window_fits(const DeviceGeometry& g) {
+  // the narrowest band (16 rows) and the widest bucket (64 columns)
+  return (size_t)(64 + band_max_box_width(g)) * 16 * sizeof(float) <=
+         kLdsLimit;
+}
+
 void band_release(BandPlan& plan) {
   free_list(plan.bwd);
   free_list(plan.fwd);
@@ -1132,8 +1152,12 @@ void band_release(BandPlan& plan) {
   plan.tuning = keep;
 }
 
-int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
-                 int64_t n_events, hipStream_t stream, std::string* err) {
+namespace {
+
+int band_prepare_once(BandPlan& plan, const DeviceGeometry& g,
+                      const EventSoA& ev, int64_t n_events, hipStream_t stream,
+                      bool allow_spans, bool* built_spans, std::string* err) {
+  *built_spans = false;
   if (n_events > 0x7fffffff) {
     if (err) *err = "too many events for one band plan";
     return -1;
@@ -1217,14 +1241,21 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
                        ev, plan.order, n, plan.records);
     BAND_TRY(hipGetLastError());
   }
-  // span table, when its byte runs fit and it takes at most half the free
-  // device memory
+  // span table, when its byte runs fit, it takes at most half the free
+  // device memory and what the rest of the plan needs still fits beside it:
+  // pair lists with their sort buffers (16 B per pair at 16-row bands, a
+  // record reaching at most rows / 16 + 2 bands, in both passes) and the
+  // per-(event, band) partials
   plan.span_stride = max_extent[1];
-  if (n > 0 && plan.tuning.spans && max_width <= 255) {
+  if (n > 0 && allow_spans && plan.tuning.spans && max_width <= 255) {
     const size_t bytes = sizeof(uint16_t) * (size_t)n * plan.span_stride;
+    const size_t bands = (size_t)max_extent[1] / 16 + 2;
+    const size_t rest = (size_t)n * bands * (2 * 16 + sizeof(float)) +
+                        (size_t)n * 16;
     size_t free_bytes = 0, total_bytes = 0;
     BAND_TRY(hipMemGetInfo(&free_bytes, &total_bytes));
-    if (bytes <= free_bytes / 2) {
+    if (bytes <= free_bytes / 2 && bytes + rest <= free_bytes) {
+      *built_spans = true;
       BAND_TRY(hipMalloc(&plan.spans, bytes));
       const unsigned span_blocks = (unsigned)((n + 3) / 4);
       hipLaunchKernelGGL(span_table_kernel, dim3(span_blocks), dim3(256), 0,
@@ -1241,13 +1272,16 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   // a task reaches at most its bucket plus the widest bbox (the bound used
   // to pick band heights); its actual window is set by its pairs' runs
   plan.Wt = bucket + max_width;
-  const int Hf = tu.fwd_rows == 16 ? 16 : (tu.fwd_rows == 32 ? 32 : 64);
+  // band heights as asked, halved (down to 16 rows) where a task window of
+  // the widest bbox would not fit in LDS
+  int Hf = tu.fwd_rows == 16 ? 16 : (tu.fwd_rows == 32 ? 32 : 64);
   int Hb = tu.bwd_rows == 64 ? 64 : (tu.bwd_rows == 16 ? 16 : 32);
+  while (Hf > 16 && (size_t)plan.Wt * Hf * sizeof(float) > kLdsLimit) Hf /= 2;
+  while (Hb > 16 && (size_t)plan.Wt * Hb * sizeof(float) > kLdsLimit) Hb /= 2;
   if ((size_t)plan.Wt * Hf * sizeof(float) > kLdsLimit) {
     if (err) *err = "task window does not fit in LDS";
     return -1;
   }
-  if ((size_t)plan.Wt * Hb * sizeof(float) > kLdsLimit) Hb = 32;
   const uint16_t* sp = plan.spans;
   const int ss = plan.span_stride;
   std::vector<int> first;
@@ -1331,6 +1365,27 @@ int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
   BAND_TRY(hipStreamSynchronize(stream));
   plan.ready = true;
   return 0;
+}
+
+}  // namespace
+
+int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
+                 int64_t n_events, hipStream_t stream, std::string* err) {
+  std::string first;
+  bool spans = false;
+  if (!band_prepare_once(plan, g, ev, n_events, stream, true, &spans, &first))
+    return 0;
+  // a plan that ran out of memory beside its span table is built again
+  // without it (the kernels then solve every row's run themselves)
+  if (spans && first.find("out of memory") != std::string::npos) {
+    (void)hipGetLastError();
+    free_ptr(plan.spans);
+    if (!band_prepare_once(plan, g, ev, n_events, stream, false, &spans, err))
+      return 0;
+    return -1;
+  }
+  if (err) *err = first;
+  return -1;
 }
 
 int band_zero_ints(int* p, int n, hipStream_t stream, std::string* err) {

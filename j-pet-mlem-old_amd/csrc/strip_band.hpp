@@ -195,6 +195,11 @@ int band_launch_batch_update(const BandInstance* desc, int n_instances,
 
 // True when the row-band algorithm can run on this grid.
 bool band_supported_grid(const DeviceGeometry& g);
+// Upper bound of any event's clamped bbox width (columns), from sigma and the
+// pixel width alone.
+int band_max_box_width(const DeviceGeometry& g);
+// A 16-row task window of the widest possible bbox fits in LDS.
+bool band_window_fits(const DeviceGeometry& g);
 
 // Builds the pair lists and tasks for the resident events.
 int band_prepare(BandPlan& plan, const DeviceGeometry& g, const EventSoA& ev,
@@ -259,10 +264,10 @@ struct AdjointWork {
   int64_t ev_cap = 0;
   float* img = nullptr;      // 3 column-major images: h, bp1, bp2
   int64_t img_cap = 0;
-  float* sums = nullptr;     // device reduction results (8 floats)
-  void* tmp = nullptr;
-  size_t tmp_bytes = 0;
+  // FP64 reduction results (kSumSlots) followed by the block partials
+  double* sums = nullptr;
 };
+constexpr int kSumSlots = 8;
 
 void adjoint_release(AdjointWork& wk);
 

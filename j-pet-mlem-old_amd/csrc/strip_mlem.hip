@@ -570,7 +570,8 @@ int launch_event_pass(strip_mlem_ctx* ctx, const float* rho) {
 
 bool band_supported(const strip_mlem_ctx* ctx) {
   return ctx->mode.kernel == 0 && ctx->mode.sensitivity != 1 &&
-         strip::band_supported_grid(ctx->g);
+         strip::band_supported_grid(ctx->g) &&
+         strip::band_window_fits(ctx->g);
 }
 
 int effective_algorithm(const strip_mlem_ctx* ctx) {
@@ -1373,7 +1374,13 @@ int strip_mlem_set_algorithm(strip_mlem_ctx* ctx, int algorithm) {
   if (algorithm == 2 && !band_supported(ctx))
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
                        "the row-band algorithm needs the Gaussian kernel, no "
-                       "sensitivity in the denominator and n_z <= 8192");
+                       "sensitivity in the denominator, n_z and n_y <= 32767 "
+                       "and a 16-row window of the widest bbox within 160 KiB "
+                       "of LDS");
+  if (algorithm == 1 && ctx->n_subsets > 1)
+    return strip::fail(STRIP_MLEM_ERR_STATE,
+                       "ordered subsets run on the row-band algorithm: set "
+                       "one subset before choosing the bbox kernel");
   ctx->algorithm = algorithm;
   return STRIP_MLEM_OK;
 }
@@ -2065,9 +2072,10 @@ int strip_mlem_batch_add(strip_mlem_batch* b,
   strip_mlem_ctx* c = nullptr;
   int rc = strip_mlem_create(geometry, &mode, b->device, &c);
   if (rc) return rc;
-  if (!strip::band_supported_grid(c->g)) {
+  if (!strip::band_supported_grid(c->g) || !strip::band_window_fits(c->g)) {
     strip_mlem_destroy(c);
-    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED, "grid too large for a batch");
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "grid too large or bbox too wide for a batch");
   }
   c->algorithm = 2;
   c->band.tuning = b->tuning;

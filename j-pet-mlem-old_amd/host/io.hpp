@@ -193,13 +193,20 @@ inline void write_tuples_text(const std::string& fn, const float* img,
     }
 }
 
+/// Start image for --rho (reconstruction_cmd.cpp:145-165).  Binary files
+/// hold n floats, or n int32 counts that become count + 1000 with
+/// --rho-is-int.  A ".txt" file is read as the reference reads it: its
+/// PixelMap text extractor streams every value into a copy of the pixel
+/// (`for (auto voxel : map) in >> voxel`, pixel_map.h:127-132), so the image
+/// keeps its values (rho = 1) and a missing file is no error.
 inline void read_image(const std::string& fn, bool is_int, float* img,
                        size_t n) {
   const bool txt = fn.size() >= 4 && fn.compare(fn.size() - 4, 4, ".txt") == 0;
   if (txt) {
     std::ifstream in(fn);
-    if (!in) throw std::runtime_error("cannot open start image: " + fn);
-    for (size_t i = 0; i < n && (in >> img[i]); ++i) {
+    for (size_t i = 0; i < n; ++i) {
+      float copy = img[i];
+      in >> copy;
     }
     return;
   }

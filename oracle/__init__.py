@@ -22,6 +22,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_LIB = os.path.join(HERE, "_ref", "libstrip_ref.so")
+# the same harness built with -march=x86-64-v4 (AVX-512)
+REF_LIB_V4 = os.path.join(HERE, "_ref", "libstrip_ref_v4.so")
+AVX512_FLAGS = ("avx512f", "avx512cd", "avx512bw", "avx512dq", "avx512vl")
 
 
 class OracleGeometry(ctypes.Structure):
@@ -52,7 +55,7 @@ MODE_CPU = (0, 0, 0, 1)
 MODE_LEGACY_GPU = (1, 1, 1, 0)
 
 _lib = None
-_ref = None
+_ref = {}
 
 
 def build(reference: bool = False):
@@ -128,13 +131,39 @@ def ref_available() -> bool:
     return os.path.exists(REF_LIB)
 
 
-def ref():
-    """The reference's own code (raises if oracle/_ref was not built)."""
-    global _ref
-    if _ref is None:
-        if not ref_available():
-            raise RuntimeError("oracle/_ref/libstrip_ref.so not built")
-        L = ctypes.CDLL(REF_LIB)
+def host_cpu():
+    """(model name, set of ISA flags) of this host, from /proc/cpuinfo."""
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, flags
+
+
+def ref_native_isa() -> str:
+    """The widest build of the reference harness this host can run:
+    "x86-64-v4" where the AVX-512 build exists and the CPU has its
+    extensions, else "x86-64-v3"."""
+    _, flags = host_cpu()
+    if os.path.exists(REF_LIB_V4) and all(f in flags for f in AVX512_FLAGS):
+        return "x86-64-v4"
+    return "x86-64-v3"
+
+
+def ref(isa="x86-64-v3"):
+    """The reference's own code (raises if oracle/_ref was not built), in
+    the x86-64-v3 build or, with isa="x86-64-v4", the AVX-512 build."""
+    if isa not in _ref:
+        path = REF_LIB_V4 if isa == "x86-64-v4" else REF_LIB
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built")
+        L = ctypes.CDLL(path)
         G = ctypes.POINTER(OracleGeometry)
         f = ctypes.c_float
         d = ctypes.c_double
@@ -160,8 +189,8 @@ def ref():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
-        _ref = L
-    return _ref
+        _ref[isa] = L
+    return _ref[isa]
 
 
 def _p(a, t):
@@ -235,17 +264,17 @@ def event_params(g, z_u, z_d, dl, dtype=np.float32):
 
 
 def ref_mlem(g, events, rho0=None, iterations=1, n_threads=0,
-             dtype=np.float32):
+             dtype=np.float32, isa="x86-64-v3"):
     ev = np.ascontiguousarray(events, dtype=dtype)
     shape = (g.n_y_pixels, g.n_z_pixels)
     rho = (np.ones(shape, dtype) if rho0 is None
            else np.ascontiguousarray(rho0, dtype=dtype).reshape(shape).copy())
     if dtype == np.float32:
-        ref().ref_mlem_f32(ctypes.byref(g), _p(ev, ctypes.c_float),
+        ref(isa).ref_mlem_f32(ctypes.byref(g), _p(ev, ctypes.c_float),
                            ev.shape[0], _p(rho, ctypes.c_float), iterations,
                            n_threads)
     else:
-        ref().ref_mlem_f64(ctypes.byref(g), _p(ev, ctypes.c_double),
+        ref(isa).ref_mlem_f64(ctypes.byref(g), _p(ev, ctypes.c_double),
                            ev.shape[0], _p(rho, ctypes.c_double), iterations,
                            n_threads)
     return rho
@@ -262,20 +291,22 @@ def _dp(a):
     return None if a is None else a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def loglik_grad(g, events, rho, sigma=None):
+def loglik_grad(g, events, rho, sigma=None, want_grad_rho=True):
     """FP64 Poisson log-likelihood L = sum_{D>0} log D_e - sum rho and its
     gradients (d L / d rho, d L / d (sigma_z, sigma_dl)); the kernel at
-    `sigma` (default: the geometry's), the support at the geometry's sigmas."""
+    `sigma` (default: the geometry's), the support at the geometry's sigmas.
+    With want_grad_rho False the image gradient is not formed (None)."""
     ev = np.ascontiguousarray(events, np.float64)
     r = np.ascontiguousarray(rho, np.float64)
     sg = np.array(sigma if sigma is not None else (g.sigma_z, g.sigma_dl),
                   np.float64)
     value = np.zeros(1)
-    grad_rho = np.empty_like(r)
+    grad_rho = np.empty_like(r) if want_grad_rho else None
     grad_sigma = np.zeros(2)
     rc = lib().oracle_loglik_grad_f64(ctypes.byref(g), _dp(ev), ev.shape[0],
                                       _dp(r), _dp(sg), _dp(value),
-                                      _dp(grad_rho), _dp(grad_sigma))
+                                      _dp(grad_rho) if want_grad_rho else None,
+                                      _dp(grad_sigma))
     if rc:
         raise RuntimeError("oracle_loglik_grad_f64 failed")
     return float(value[0]), grad_rho, grad_sigma

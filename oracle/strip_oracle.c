@@ -228,6 +228,8 @@ static void diff_bwd_visit(int64_t idx, double K, double gz, double gd,
   if (b->out2) b->out2[idx] += K * (b->wz * gz + b->wd * gd) * b->w2;
 }
 
+/* Events are split over the OpenMP threads (OMP_NUM_THREADS); each thread
+ * keeps its own sums and gradient image, added in thread order at the end. */
 int oracle_loglik_grad_f64(const oracle_geometry* g, const double* events,
                            int64_t n_events, const double* rho,
                            const double* sigma, double* value,
@@ -237,24 +239,53 @@ int oracle_loglik_grad_f64(const oracle_geometry* g, const double* events,
   diff_sigma_t k;
   diff_sigma(sigma, &k);
   int64_t np = (int64_t)s.n_z * s.n_y;
-  double L = 0, gz = 0, gd = 0;
-  if (grad_rho)
-    for (int64_t i = 0; i < np; ++i) grad_rho[i] = 0;
-  for (int64_t e = 0; e < n_events; ++e) {
-    diff_fwd_t f = {rho, NULL, 0, 0, 0, 0, 0, 0};
-    diff_event(&s, &k, events + 3 * e, diff_fwd_visit, &f);
-    if (!(f.t0 > 0)) continue;
-    L += log(f.t0);
-    gz += f.z0 / f.t0;
-    gd += f.y0 / f.t0;
-    if (grad_rho) {
-      diff_bwd_t b = {grad_rho, 1 / f.t0, NULL, 0, 0, NULL, 0};
-      diff_event(&s, &k, events + 3 * e, diff_bwd_visit, &b);
+  int nt = omp_get_max_threads();
+  double* part = (double*)calloc((size_t)3 * nt, sizeof(double));
+  double* imgs = grad_rho ? (double*)calloc((size_t)np * nt, sizeof(double))
+                          : NULL;
+  if (!part || (grad_rho && !imgs)) {
+    free(part);
+    free(imgs);
+    return -1;
+  }
+#pragma omp parallel num_threads(nt)
+  {
+    int t = omp_get_thread_num();
+    double* gimg = imgs ? imgs + np * t : NULL;
+    double L = 0, gz = 0, gd = 0;
+#pragma omp for schedule(dynamic, 64)
+    for (int64_t e = 0; e < n_events; ++e) {
+      diff_fwd_t f = {rho, NULL, 0, 0, 0, 0, 0, 0};
+      diff_event(&s, &k, events + 3 * e, diff_fwd_visit, &f);
+      if (!(f.t0 > 0)) continue;
+      L += log(f.t0);
+      gz += f.z0 / f.t0;
+      gd += f.y0 / f.t0;
+      if (gimg) {
+        diff_bwd_t b = {gimg, 1 / f.t0, NULL, 0, 0, NULL, 0};
+        diff_event(&s, &k, events + 3 * e, diff_bwd_visit, &b);
+      }
     }
+    part[3 * t] = L;
+    part[3 * t + 1] = gz;
+    part[3 * t + 2] = gd;
+  }
+  double L = 0, gz = 0, gd = 0;
+  for (int t = 0; t < nt; ++t) {
+    L += part[3 * t];
+    gz += part[3 * t + 1];
+    gd += part[3 * t + 2];
   }
   for (int64_t i = 0; i < np; ++i) L -= rho[i];
-  if (grad_rho)
-    for (int64_t i = 0; i < np; ++i) grad_rho[i] -= 1;
+  if (grad_rho) {
+    for (int64_t i = 0; i < np; ++i) {
+      double acc = 0;
+      for (int t = 0; t < nt; ++t) acc += imgs[np * t + i];
+      grad_rho[i] = acc - 1;
+    }
+  }
+  free(part);
+  free(imgs);
   if (value) *value = L;
   if (grad_sigma) {
     grad_sigma[0] = gz / sigma[0];
@@ -296,22 +327,48 @@ int oracle_mlem_loss_f64(const oracle_geometry* g, const double* events,
   }
   memcpy(rho, rho0, sizeof(double) * np);
   if (d_rho0) memcpy(drho, d_rho0, sizeof(double) * np);
+  int nt = omp_get_max_threads();
+  /* per-thread b1 and b2 images, added in thread order */
+  double* priv = (double*)malloc(sizeof(double) * (size_t)np * 2 * nt);
+  if (!priv) {
+    free(rho);
+    free(drho);
+    free(b1);
+    free(b2);
+    return -1;
+  }
   for (int it = 0; it < n_iterations; ++it) {
-    for (int64_t i = 0; i < np; ++i) b1[i] = b2[i] = 0;
-    for (int64_t e = 0; e < n_events; ++e) {
-      diff_fwd_t f = {rho, tangent ? drho : NULL, 0, 0, 0, 0, 0, 0};
-      diff_event(&s, &k, events + 3 * e, diff_fwd_visit, &f);
-      if (!(f.t0 > 0)) continue;
-      double D = f.t0;
-      /* D' = sum K' rho + sum K rho' = wz z0 + wd y0 + t1 */
-      double dD = wz * f.z0 + wd * f.y0 + f.t1;
-      diff_bwd_t b = {b1, 1 / D, NULL, wz, wd, tangent ? b2 : NULL, 1 / D};
-      diff_event(&s, &k, events + 3 * e, diff_bwd_visit, &b);
-      if (tangent) {
-        /* the K D'/D^2 part of b2 */
-        diff_bwd_t c = {b2, -dD / (D * D), NULL, 0, 0, NULL, 0};
-        diff_event(&s, &k, events + 3 * e, diff_bwd_visit, &c);
+    memset(priv, 0, sizeof(double) * (size_t)np * 2 * nt);
+#pragma omp parallel num_threads(nt)
+    {
+      int t = omp_get_thread_num();
+      double* p1 = priv + (size_t)np * 2 * t;
+      double* p2 = p1 + np;
+#pragma omp for schedule(dynamic, 64)
+      for (int64_t e = 0; e < n_events; ++e) {
+        diff_fwd_t f = {rho, tangent ? drho : NULL, 0, 0, 0, 0, 0, 0};
+        diff_event(&s, &k, events + 3 * e, diff_fwd_visit, &f);
+        if (!(f.t0 > 0)) continue;
+        double D = f.t0;
+        /* D' = sum K' rho + sum K rho' = wz z0 + wd y0 + t1 */
+        double dD = wz * f.z0 + wd * f.y0 + f.t1;
+        diff_bwd_t b = {p1, 1 / D, NULL, wz, wd, tangent ? p2 : NULL, 1 / D};
+        diff_event(&s, &k, events + 3 * e, diff_bwd_visit, &b);
+        if (tangent) {
+          /* the K D'/D^2 part of b2 */
+          diff_bwd_t c = {p2, -dD / (D * D), NULL, 0, 0, NULL, 0};
+          diff_event(&s, &k, events + 3 * e, diff_bwd_visit, &c);
+        }
       }
+    }
+    for (int64_t i = 0; i < np; ++i) {
+      double a1 = 0, a2 = 0;
+      for (int t = 0; t < nt; ++t) {
+        a1 += priv[(size_t)np * 2 * t + i];
+        a2 += priv[(size_t)np * 2 * t + np + i];
+      }
+      b1[i] = a1;
+      b2[i] = a2;
     }
     for (int64_t i = 0; i < np; ++i) {
       double r = rho[i];
@@ -319,6 +376,7 @@ int oracle_mlem_loss_f64(const oracle_geometry* g, const double* events,
       if (tangent) drho[i] = drho[i] * b1[i] + r * b2[i];
     }
   }
+  free(priv);
   double l = 0, j = 0;
   for (int64_t i = 0; i < np; ++i) {
     double d = rho[i] - target[i];

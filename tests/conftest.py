@@ -48,3 +48,21 @@ def phantom_mod():
 def print_case(*parts):
     """Names the case on stdout (flushed) before a GPU step."""
     print("[case]", *parts, flush=True)
+
+
+def expected_digest(name, n, seed=None):
+    """SHA-256 of a seeded event set (tests/golden/event_digests.json,
+    written by tests/golden/make_event_digests.py)."""
+    import json
+    path = os.path.join(ROOT, "tests", "golden", "event_digests.json")
+    with open(path) as f:
+        table = json.load(f)
+    return table[f"{name}:{n}:{'default' if seed is None else seed}"]
+
+
+def check_events(phantom_mod, events, name, seed=None):
+    """Asserts the generated events are the pinned input set: a changed
+    generator stream fails here, not as a parity error."""
+    got = phantom_mod.events_digest(events)
+    assert got == expected_digest(name, len(events), seed), (
+        f"{name}: the seeded events differ from the pinned set ({got})")

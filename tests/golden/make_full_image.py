@@ -5,7 +5,8 @@ workload: the REFERENCE's own Reconstruction<float, GaussianKernel<float>>
 under /root/reference/src), 20 MLEM iterations from rho = 1 on the
 workload's seeded events (pystrip/phantom.py), and with --fp64 also the
 Reconstruction<double, ...> image.  The events are regenerated from the seed
-by the test, so only the images are stored.  About 45 min (FP32) and 70 min
+by the test, so only the images and the events' SHA-256 (which the test
+checks before it compares images) are stored.  About 45 min (FP32) and 70 min
 (FP64) on 8 host threads.
 
   python tests/golden/make_full_image.py [--fp64] [--threads N]
@@ -38,7 +39,8 @@ def main():
     ev = w.events()
     g = oracle.geometry(*w.oracle_geometry_tuple())
     out = {"workload": np.array(NAME), "n_events": np.int64(len(ev)),
-           "seed": np.int64(w.seed), "iterations": np.int32(ITERATIONS)}
+           "seed": np.int64(w.seed), "iterations": np.int32(ITERATIONS),
+           "events_sha256": np.array(phantom.events_digest(ev))}
     out["rho_f32"] = oracle.ref_mlem(g, ev, iterations=ITERATIONS,
                                      n_threads=args.threads)
     if args.fp64:

@@ -336,6 +336,8 @@ def test_full_size_against_reference_image(pystrip_mod, phantom_mod,
     w = phantom_mod.WORKLOADS[CFG2]
     assert w.seed == int(gold["seed"])
     ev = w.events(int(gold["n_events"]))
+    # the events the reference image was made from
+    assert phantom_mod.events_digest(ev) == str(gold["events_sha256"])
     rec = pystrip_mod.Reconstruction(w.geometry())
     rec.set_events(ev)
     rec.iterate(int(gold["iterations"]))
@@ -347,3 +349,32 @@ def test_full_size_against_reference_image(pystrip_mod, phantom_mod,
               "reference FP32 vs FP64",
               oracle_mod.rel_l2(gold["rho_f32"], gold["rho_f64"]))
     assert e32 <= 1e-4
+
+
+def test_legacy_constructed_cases(pystrip_mod, oracle_mod):
+    # the legacy warp kernel's float-reciprocal row split on a bbox 41
+    # columns wide (columns 0 of rows 26..32 skipped, column 41 of rows
+    # 28..31 visited) and its unguarded 1/D (rho0 = 0 on the support: NaN
+    # there), against the oracle's legacy mode, whose both behaviours are
+    # pinned in tests/test_oracle.py
+    print_case("legacy float split and unguarded 1/D")
+    G = pystrip_mod.Geometry.make(350, 500, 128, 64, 2, 20, 42)
+    og = oracle_mod.geometry(350, 500, 128, 64, 2, 2, 20, 42)
+    ev = np.array([[-87.0, -87.0, 0.0]], np.float32)
+    rec = pystrip_mod.Reconstruction(G, pystrip_mod.MODE_LEGACY_GPU)
+    rec.set_events(ev)
+    rec.iterate(1)
+    ref = oracle_mod.mlem(og, ev, iterations=1, mode=oracle_mod.MODE_LEGACY_GPU)
+    gpu = rec.rho
+    assert np.array_equal(gpu > 0, ref > 0)
+    assert np.all(gpu[26:33, 0] == 0) and np.all(gpu[28:32, 41] > 0)
+    assert oracle_mod.rel_l2(gpu, ref) <= 1e-5
+    zero = np.zeros(rec.shape, np.float32)
+    rec.rho = zero
+    rec.iterate(1)
+    ref0 = oracle_mod.mlem(og, ev, rho0=zero, iterations=1,
+                           mode=oracle_mod.MODE_LEGACY_GPU)
+    nan = np.isnan(rec.rho)
+    assert nan.sum() > 100
+    assert np.array_equal(nan, np.isnan(ref0))
+    assert np.all(rec.rho[~nan] == 0)

@@ -402,3 +402,134 @@ def test_osem_oracle_composition(oracle_mod, phantom_mod):
     d = oracle_mod.forward(g, ev[lo:hi], before, dtype=np.float64)
     expect = n / (hi - lo) * np.count_nonzero(d > 0)
     assert abs(got.sum() - expect) <= 1e-9 * expect
+
+
+# -- 5. legacy --gpu composition (warp_granularity.cuh:53-152) ------------------
+#
+# The CUDA warp kernel cannot be built here, so the oracle's legacy mode is
+# pinned by what its arithmetic fixes: the single-event closed form (analytic
+# kernel, 5-point sensitivity in the denominator only, rho in both), the
+# unguarded 1/D, and the pixels the float-reciprocal row split visits.
+
+LEGACY = (1, 1, 1, 0)
+
+
+def _support(L, g_tuple, e, analytic=False):
+    """FP64 enumeration of one event's clamped bbox: {(iy, ix): K} over the
+    pixels with Q <= 9 (reconstruction.h:255-264)."""
+    R, _, n_z, n_y, pw, ph, sz, sdl = g_tuple[:8]
+    x0, y0, x1, y1 = e["box"]
+    fn = L.oracle_analytic_kernel_f64 if analytic else \
+        L.oracle_gaussian_kernel_f64
+    out = {}
+    for iy in range(y0, y1):
+        for ix in range(x0, x1):
+            pz = -n_z * pw / 2 + pw / 2 + ix * pw
+            py = n_y * ph / 2 - ph / 2 - iy * ph
+            dy, dz = py - e["y"], pz - e["z"]
+            if e["A"] * dy * dy + e["B"] * dy * dz + e["C"] * dz * dz <= 9:
+                out[(iy, ix)] = fn(sz, sdl, e["y"], e["tan"], e["sec"], R,
+                                   dz, dy)
+    return out
+
+
+def _missplit_rows(width, height, dtype):
+    """Rows r whose first index r*width the product index * (1/width) in
+    `dtype` truncates to r - 1 (IEEE multiplication of the rounded
+    reciprocal)."""
+    inv = dtype(1) / dtype(width)
+    return [r for r in range(1, height)
+            if int(dtype(r * width) * inv) != r]
+
+
+def test_legacy_single_event_closed_form(oracle_mod):
+    """One event, any rho0: rho1(p) = K^A(p) rho0(p) / sum_q K^A(q) rho0(q)
+    s5(q) with K^A the analytic kernel and s5 the 5-point pixel
+    sensitivity, which enters the denominator only."""
+    L = oracle_mod.lib()
+    gt = (350, 500, 64, 48, 2, 2, 10, 42)
+    g = oracle_mod.geometry(*gt)
+    rng = np.random.default_rng(12)
+    rho0 = rng.uniform(0.5, 1.5, size=(48, 64))
+    s5 = oracle_mod.sensitivity_map(g, 1, dtype=np.float64)
+    s0 = oracle_mod.sensitivity_map(g, 0, dtype=np.float64)
+    assert np.max(np.abs(s5 - s0)) > 1e-4  # the two maps are distinguishable
+    checked = 0
+    for zu, zd, dl in [(12.0, -30.0, 15.0), (-40.0, 25.0, -60.0),
+                       (5.0, 8.0, 90.0), (70.0, 20.0, -20.0)]:
+        e = _event(oracle_mod, g, zu, zd, dl)
+        x0, y0, x1, y1 = e["box"]
+        if _missplit_rows(x1 - x0, y1 - y0, np.float64):
+            continue
+        ka = _support(L, gt, e, analytic=True)
+        den = sum(k * rho0[p] * s5[p] for p, k in ka.items())
+        expect = np.zeros((48, 64))
+        for p, k in ka.items():
+            expect[p] = k * rho0[p] / den
+        got = oracle_mod.mlem(g, np.array([[zu, zd, dl]]), rho0=rho0,
+                              iterations=1, mode=LEGACY, dtype=np.float64)
+        assert np.allclose(got, expect, rtol=1e-12, atol=1e-300)
+        # the CPU semantics differ (Gaussian kernel, no sensitivity)
+        cpu = oracle_mod.mlem(g, np.array([[zu, zd, dl]]), rho0=rho0,
+                              iterations=1, dtype=np.float64)
+        assert not np.allclose(cpu, got, rtol=1e-3)
+        checked += 1
+    assert checked >= 3
+
+
+def test_legacy_unguarded_zero_denominator(oracle_mod):
+    """rho0 = 0 on an event's support: D = 0, and 1/D is not guarded in the
+    warp kernel (warp_granularity.cuh:122), so every support pixel receives
+    K * 0 * inf = NaN; the CPU path guards D > 0 (reconstruction.h:290)."""
+    gt = (350, 500, 64, 48, 2, 2, 10, 42)
+    g = oracle_mod.geometry(*gt)
+    ev = np.array([[12.0, -30.0, 15.0]], np.float32)
+    support = oracle_mod.mlem(g, ev, iterations=1) > 0  # rho0 = 1
+    assert support.sum() > 100
+    zero = np.zeros((48, 64), np.float32)
+    legacy = oracle_mod.mlem(g, ev, rho0=zero, iterations=1, mode=LEGACY)
+    assert np.array_equal(np.isnan(legacy), support)
+    assert np.all(legacy[~support] == 0)
+    cpu = oracle_mod.mlem(g, ev, rho0=zero, iterations=1)
+    assert np.all(cpu == 0)
+
+
+def test_legacy_float_reciprocal_row_split(oracle_mod):
+    """A bbox 41 pixels wide (its left edge clamped at column 0): in FP32,
+    fl(1/41) * (41 r) truncates to r - 1 for r = 26..32, so the warp kernel
+    visits column x1 = 41 of row r - 1 instead of column 0 of row r
+    (warp_granularity.cuh:162-176).  Column 0 of rows 26..32 lies in the
+    ellipse (Q = 8 + dy^2/441 there) and gets nothing in legacy mode; column
+    41 of rows 28..31, outside the bbox, lies in the ellipse and gets a
+    share."""
+    # sigma_z 20 mm, 2 mm pixels: bb_z = 42.4 mm, half width 21; centre
+    # pixel 20 at t = 0, y = 0 -> bbox columns [0, 41), rows [0, 63)
+    gt = (350, 500, 128, 64, 2, 2, 20, 42)
+    g = oracle_mod.geometry(*gt)
+    e = _event(oracle_mod, g, -87.0, -87.0, 0.0)
+    assert e["box"] == (0, 0, 41, 63)
+    assert _missplit_rows(41, 63, np.float32)[:3] == [1, 2, 4]
+    assert set(range(26, 33)) <= set(_missplit_rows(41, 63, np.float32))
+    ev = np.array([[-87.0, -87.0, 0.0]], np.float32)
+    one = np.ones((64, 128), np.float32)
+    # no sensitivity and the Gaussian kernel isolate the split
+    split_only = (0, 0, 0, 1)
+    exact = oracle_mod.mlem(g, ev, rho0=one, iterations=1, mode=split_only)
+    assert np.all(exact[26:33, 0] > 0) and np.all(exact[:, 41] == 0)
+    legacy = oracle_mod.mlem(g, ev, rho0=one, iterations=1, mode=LEGACY)
+    assert np.all(legacy[26:33, 0] == 0)
+    assert np.all(legacy[28:32, 41] > 0)
+    assert np.all(legacy[[21, 22, 23, 24, 25, 33, 34], 0] > 0)
+    # every visited pixel is the float split of its linear index
+    w, h = 41, 63
+    inv = np.float32(1) / np.float32(w)
+    visited = np.zeros((64, 128), bool)
+    for i in range(w * h):
+        r = int(np.float32(i) * inv)
+        visited[r, i - w * r] = True
+    iy, ix = np.mgrid[0:64, 0:128]
+    dz = (-128.0 + 1.0 + 2.0 * ix) - e["z"]
+    dy = (64.0 - 1.0 - 2.0 * iy) - e["y"]
+    in_ellipse = e["A"] * dy * dy + e["B"] * dy * dz + e["C"] * dz * dz <= 9
+    assert np.array_equal(legacy > 0, visited & in_ellipse)
+    assert np.array_equal(exact > 0, in_ellipse & (ix < 41) & (iy < 63))
