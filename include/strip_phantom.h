@@ -79,6 +79,49 @@ int strip_phantom_generate(int device, const strip_phantom_params* params,
                            int* image_emitted, int* image_detected,
                            int* image_detected_w_error);
 
+/* Everything 2d_strip_phantom can write (phantom_cmd.cpp:210-307,330-444):
+ * the arrays the caller does not want are NULL. */
+typedef struct strip_phantom_outputs {
+  float* responses; /* 3 n: detected responses with error (z_u, z_d, dl) */
+  float* exact;     /* 3 n: the same without error (= the full response,
+                       scanner.h:252-255) */
+  float* events;    /* 4 n: emission point (z, y) and direction
+                       (sin phi, cos phi) of each detected event
+                       (Event(origin, phi), event.h:36-37) */
+  int* image_emitted;          /* n_y x n_z, emission points */
+  int* image_detected;         /* n_y x n_z, detected emission points */
+  int* image_detected_w_error; /* n_y x n_z, points from the responses with
+                                  error */
+  int tan_bins;                /* > 0: image_tan_bins is filled */
+  int* image_tan_bins;         /* tan_bins x n_y x n_z: detected events by the
+                                  point and tangent reconstructed from the
+                                  response with error, tangent planes of
+                                  width 2 max_tan / tan_bins from -max_tan,
+                                  max_tan = 1.1 L / (2 R)
+                                  (phantom_cmd.cpp:246-252,350-359) */
+  int64_t n_detected;          /* out */
+  int64_t n_emitted;           /* out */
+} strip_phantom_outputs;
+
+/* strip_phantom_generate with every output of the phantom tool
+ * (--full dumps and --tan-bins counts).  Replaces the Monte-Carlo loop of
+ * phantom_cmd.cpp:300-371. */
+int strip_phantom_generate_full(int device, const strip_phantom_params* params,
+                                const strip_phantom_region* regions,
+                                int n_regions, int64_t n,
+                                strip_phantom_outputs* out);
+
+/* The --tan-bins kernel map (phantom_cmd.cpp:393-431): for each voxel
+ * (column, row, tangent plane) of the tangent-binned image grid of params
+ * (n_z, n_y, pixel, image origin), the AnalyticKernel of an event at the
+ * voxel centre with the plane's central tangent, evaluated at the kernel
+ * point (kernel_z, kernel_y), times 4 R pixel^2 dtan sqrt(1 + tan^2), and
+ * divided by Scanner::sensitivity at the kernel point when
+ * params->only_detected.  out: tan_bins x n_y x n_z floats. */
+int strip_phantom_tan_kernel_map(int device, const strip_phantom_params* params,
+                                 int tan_bins, double kernel_z,
+                                 double kernel_y, float* out);
+
 /* Appends n_detected generated events directly to the context's resident
  * events (no host copy; the MI355X path for very large event lists).  The
  * context's own R, L, sigma_z and sigma_dl are used; params supplies seed,

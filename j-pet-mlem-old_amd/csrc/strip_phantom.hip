@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "strip_device.hpp"
 #include "strip_phantom.h"
 #include "strip_phantom.hpp"
 
@@ -102,7 +103,8 @@ __global__ __launch_bounds__(256) void emit_kernel(
     PhantomKernelArgs a, int64_t first, int64_t count,
     uint8_t* __restrict__ flags, float* __restrict__ resp,
     float* __restrict__ exact, int* __restrict__ pix_origin,
-    int* __restrict__ pix_err) {
+    int* __restrict__ pix_err, float* __restrict__ ev4,
+    int* __restrict__ pix_tan) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= count) return;
   const uint64_t index = (uint64_t)(first + j);
@@ -136,7 +138,15 @@ __global__ __launch_bounds__(256) void emit_kernel(
     if (!taken || attempt >= 1u << 20) break;
   }
   const float phi = 1.57079632679489661923f * (2.0f * phi_u - 1.0f);
-  const float t = sinf(phi) / cosf(phi);
+  const float sin_phi = sinf(phi), cos_phi = cosf(phi);
+  const float t = sin_phi / cos_phi;
+  if (ev4) {
+    // Event(origin, phi): direction (sin phi, cos phi) (event.h:36-37)
+    ev4[4 * j] = px;
+    ev4[4 * j + 1] = py;
+    ev4[4 * j + 2] = sin_phi;
+    ev4[4 * j + 3] = cos_phi;
+  }
   // to_projection_space_tan (scanner.h:107-113)
   const float z_u = px + (a.radius - py) * t;
   const float z_d = px - (a.radius + py) * t;
@@ -177,10 +187,17 @@ __global__ __launch_bounds__(256) void emit_kernel(
     const float zz = 0.5f * (w_u + w_d + 2.0f * yy * tt);
     const int ix = (int)floorf((zz - a.img_z0) / a.img_pixel);
     const int iy = (int)floorf((yy - a.img_y0) / a.img_pixel);
-    pix_err[j] = (detected && ix >= 0 && ix < a.img_nz && iy >= 0 &&
-                  iy < a.img_ny)
-                     ? iy * a.img_nz + ix
-                     : -1;
+    const bool in_image =
+        detected && ix >= 0 && ix < a.img_nz && iy >= 0 && iy < a.img_ny;
+    pix_err[j] = in_image ? iy * a.img_nz + ix : -1;
+    if (pix_tan) {
+      // its tangent bin (VariableVoxelSizeVoxelGrid::voxel_at,
+      // voxel_grid.h:109-118; VoxelMap index (plane n_y + row) n_z + column)
+      const int it = (int)floorf((tt - a.tan_left) / a.tan_voxel);
+      pix_tan[j] = (in_image && it >= 0 && it < a.tan_bins)
+                       ? (it * a.img_ny + iy) * a.img_nz + ix
+                       : -1;
+    }
   }
 }
 
@@ -189,8 +206,9 @@ __global__ __launch_bounds__(256) void emit_kernel(
 __global__ __launch_bounds__(256) void image_kernel(
     int64_t used, const uint8_t* __restrict__ flags,
     const int* __restrict__ pix_origin, const int* __restrict__ pix_err,
-    int* __restrict__ img_em, int* __restrict__ img_det,
-    int* __restrict__ img_err) {
+    const int* __restrict__ pix_tan, int* __restrict__ img_em,
+    int* __restrict__ img_det, int* __restrict__ img_err,
+    int* __restrict__ img_tan) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= used) return;
   const int p = pix_origin[j];
@@ -199,6 +217,43 @@ __global__ __launch_bounds__(256) void image_kernel(
   if (p >= 0 && img_det) atomicAdd(&img_det[p], 1);
   const int q = pix_err[j];
   if (q >= 0 && img_err) atomicAdd(&img_err[q], 1);
+  if (img_tan) {
+    const int v = pix_tan[j];
+    if (v >= 0) atomicAdd(&img_tan[v], 1);
+  }
+}
+
+// The --tan-bins kernel map (phantom_cmd.cpp:405-431): for every voxel
+// (column x, row y, tangent plane t) of the tangent-binned grid, the analytic
+// kernel of an event at the voxel centre with the plane's tangent, evaluated
+// at the kernel point, times the bin's phase-space volume
+// 4 R pixel^2 dtan sqrt(1 + tan^2), times inv_sens.
+__global__ __launch_bounds__(256) void tan_kernel_map_kernel(
+    DeviceGeometry g, int n_z, int n_y, int n_t, float pixel, float z0,
+    float y0, float tan_left, float tan_voxel, float kz, float ky,
+    float inv_sens, float* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t n = (int64_t)n_z * n_y * n_t;
+  if (i >= n) return;
+  const int x = (int)(i % n_z);
+  const int y = (int)((i / n_z) % n_y);
+  const int t = (int)(i / ((int64_t)n_z * n_y));
+  const float tan = ((float)t + 0.5f) * tan_voxel + tan_left;
+  const float sec = sqrtf(1.0f + tan * tan);
+  // PixelGrid::center_at (pixel_grid.h:54-57)
+  const float half = pixel / 2.0f;
+  const float cz = (z0 + half) + (float)x * pixel;
+  const float cy = (y0 + half) + (float)y * pixel;
+  const float k = analytic_kernel(g, cy, tan, sec, kz - cz, ky - cy);
+  const float bin_volume = pixel * pixel * tan_voxel;
+  const float volume = 4.0f * g.radius * bin_volume * sqrtf(1.0f + tan * tan);
+  out[i] = k * volume * inv_sens;
+}
+
+// Scanner::sensitivity at one point (for the kernel map's normalisation).
+__global__ void sensitivity_at_kernel(DeviceGeometry g, float z, float y,
+                                      float* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *out = scanner_sensitivity(g, z, y);
 }
 
 __global__ __launch_bounds__(256) void iota_kernel(int64_t first,
@@ -219,6 +274,10 @@ __global__ void random_blocks_kernel(uint64_t k0, uint64_t k1, uint64_t first,
 
 struct Triple {
   float v[3];
+};
+
+struct Quad {
+  float v[4];
 };
 
 }  // namespace
@@ -293,13 +352,27 @@ int phantom_prepare(PhantomDevice& P, const strip_phantom_params& p,
   P.args.img_pixel = p.pixel > 0 ? (float)p.pixel : 1.0f;
   P.args.img_z0 = (float)(-p.pixel * p.n_z / 2 + p.image_origin_z);
   P.args.img_y0 = (float)(-p.pixel * p.n_y / 2 + p.image_origin_y);
+  P.args.tan_bins = 0;
+  P.args.tan_left = 0.0f;
+  P.args.tan_voxel = 1.0f;
   return 0;
+}
+
+// Tangent axis of the --tan-bins grid (phantom_cmd.cpp:246-250), in FP32:
+// max_tan = 1.1 L / (2 R), planes of max_tan / bins * 2 from -max_tan.
+void phantom_tan_axis(double radius, double length, int tan_bins,
+                      float* left, float* voxel) {
+  const float max_tan =
+      (float)1.1 * (float)length / (2.0f * (float)radius);
+  *left = -max_tan;
+  *voxel = max_tan / (float)tan_bins * 2.0f;
 }
 
 void phantom_release(PhantomDevice& P) {
   if (P.regions) (void)hipFree(P.regions);
   for (void* q : {(void*)P.flags, (void*)P.resp, (void*)P.exact,
-                  (void*)P.pix_origin, (void*)P.pix_err, (void*)P.sel_resp,
+                  (void*)P.pix_origin, (void*)P.pix_err, (void*)P.pix_tan,
+                  (void*)P.ev4, (void*)P.sel_ev4, (void*)P.sel_resp,
                   (void*)P.sel_exact, (void*)P.sel_index, (void*)P.index,
                   (void*)P.count, P.tmp})
     if (q) (void)hipFree(q);
@@ -309,8 +382,12 @@ void phantom_release(PhantomDevice& P) {
 int phantom_detect(PhantomDevice& P, int64_t need, int64_t max_emissions,
                    float* d_resp, float* d_exact, int* img_em, int* img_det,
                    int* img_err, int64_t* produced, hipStream_t stream,
-                   std::string* err) {
-  const bool images = img_em || img_det || img_err;
+                   std::string* err, float* d_events, int* img_tan) {
+  if (img_tan && P.args.tan_bins < 1) {
+    if (err) *err = "tangent-bin image without tangent bins";
+    return -1;
+  }
+  const bool images = img_em || img_det || img_err || img_tan;
   if (!P.flags) {
     const int64_t B = kPhantomBatch;
     PH_TRY(hipMalloc(&P.flags, B));
@@ -330,8 +407,18 @@ int phantom_detect(PhantomDevice& P, int64_t need, int64_t max_emissions,
     PH_TRY(hipcub::DeviceSelect::Flagged(nullptr, b2, P.index, P.flags,
                                          P.sel_index, P.count, (int)B,
                                          stream));
-    P.tmp_bytes = std::max(b1, b2);
+    size_t b3 = 0;
+    PH_TRY(hipcub::DeviceSelect::Flagged(
+        nullptr, b3, reinterpret_cast<Quad*>(P.resp), P.flags,
+        reinterpret_cast<Quad*>(P.sel_resp), P.count, (int)B, stream));
+    P.tmp_bytes = std::max(std::max(b1, b2), b3);
     PH_TRY(hipMalloc(&P.tmp, P.tmp_bytes));
+  }
+  if (img_tan && !P.pix_tan)
+    PH_TRY(hipMalloc(&P.pix_tan, sizeof(int) * kPhantomBatch));
+  if (d_events && !P.ev4) {
+    PH_TRY(hipMalloc(&P.ev4, sizeof(float) * 4 * kPhantomBatch));
+    PH_TRY(hipMalloc(&P.sel_ev4, sizeof(float) * 4 * kPhantomBatch));
   }
   int64_t done = 0;
   int64_t emitted = 0;
@@ -344,7 +431,9 @@ int phantom_detect(PhantomDevice& P, int64_t need, int64_t max_emissions,
                        P.args, first, count, P.flags, P.resp,
                        d_exact ? P.exact : nullptr,
                        images ? P.pix_origin : nullptr,
-                       images ? P.pix_err : nullptr);
+                       images ? P.pix_err : nullptr,
+                       d_events ? P.ev4 : nullptr,
+                       img_tan ? P.pix_tan : nullptr);
     PH_TRY(hipGetLastError());
     // emission numbers of the batch, compacted with the responses
     hipLaunchKernelGGL(iota_kernel, dim3(blocks), dim3(256), 0, stream,
@@ -386,11 +475,22 @@ int phantom_detect(PhantomDevice& P, int64_t need, int64_t max_emissions,
                               sizeof(float) * 3 * take,
                               hipMemcpyDeviceToDevice, stream));
       }
+      if (d_events) {
+        bytes = P.tmp_bytes;
+        PH_TRY(hipcub::DeviceSelect::Flagged(
+            P.tmp, bytes, reinterpret_cast<Quad*>(P.ev4), P.flags,
+            reinterpret_cast<Quad*>(P.sel_ev4), P.count, (int)count,
+            stream));
+        PH_TRY(hipMemcpyAsync(d_events + 4 * done, P.sel_ev4,
+                              sizeof(float) * 4 * take,
+                              hipMemcpyDeviceToDevice, stream));
+      }
     }
     if (images) {
       hipLaunchKernelGGL(image_kernel, dim3((unsigned)((used + 255) / 256)),
                          dim3(256), 0, stream, used, P.flags, P.pix_origin,
-                         P.pix_err, img_em, img_det, img_err);
+                         P.pix_err, P.pix_tan, img_em, img_det, img_err,
+                         img_tan);
       PH_TRY(hipGetLastError());
     }
     PH_TRY(hipStreamSynchronize(stream));
@@ -432,18 +532,20 @@ int strip_phantom_random_blocks(int device, uint64_t key0, uint64_t key1,
   return STRIP_MLEM_OK;
 }
 
-int strip_phantom_generate(int device, const strip_phantom_params* params,
-                           const strip_phantom_region* regions, int n_regions,
-                           int64_t n, float* responses, float* exact,
-                           int64_t* n_detected, int64_t* n_emitted,
-                           int* image_emitted, int* image_detected,
-                           int* image_detected_w_error) {
+int strip_phantom_generate_full(int device, const strip_phantom_params* params,
+                                const strip_phantom_region* regions,
+                                int n_regions, int64_t n,
+                                strip_phantom_outputs* out) {
   using namespace strip;
   g_phantom_error.clear();
-  if (!params || n < 0 || (n > 0 && !responses))
+  if (!params || !out || n < 0 || (n > 0 && !out->responses))
     return phantom_fail(STRIP_MLEM_ERR_ARGUMENT, "bad arguments");
-  const bool images =
-      image_emitted || image_detected || image_detected_w_error;
+  int* image_tan = out->tan_bins > 0 ? out->image_tan_bins : nullptr;
+  if (out->tan_bins < 0 || (out->tan_bins > 0 && !out->image_tan_bins))
+    return phantom_fail(STRIP_MLEM_ERR_ARGUMENT,
+                        "tan_bins > 0 needs image_tan_bins");
+  const bool images = out->image_emitted || out->image_detected ||
+                      out->image_detected_w_error || image_tan;
   if (images && (params->n_z <= 0 || params->n_y <= 0 || !(params->pixel > 0)))
     return phantom_fail(STRIP_MLEM_ERR_ARGUMENT,
                         "images need n_z, n_y and pixel > 0");
@@ -454,8 +556,11 @@ int strip_phantom_generate(int device, const strip_phantom_params* params,
   hipStream_t stream = nullptr;
   float* d_resp = nullptr;
   float* d_exact = nullptr;
+  float* d_events = nullptr;
   int* d_img = nullptr;
   const int64_t np = images ? (int64_t)params->n_z * params->n_y : 0;
+  const int64_t nt = image_tan ? np * out->tan_bins : 0;
+  const int64_t n1 = std::max<int64_t>(n, 1);
   int rc = STRIP_MLEM_OK;
   int64_t produced = 0;
   do {
@@ -465,15 +570,20 @@ int strip_phantom_generate(int device, const strip_phantom_params* params,
       rc = STRIP_MLEM_ERR_ARGUMENT;
       break;
     }
+    if (image_tan) {
+      P.args.tan_bins = out->tan_bins;
+      phantom_tan_axis(params->radius, params->scintillator_length,
+                       out->tan_bins, &P.args.tan_left, &P.args.tan_voxel);
+    }
     if (hipStreamCreate(&stream) != hipSuccess ||
-        hipMalloc(&d_resp, sizeof(float) * 3 * std::max<int64_t>(n, 1)) !=
-            hipSuccess ||
-        (exact && hipMalloc(&d_exact, sizeof(float) * 3 *
-                                          std::max<int64_t>(n, 1)) !=
-                      hipSuccess) ||
+        hipMalloc(&d_resp, sizeof(float) * 3 * n1) != hipSuccess ||
+        (out->exact &&
+         hipMalloc(&d_exact, sizeof(float) * 3 * n1) != hipSuccess) ||
+        (out->events &&
+         hipMalloc(&d_events, sizeof(float) * 4 * n1) != hipSuccess) ||
         (images &&
-         (hipMalloc(&d_img, sizeof(int) * 3 * np) != hipSuccess ||
-          hipMemsetAsync(d_img, 0, sizeof(int) * 3 * np, stream) !=
+         (hipMalloc(&d_img, sizeof(int) * (3 * np + nt)) != hipSuccess ||
+          hipMemsetAsync(d_img, 0, sizeof(int) * (3 * np + nt), stream) !=
               hipSuccess))) {
       err = "device allocation failed";
       rc = STRIP_MLEM_ERR_ALLOC;
@@ -481,38 +591,127 @@ int strip_phantom_generate(int device, const strip_phantom_params* params,
     }
     const int64_t max_em = params->only_detected ? INT64_MAX : n;
     if (phantom_detect(P, n, max_em, d_resp, d_exact,
-                       image_emitted ? d_img : nullptr,
-                       image_detected ? d_img + np : nullptr,
-                       image_detected_w_error ? d_img + 2 * np : nullptr,
-                       &produced, stream, &err)) {
+                       out->image_emitted ? d_img : nullptr,
+                       out->image_detected ? d_img + np : nullptr,
+                       out->image_detected_w_error ? d_img + 2 * np : nullptr,
+                       &produced, stream, &err, d_events,
+                       image_tan ? d_img + 3 * np : nullptr)) {
       rc = STRIP_MLEM_ERR_HIP;
       break;
     }
     if (produced > 0 &&
-        (hipMemcpy(responses, d_resp, sizeof(float) * 3 * produced,
+        (hipMemcpy(out->responses, d_resp, sizeof(float) * 3 * produced,
                    hipMemcpyDeviceToHost) != hipSuccess ||
-         (exact && hipMemcpy(exact, d_exact, sizeof(float) * 3 * produced,
-                             hipMemcpyDeviceToHost) != hipSuccess))) {
+         (out->exact &&
+          hipMemcpy(out->exact, d_exact, sizeof(float) * 3 * produced,
+                    hipMemcpyDeviceToHost) != hipSuccess) ||
+         (out->events &&
+          hipMemcpy(out->events, d_events, sizeof(float) * 4 * produced,
+                    hipMemcpyDeviceToHost) != hipSuccess))) {
       err = "copy to host failed";
       rc = STRIP_MLEM_ERR_HIP;
       break;
     }
-    int* outs[3] = {image_emitted, image_detected, image_detected_w_error};
+    int* outs[3] = {out->image_emitted, out->image_detected,
+                    out->image_detected_w_error};
     for (int k = 0; k < 3 && images; ++k)
       if (outs[k] && hipMemcpy(outs[k], d_img + k * np, sizeof(int) * np,
                                hipMemcpyDeviceToHost) != hipSuccess) {
         err = "copy to host failed";
         rc = STRIP_MLEM_ERR_HIP;
       }
+    if (image_tan && hipMemcpy(image_tan, d_img + 3 * np, sizeof(int) * nt,
+                               hipMemcpyDeviceToHost) != hipSuccess) {
+      err = "copy to host failed";
+      rc = STRIP_MLEM_ERR_HIP;
+    }
   } while (false);
-  if (n_detected) *n_detected = produced;
-  if (n_emitted) *n_emitted = P.emitted;
+  out->n_detected = produced;
+  out->n_emitted = P.emitted;
   phantom_release(P);
   if (d_resp) (void)hipFree(d_resp);
   if (d_exact) (void)hipFree(d_exact);
+  if (d_events) (void)hipFree(d_events);
   if (d_img) (void)hipFree(d_img);
   if (stream) (void)hipStreamDestroy(stream);
   if (rc) return phantom_fail(rc, err);
+  return STRIP_MLEM_OK;
+}
+
+int strip_phantom_generate(int device, const strip_phantom_params* params,
+                           const strip_phantom_region* regions, int n_regions,
+                           int64_t n, float* responses, float* exact,
+                           int64_t* n_detected, int64_t* n_emitted,
+                           int* image_emitted, int* image_detected,
+                           int* image_detected_w_error) {
+  strip_phantom_outputs out{};
+  out.responses = responses;
+  out.exact = exact;
+  out.image_emitted = image_emitted;
+  out.image_detected = image_detected;
+  out.image_detected_w_error = image_detected_w_error;
+  const int rc = strip_phantom_generate_full(device, params, regions,
+                                             n_regions, n, &out);
+  if (n_detected) *n_detected = out.n_detected;
+  if (n_emitted) *n_emitted = out.n_emitted;
+  return rc;
+}
+
+int strip_phantom_tan_kernel_map(int device, const strip_phantom_params* params,
+                                 int tan_bins, double kernel_z,
+                                 double kernel_y, float* out) {
+  using namespace strip;
+  g_phantom_error.clear();
+  if (!params || !out || tan_bins < 1 || params->n_z <= 0 ||
+      params->n_y <= 0 || !(params->pixel > 0))
+    return phantom_fail(STRIP_MLEM_ERR_ARGUMENT, "bad arguments");
+  if (hipSetDevice(device) != hipSuccess)
+    return phantom_fail(STRIP_MLEM_ERR_HIP, "hipSetDevice failed");
+  strip_mlem_geometry geo{};
+  geo.radius = params->radius;
+  geo.scintillator_length = params->scintillator_length;
+  geo.n_z_pixels = params->n_z;
+  geo.n_y_pixels = params->n_y;
+  geo.pixel_width = params->pixel;
+  geo.pixel_height = params->pixel;
+  geo.sigma_z = params->sigma_z;
+  geo.sigma_dl = params->sigma_dl;
+  const DeviceGeometry g = make_device_geometry(geo);
+  float left = 0.f, voxel = 1.f;
+  phantom_tan_axis(params->radius, params->scintillator_length, tan_bins,
+                   &left, &voxel);
+  const int64_t n = (int64_t)params->n_z * params->n_y * tan_bins;
+  float* d = nullptr;
+  if (hipMalloc(&d, sizeof(float) * (n + 1)) != hipSuccess)
+    return phantom_fail(STRIP_MLEM_ERR_ALLOC, "device allocation failed");
+  // only detected emissions: normalised by the kernel point's sensitivity
+  float inv_sens = 1.0f;
+  hipError_t e = hipSuccess;
+  if (params->only_detected) {
+    hipLaunchKernelGGL(sensitivity_at_kernel, dim3(1), dim3(64), 0, 0, g,
+                       (float)kernel_z, (float)kernel_y, d + n);
+    float s = 0.f;
+    e = hipMemcpy(&s, d + n, sizeof(float), hipMemcpyDeviceToHost);
+    inv_sens = 1.0f / s;
+  }
+  if (e == hipSuccess) {
+    const float pixel = (float)params->pixel;
+    const float z0 = (float)(-params->pixel * params->n_z / 2 +
+                             params->image_origin_z);
+    const float y0 = (float)(-params->pixel * params->n_y / 2 +
+                             params->image_origin_y);
+    hipLaunchKernelGGL(tan_kernel_map_kernel,
+                       dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, g,
+                       params->n_z, params->n_y, tan_bins, pixel, z0, y0,
+                       left, voxel, (float)kernel_z, (float)kernel_y, inv_sens,
+                       d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpy(out, d, sizeof(float) * n, hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  if (e != hipSuccess)
+    return phantom_fail(STRIP_MLEM_ERR_HIP, hipGetErrorString(e));
   return STRIP_MLEM_OK;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <string>
 
+#include "strip_device.hpp"
 #include "strip_phantom.h"
 
 namespace strip {
@@ -29,6 +30,10 @@ struct PhantomKernelArgs {
   float radius, half_length, sigma_z, sigma_dl;
   int img_nz, img_ny;
   float img_pixel, img_z0, img_y0;
+  // tangent bins of the points reconstructed with error (0: none): plane =
+  // floor((tan - tan_left) / tan_voxel)
+  int tan_bins;
+  float tan_left, tan_voxel;
 };
 
 struct PhantomDevice {
@@ -42,6 +47,9 @@ struct PhantomDevice {
   float* exact = nullptr;
   int* pix_origin = nullptr;
   int* pix_err = nullptr;
+  int* pix_tan = nullptr;
+  float* ev4 = nullptr;      // emission point and direction per emission
+  float* sel_ev4 = nullptr;
   float* sel_resp = nullptr;
   float* sel_exact = nullptr;
   int64_t* sel_index = nullptr;
@@ -50,6 +58,13 @@ struct PhantomDevice {
   void* tmp = nullptr;
   size_t tmp_bytes = 0;
 };
+
+// FP32 Scanner and kernel constants of a geometry (strip_mlem.hip).
+DeviceGeometry make_device_geometry(const strip_mlem_geometry& geo);
+
+// Tangent axis of the --tan-bins grid: first plane's lower edge and width.
+void phantom_tan_axis(double radius, double length, int tan_bins,
+                      float* left, float* voxel);
 
 // Regions to the device; the scanner values given here (not those in p) are
 // the ones used, so a context can impose its own.
@@ -61,11 +76,14 @@ void phantom_release(PhantomDevice& P);
 
 // Emissions from P.next on, until `need` detected responses are produced or
 // max_emissions emissions are used; responses (AoS triples, emission order)
-// go to d_resp (and d_exact without error); images (device, may be null)
-// count the emissions used.
+// go to d_resp (and d_exact without error, d_events the emission point and
+// direction, 4 floats each); images (device, may be null) count the
+// emissions used, img_tan (tan_bins planes of n_y x n_z) the detected ones by
+// the point and tangent reconstructed from the response with error.
 int phantom_detect(PhantomDevice& P, int64_t need, int64_t max_emissions,
                    float* d_resp, float* d_exact, int* img_em, int* img_det,
                    int* img_err, int64_t* produced, hipStream_t stream,
-                   std::string* err);
+                   std::string* err, float* d_events = nullptr,
+                   int* img_tan = nullptr);
 
 }  // namespace strip

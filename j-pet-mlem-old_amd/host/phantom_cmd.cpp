@@ -13,10 +13,24 @@
 //                            int32 images of emission points, detected
 //                            emission points and points reconstructed from
 //                            the responses with error (pixel_grid.h:59-69)
+//   <base>_events<ext>       (--full) emission point and direction of each
+//                            detected event ("z y dz dy"; binary: 4 floats)
+//   <base>_full_response<ext>
+//                            (--full) the exact responses
+//   <base>_tan_bins(.nrrd)   (--tan-bins N) int32 counts of the detected
+//                            events by the point and tangent reconstructed
+//                            from their responses with error, N tangent
+//                            planes of the image grid
+//   <base>_tan_kernel(.nrrd) (--tan-bins N) float map of the analytic kernel
+//                            at --kernel-point (default 0,0) for an event in
+//                            each bin, times the bin's volume (and divided by
+//                            the kernel point's sensitivity with --detected)
 // -e counts emissions, or detected events with --detected.  Additions:
 // --seed (the generator's stream is Philox4x64-10 keyed by it) and
-// --device.  Not available: --tan-bins / --kernel-point maps and the
-// --full _events / _full_response files.
+// --device.  One error draw per detected event serves both the written
+// response and the _naive / _tan_bins images (the reference draws the
+// written response's error a second time, phantom_cmd.cpp:300-331; the
+// distributions are the same).
 #include <cmath>
 #include <fstream>
 #include <iostream>
@@ -47,8 +61,9 @@ void add_options(strip_cli::Parser& cl) {
   cl.add("scale", 0, Kind::Double, "1", "Scale phantom with given constant");
   cl.add("additive", 0, Kind::Flag, "",
          "phantom regions are additive, not disjunctive");
-  cl.add("tan-bins", 0, Kind::Int, "", "tangent bins (not available)", true);
-  cl.add("kernel-point", 0, Kind::Doubles, "", "kernel point (not available)",
+  cl.add("tan-bins", 0, Kind::Int, "",
+         "Bin reconstructed events using given number of tangent bins", true);
+  cl.add("kernel-point", 0, Kind::Doubles, "", "Point to evaluate kernel in",
          true);
   cl.add("full", 0, Kind::Flag, "", "Emit additional full information", true);
   cl.add("no-responses", 0, Kind::Flag, "", "Do not emit responses", true);
@@ -93,24 +108,34 @@ void read_phantom(const std::string& fn, double scale,
   }
 }
 
+// Raw map plus its NRRD header; the header names the raw file as the
+// reference's writer gets it, "./" + the output base with its directories
+// (phantom_cmd.cpp:374-386,388-391).
+// depth 0: a 2D image; depth >= 1: a 3D voxel map of `depth` planes.
+template <typename T>
+void write_map(const std::string& raw, const char* type, const T* data,
+               int n_z, int n_y, int depth) {
+  const size_t n = (size_t)n_z * n_y * (depth > 0 ? depth : 1);
+  {
+    std::ofstream out(raw, std::ios::binary);
+    out.write(reinterpret_cast<const char*>(data), sizeof(T) * n);
+  }
+  std::ofstream nrrd(raw + ".nrrd");
+  nrrd << "NRRD0001" << std::endl
+       << "type: " << type << std::endl
+       << "endian: little" << std::endl
+       << "dimension: " << (depth > 0 ? 3 : 2) << std::endl
+       << "sizes: " << n_z << " " << n_y;
+  if (depth > 0) nrrd << " " << depth;
+  nrrd << std::endl
+       << "data file: ./" << raw << std::endl
+       << "encoding: raw" << std::endl;
+}
+
 void write_int_image(const std::string& base, const std::string& name,
                      const std::vector<int>& img, int n_z, int n_y) {
   const std::string raw = base + name;
-  {
-    std::ofstream out(raw, std::ios::binary);
-    out.write(reinterpret_cast<const char*>(img.data()),
-              sizeof(int) * img.size());
-  }
-  {
-    std::ofstream nrrd(raw + ".nrrd");
-    nrrd << "NRRD0001" << std::endl
-         << "type: int" << std::endl
-         << "endian: little" << std::endl
-         << "dimension: 2" << std::endl
-         << "sizes: " << n_z << " " << n_y << std::endl
-         << "data file: ./" << strip_cli::path::wo_path(raw) << std::endl
-         << "encoding: raw" << std::endl;
-  }
+  write_map(raw, "int", img.data(), n_z, n_y, 0);
   std::vector<float> f(img.begin(), img.end());
   strip_io::write_png(raw + ".png", f.data(), n_z, n_y);
 }
@@ -127,8 +152,16 @@ int main(int argc, char** argv) {
       throw std::runtime_error(
           "at least one input phantom description expected, consult --help");
     }
-    if (cl.exist("tan-bins") || cl.exist("kernel-point"))
-      throw std::runtime_error("--tan-bins / --kernel-point are not available");
+    const int tan_bins = cl.exist("tan-bins") ? (int)cl.get_int("tan-bins") : 0;
+    double kernel_z = 0, kernel_y = 0;
+    if (cl.exist("kernel-point")) {
+      const std::vector<double> k = cl.get_doubles("kernel-point");
+      if (k.size() != 2)
+        throw std::runtime_error(
+            "--kernel-point must supply exactly 2 values for x, y");
+      kernel_z = k[0];
+      kernel_y = k[1];
+    }
     // calculate_scanner_options (options.cpp:104-124)
     const std::vector<double> radius = cl.get_doubles("radius");
     if (radius.empty()) throw std::runtime_error("at least one --radius has to be specified");
@@ -182,16 +215,26 @@ int main(int argc, char** argv) {
                 << "    pixel size: " << s_pixel << " x " << s_pixel
                 << std::endl;
 
-    std::vector<float> resp(3 * (size_t)std::max<int64_t>(n, 1));
-    std::vector<float> exact(full ? resp.size() : 0);
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1);
+    std::vector<float> resp(3 * n1);
+    std::vector<float> exact(full ? 3 * n1 : 0), events(full ? 4 * n1 : 0);
     const size_t np = (size_t)n_z * n_y;
     std::vector<int> emitted(np), detected(np), naive(np);
-    int64_t n_det = 0, n_em = 0;
-    const int rc = strip_phantom_generate(
-        (int)cl.get_int("device"), &params, regions.data(),
-        (int)regions.size(), n, resp.data(), full ? exact.data() : nullptr,
-        &n_det, &n_em, emitted.data(), detected.data(), naive.data());
+    std::vector<int> tan_map(tan_bins > 0 ? np * tan_bins : 0);
+    strip_phantom_outputs outs{};
+    outs.responses = resp.data();
+    outs.exact = full ? exact.data() : nullptr;
+    outs.events = full ? events.data() : nullptr;
+    outs.image_emitted = emitted.data();
+    outs.image_detected = detected.data();
+    outs.image_detected_w_error = naive.data();
+    outs.tan_bins = tan_bins > 0 ? tan_bins : 0;
+    outs.image_tan_bins = tan_bins > 0 ? tan_map.data() : nullptr;
+    const int device = (int)cl.get_int("device");
+    int rc = strip_phantom_generate_full(device, &params, regions.data(),
+                                         (int)regions.size(), n, &outs);
     if (rc) throw std::runtime_error(strip_phantom_last_error());
+    const int64_t n_det = outs.n_detected, n_em = outs.n_emitted;
     if (verbose)
       std::cerr << " emitted: " << n_em << " events" << std::endl
                 << "detected: " << n_det << " events" << std::endl;
@@ -200,18 +243,26 @@ int main(int argc, char** argv) {
     const std::string base = strip_cli::path::wo_ext(output);
     const std::string ext = strip_cli::path::ext(output);
     if (!base.empty() && !cl.exist("no-responses")) {
-      auto write = [&](const std::string& fn, const std::vector<float>& v) {
+      // k floats per detected event: "a b c[ d]" lines or raw floats
+      auto write = [&](const std::string& fn, const std::vector<float>& v,
+                       int k) {
         if (ext == ".txt") {
           std::ofstream out(fn);
-          for (int64_t i = 0; i < n_det; ++i)
-            out << v[3 * i] << ' ' << v[3 * i + 1] << ' ' << v[3 * i + 2]
-                << "\n";
+          for (int64_t i = 0; i < n_det; ++i) {
+            for (int c = 0; c < k; ++c)
+              out << (c ? " " : "") << v[(size_t)k * i + c];
+            out << "\n";
+          }
         } else {
-          strip_io::write_raw(fn, v.data(), 3 * (size_t)n_det);
+          strip_io::write_raw(fn, v.data(), (size_t)k * n_det);
         }
       };
-      write(output, resp);
-      if (full) write(base + "_wo_error" + ext, exact);
+      write(output, resp, 3);
+      if (full) {
+        write(base + "_wo_error" + ext, exact, 3);
+        write(base + "_events" + ext, events, 4);
+        write(base + "_full_response" + ext, exact, 3);
+      }
     }
     if (!base.empty()) {
       // <base>.cfg: the scanner options for the reconstruction tool
@@ -226,6 +277,17 @@ int main(int argc, char** argv) {
       write_int_image(base, "_detected", detected, n_z, n_y);
       write_int_image(base, "_emitted", emitted, n_z, n_y);
       write_int_image(base, "_naive", naive, n_z, n_y);
+      if (tan_bins > 0) {
+        write_map(base + "_tan_bins", "int", tan_map.data(), n_z, n_y,
+                  tan_bins);
+        if (verbose) std::cerr << "generating kernel map..." << std::endl;
+        std::vector<float> kmap(np * tan_bins);
+        rc = strip_phantom_tan_kernel_map(device, &params, tan_bins, kernel_z,
+                                          kernel_y, kmap.data());
+        if (rc) throw std::runtime_error(strip_phantom_last_error());
+        write_map(base + "_tan_kernel", "float", kmap.data(), n_z, n_y,
+                  tan_bins);
+      }
     }
     return 0;
   } catch (const std::exception& ex) {

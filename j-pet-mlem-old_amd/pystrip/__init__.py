@@ -92,6 +92,20 @@ class PhantomParams(ctypes.Structure):
                 ("image_origin_y", ctypes.c_double)]
 
 
+class PhantomOutputs(ctypes.Structure):
+    """strip_phantom_outputs (include/strip_phantom.h)."""
+
+    _fields_ = [("responses", ctypes.c_void_p), ("exact", ctypes.c_void_p),
+                ("events", ctypes.c_void_p),
+                ("image_emitted", ctypes.c_void_p),
+                ("image_detected", ctypes.c_void_p),
+                ("image_detected_w_error", ctypes.c_void_p),
+                ("tan_bins", ctypes.c_int),
+                ("image_tan_bins", ctypes.c_void_p),
+                ("n_detected", ctypes.c_int64),
+                ("n_emitted", ctypes.c_int64)]
+
+
 # Reference CPU path (graded parity mode), reconstruction.h:220-298
 MODE_CPU = (0, 0, 0, 1)
 # Legacy CUDA warp kernel, cuda/reconstruction/warp_granularity.cuh:88-152
@@ -151,6 +165,13 @@ def load_library():
             ctypes.POINTER(PhantomRegion), ctypes.c_int, i64, fptr, fptr,
             ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.c_void_p,
             ctypes.c_void_p, ctypes.c_void_p]),
+        "strip_phantom_generate_full": (ctypes.c_int, [
+            ctypes.c_int, ctypes.POINTER(PhantomParams),
+            ctypes.POINTER(PhantomRegion), ctypes.c_int, i64,
+            ctypes.POINTER(PhantomOutputs)]),
+        "strip_phantom_tan_kernel_map": (ctypes.c_int, [
+            ctypes.c_int, ctypes.POINTER(PhantomParams), ctypes.c_int,
+            ctypes.c_double, ctypes.c_double, fptr]),
         "strip_mlem_generate_events": (ctypes.c_int, [
             P, ctypes.POINTER(PhantomParams), ctypes.POINTER(PhantomRegion),
             ctypes.c_int, i64]),
@@ -261,6 +282,7 @@ def exported_symbols():
         "strip_mlem_batch_last_ms", "strip_mlem_batch_autotune",
         "strip_mlem_reserve_events", "strip_mlem_append_events",
         "strip_phantom_generate", "strip_mlem_generate_events",
+        "strip_phantom_generate_full", "strip_phantom_tan_kernel_map",
         "strip_phantom_random_blocks", "strip_phantom_last_error",
         "strip_mlem_set_subsets", "strip_mlem_image_total",
         "strip_mlem_set_plan_events",
@@ -770,6 +792,74 @@ def generate_events(regions, radius, length, sigma_z, sigma_dl, n, seed=0,
     m = nd.value
     return {"responses": resp[:m], "exact": None if ex is None else ex[:m],
             "n_emitted": ne.value, "images": imgs}
+
+
+def _phantom_params(regions, radius, length, sigma_z, sigma_dl, seed,
+                    additive, only_detected, image_grid):
+    p = PhantomParams(float(radius), float(length), float(sigma_z),
+                      float(sigma_dl), int(seed), int(bool(additive)),
+                      int(bool(only_detected)), 0, 0, 0.0, 0.0, 0.0)
+    if image_grid is not None:
+        g = list(image_grid) + [0.0] * (5 - len(image_grid))
+        p.n_z, p.n_y, p.pixel = int(g[0]), int(g[1]), float(g[2])
+        p.image_origin_z, p.image_origin_y = float(g[3]), float(g[4])
+    return p
+
+
+def generate_full(regions, radius, length, sigma_z, sigma_dl, n, image_grid,
+                  seed=0, additive=False, only_detected=True, tan_bins=0,
+                  device=0):
+    """Every output of the phantom tool (strip_phantom_generate_full):
+    dict(responses, exact, events (m, 4): emission point and direction,
+    images (emitted, detected, detected_w_error), tan_bins (tan_bins, n_y,
+    n_z) int32 or None, n_emitted)."""
+    lib = load_library()
+    arr = _regions_array(regions)
+    p = _phantom_params(regions, radius, length, sigma_z, sigma_dl, seed,
+                        additive, only_detected, image_grid)
+    m1 = max(int(n), 1)
+    resp = np.empty((m1, 3), np.float32)
+    ex = np.empty((m1, 3), np.float32)
+    evs = np.empty((m1, 4), np.float32)
+    imgs = [np.zeros((p.n_y, p.n_z), np.int32) for _ in range(3)]
+    tan = (np.zeros((tan_bins, p.n_y, p.n_z), np.int32) if tan_bins > 0
+           else None)
+    o = PhantomOutputs()
+    vp = ctypes.c_void_p
+    o.responses = resp.ctypes.data_as(vp)
+    o.exact = ex.ctypes.data_as(vp)
+    o.events = evs.ctypes.data_as(vp)
+    o.image_emitted = imgs[0].ctypes.data_as(vp)
+    o.image_detected = imgs[1].ctypes.data_as(vp)
+    o.image_detected_w_error = imgs[2].ctypes.data_as(vp)
+    o.tan_bins = int(tan_bins)
+    o.image_tan_bins = tan.ctypes.data_as(vp) if tan is not None else None
+    rc = lib.strip_phantom_generate_full(device, ctypes.byref(p), arr,
+                                         len(arr), int(n), ctypes.byref(o))
+    if rc != 0:
+        raise StripMLEMError(
+            f"status {rc}: {lib.strip_phantom_last_error().decode()}")
+    m = o.n_detected
+    return {"responses": resp[:m], "exact": ex[:m], "events": evs[:m],
+            "images": imgs, "tan_bins": tan, "n_emitted": o.n_emitted}
+
+
+def tan_kernel_map(radius, length, sigma_z, sigma_dl, image_grid, tan_bins,
+                   kernel_point=(0.0, 0.0), only_detected=True, device=0):
+    """The phantom tool's --tan-bins kernel map (tan_bins, n_y, n_z) float32
+    (strip_phantom_tan_kernel_map)."""
+    lib = load_library()
+    p = _phantom_params([], radius, length, sigma_z, sigma_dl, 0, False,
+                        only_detected, image_grid)
+    out = np.empty((tan_bins, p.n_y, p.n_z), np.float32)
+    rc = lib.strip_phantom_tan_kernel_map(device, ctypes.byref(p),
+                                          int(tan_bins),
+                                          float(kernel_point[0]),
+                                          float(kernel_point[1]), _fptr(out))
+    if rc != 0:
+        raise StripMLEMError(
+            f"status {rc}: {lib.strip_phantom_last_error().decode()}")
+    return out
 
 
 def random_blocks(key0, key1, first, c1, c2, n_blocks, device=0):

@@ -11,14 +11,23 @@
 //   PET2D::Strip::Scanner<F, short>                      scanner.h:26-291
 //   PET2D::Strip::GaussianKernel<F>, AnalyticKernel<F>   gaussian_kernel.h, analytic_kernel.h
 //   util::text_parser (fast .txt event loading)          util/text_parser.h:29-175
+//   PET2D::PixelMap text/binary streams, util::nrrd_writer,
+//   util::obstream / ibstream (image outputs and --rho inputs)
+//                                                        pixel_map.h:113-157, nrrd_writer.h, bstream.h
 #include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <string>
 #include <vector>
 
 #include "2d/strip/reconstruction.h"
 #include "2d/strip/gaussian_kernel.h"
 #include "2d/strip/analytic_kernel.h"
 #include "2d/strip/scanner.h"
+#include "2d/geometry/pixel_map.h"
+#include "common/types.h"
+#include "util/bstream.h"
+#include "util/nrrd_writer.h"
 
 #include "strip_oracle.h"  // oracle_geometry only (plain struct)
 
@@ -161,6 +170,77 @@ int64_t ref_load_txt_events(const char* fn, int is_3d, float* out,
     out[3 * e + 2] = rec.responses[e].dl;
   }
   return n;
+}
+
+// The image files 2d_strip_reconstruction writes, through the reference's
+// own writers, into directory `dir`:
+//   matrix.txt  PixelMap text stream, the --gpu protocol's .txt
+//               (pixel_map.h:134-147, reconstruction_cmd.cpp:242-244)
+//   tuples.txt  Reconstruction::output_tuples with precision 12, fixed: the
+//               CPU protocol's .txt (reconstruction.h:203-213,
+//               reconstruction_cmd.cpp:281-285)
+//   image.bin   raw floats (pixel_map.h:154-157)
+//   raw.nrrd    NRRD header of image.bin (nrrd_writer.h:19-28)
+//   ascii.nrrd  NRRD header of matrix.txt, textual
+int ref_write_images(const oracle_geometry* g, const float* img,
+                     const char* dir) {
+  using Image = PET2D::PixelMap<PET2D::Pixel<short>, float>;
+  const short w = (short)g->n_z_pixels, h = (short)g->n_y_pixels;
+  Image map(w, h);
+  for (int i = 0; i < map.size; ++i) map[i] = img[i];
+  const std::string d(dir);
+  {
+    std::ofstream txt(d + "/matrix.txt");
+    txt << map;
+  }
+  {
+    using Kernel = PET2D::Strip::GaussianKernel<float>;
+    PET2D::Strip::Reconstruction<float, Kernel> rec(make_scanner<float>(g));
+    for (int i = 0; i < map.size; ++i) rec.rho[i] = img[i];
+    std::ofstream txt(d + "/tuples.txt");
+    txt.precision(12);
+    txt << std::fixed;
+    rec.output_tuples(txt);
+  }
+  {
+    util::obstream bin(d + "/image.bin");
+    bin << map;
+  }
+  {
+    util::nrrd_writer nrrd(d + "/raw.nrrd", "image.bin");
+    nrrd << map;
+  }
+  {
+    util::nrrd_writer nrrd(d + "/ascii.nrrd", "matrix.txt", true);
+    nrrd << map;
+  }
+  return 0;
+}
+
+// A --rho start image read as reconstruction_cmd.cpp:145-165 reads it into
+// the Reconstruction's rho (all ones, reconstruction.h:59): text through the
+// PixelMap text stream, binary floats through ibstream, or int32 counts
+// plus 1000 with --rho-is-int.
+int ref_read_start_image(const oracle_geometry* g, const char* fn, int is_txt,
+                         int is_int, float* out) {
+  using Image = PET2D::PixelMap<PET2D::Pixel<short>, float>;
+  const short w = (short)g->n_z_pixels, h = (short)g->n_y_pixels;
+  Image rho(w, h, 1.0f);
+  if (is_txt) {
+    std::ifstream txt(fn);
+    txt >> rho;
+  } else {
+    util::ibstream bin(fn);
+    if (is_int) {
+      PET2D::PixelMap<PET2D::Pixel<short>, Hit> int_rho(w, h);
+      bin >> int_rho;
+      for (int i = 0; i < int_rho.size; ++i) rho[i] = int_rho[i] + 1000;
+    } else {
+      bin >> rho;
+    }
+  }
+  for (int i = 0; i < rho.size; ++i) out[i] = rho[i];
+  return 0;
 }
 
 }  // extern "C"

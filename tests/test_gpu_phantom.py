@@ -13,6 +13,7 @@ generator is pinned by:
     proportional to intensity x area, and agreement in moments with the
     numpy generator of pystrip/phantom.py.
 """
+import ctypes
 import glob
 import os
 import subprocess
@@ -153,3 +154,107 @@ def test_command_line_tools_end_to_end(tmp_path, oracle_mod, phantom_mod):
     e = oracle_mod.rel_l2(img, ref)
     print("CLI rel-L2", e)
     assert e <= 1e-4
+
+
+def test_full_outputs_and_tangent_bins(pystrip_mod, oracle_mod, phantom_mod):
+    # --full and --tan-bins outputs (phantom_cmd.cpp:210-227,300-431):
+    # each detected event's emission point and direction project exactly to
+    # its full response (to_projection_space_tan, scanner.h:107-113); the
+    # tangent-binned counts are the naive image split by the tangent of the
+    # response with error; the kernel map is the analytic kernel (pinned by
+    # the reference build's goldens) times the bin volume over the kernel
+    # point's sensitivity
+    print_case("phantom --full / --tan-bins outputs")
+    w = _phantom(phantom_mod)
+    bins = 6
+    grid = (w.n_z, w.n_y, w.pixel)
+    out = pystrip_mod.generate_full(w.phantom, w.radius, w.length, w.sigma_z,
+                                    w.sigma_dl, 300_000, grid, seed=11,
+                                    tan_bins=bins)
+    ev = out["events"].astype(np.float64)
+    ex = out["exact"].astype(np.float64)
+    assert len(ev) == 300_000
+    z, y, sx, cx = ev.T
+    assert np.allclose(sx * sx + cx * cx, 1.0, atol=1e-6)
+    t = sx / cx
+    R = w.radius
+    assert np.allclose(ex[:, 0], z + (R - y) * t, atol=2e-3)
+    assert np.allclose(ex[:, 1], z - (R + y) * t, atol=2e-3)
+    assert np.allclose(ex[:, 2], -2 * y * np.sqrt(t * t + 1), atol=2e-3)
+    tan_map = out["tan_bins"]
+    naive = out["images"][2]
+    assert tan_map.shape == (bins, w.n_y, w.n_z)
+    per_pixel = tan_map.sum(axis=0)
+    assert np.all(per_pixel <= naive)
+    # the binning of the responses with error, redone on the host: the
+    # device's FP32 division and square root may put an event on the other
+    # side of a bin edge, nothing more
+    r = out["responses"].astype(np.float32)
+    tt = (r[:, 0] - r[:, 1]) / np.float32(2 * R)
+    yy = np.float32(-0.5) * (r[:, 2] / np.sqrt(np.float32(1) + tt * tt))
+    zz = np.float32(0.5) * (r[:, 0] + r[:, 1] + np.float32(2) * yy * tt)
+    max_tan = np.float32(1.1) * np.float32(w.length) / np.float32(2 * R)
+    dt = max_tan / np.float32(bins) * np.float32(2)
+    ix = np.floor((zz + np.float32(w.n_z * w.pixel / 2)) / np.float32(w.pixel))
+    iy = np.floor((yy + np.float32(w.n_y * w.pixel / 2)) / np.float32(w.pixel))
+    it = np.floor((tt + max_tan) / dt)
+    ok = ((ix >= 0) & (ix < w.n_z) & (iy >= 0) & (iy < w.n_y) & (it >= 0) &
+          (it < bins))
+    host = np.zeros_like(tan_map)
+    np.add.at(host, (it[ok].astype(int), iy[ok].astype(int),
+                     ix[ok].astype(int)), 1)
+    assert abs(int(host.sum()) - int(tan_map.sum())) <= 5
+    assert np.abs(host - tan_map).sum() <= 10
+    # kernel map
+    kp = (12.0, -20.0)
+    km = pystrip_mod.tan_kernel_map(w.radius, w.length, w.sigma_z,
+                                    w.sigma_dl, grid, bins, kernel_point=kp)
+    L = oracle_mod.lib()
+    og = oracle_mod.geometry(*w.oracle_geometry_tuple())
+    sens = L.oracle_sensitivity_f64(ctypes.byref(og), kp[0], kp[1])
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        k, j, i = (int(rng.integers(bins)), int(rng.integers(w.n_y)),
+                   int(rng.integers(w.n_z)))
+        tan = (k + 0.5) * float(dt) - float(max_tan)
+        sec = np.sqrt(1 + tan * tan)
+        cz = -w.n_z * w.pixel / 2 + (i + 0.5) * w.pixel
+        cy = -w.n_y * w.pixel / 2 + (j + 0.5) * w.pixel
+        kv = L.oracle_analytic_kernel_f64(w.sigma_z, w.sigma_dl, cy, tan, sec,
+                                          R, kp[0] - cz, kp[1] - cy)
+        vol = 4 * R * w.pixel * w.pixel * float(dt) * sec
+        expect = kv * vol / sens
+        assert abs(km[k, j, i] - expect) <= 1e-4 * abs(expect) + 1e-30
+
+
+def test_phantom_command_full_and_tan_bins(tmp_path, phantom_mod):
+    print_case("2d_strip_phantom --full --tan-bins --kernel-point")
+    w = _phantom(phantom_mod)
+    desc = tmp_path / "three"
+    desc.write_text("".join(r.text() + "\n" for r in w.phantom))
+    out = tmp_path / "p.bin"
+    r = subprocess.run(
+        [os.path.join(BIN, "2d_strip_phantom"), str(desc), "-r",
+         str(w.radius), "-l", str(w.length), "-p", str(w.pixel),
+         "--n-z-pixels", str(w.n_z), "--n-y-pixels", str(w.n_y), "--s-z",
+         str(w.sigma_z), "--s-dl", str(w.sigma_dl), "-e", "5000",
+         "--detected", "--full", "--tan-bins", "4", "--kernel-point", "0",
+         "10", "-o", str(out), "--seed", "3"],
+        capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    resp = np.fromfile(out, np.float32).reshape(-1, 3)
+    wo = np.fromfile(tmp_path / "p_wo_error.bin", np.float32).reshape(-1, 3)
+    full = np.fromfile(tmp_path / "p_full_response.bin",
+                       np.float32).reshape(-1, 3)
+    evs = np.fromfile(tmp_path / "p_events.bin", np.float32).reshape(-1, 4)
+    assert len(resp) == len(wo) == len(full) == len(evs) == 5000
+    assert np.array_equal(wo, full)
+    tb = np.fromfile(tmp_path / "p_tan_bins", np.int32)
+    km = np.fromfile(tmp_path / "p_tan_kernel", np.float32)
+    assert tb.size == km.size == 4 * w.n_z * w.n_y
+    assert 0 < tb.sum() <= 5000 and np.all(np.isfinite(km)) and km.max() > 0
+    hdr = (tmp_path / "p_tan_bins.nrrd").read_text()
+    assert f"dimension: 3\nsizes: {w.n_z} {w.n_y} 4\n" in hdr
+    assert "type: int" in hdr
+    assert "type: float" in (tmp_path / "p_tan_kernel.nrrd").read_text()
