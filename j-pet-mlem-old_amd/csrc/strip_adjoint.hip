@@ -101,10 +101,14 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
       }
       const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
       const int col0 = s.lo + q;
-      float dz = __builtin_fmaf((float)col0, g.pixel_width,
-                                g.tl_z_half_w - v.cz);
-      const float qa = s.rowA * kExp2Scale;
-      const float qb = s.bdy * kExp2Scale;
+      // offset from the row's vertex (the pixel centre first, then the
+      // difference, as the reference rounds them), and the exponent's
+      // coefficients
+      float dz = (__builtin_fmaf((float)col0, g.pixel_width, g.tl_z_half_w) -
+                  v.cz) -
+                 s.tdy;
+      const float qa = s.qd * kExp2Scale;
+      const float qb = 0.0f;
       const float db = __builtin_fmaf(qc * step, step, qb * step);
       const float* src0 = win + (col0 - c0) * H + r;
       const float* src1 = src0 + n_words;
@@ -153,8 +157,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
       const float yp = dy + v.cy;
       const float w = v.Q0 * __builtin_fmaf(yp * yp, inv_r2, 1.0f) * s.pre *
                       s.pre;
-      const float qd = (v.A - v.B * v.B / (4.0f * g.C)) * dy * dy;
-      const float wq = w - qd;
+      const float wq = w - s.qd;
       const float R0 = s.pre * a0;
       const float m00 = wave_total(R0);
       const float m01 = wave_total(s.pre * aq0 * inv_scale);

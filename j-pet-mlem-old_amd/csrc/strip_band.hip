@@ -438,10 +438,14 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
         return;
       }
       const int col0 = s.lo + q;
-      float dz = __builtin_fmaf((float)col0, g.pixel_width,
-                                g.tl_z_half_w - v.cz);
-      const float qa = s.rowA * kExp2Scale;
-      const float qb = s.bdy * kExp2Scale;
+      // offset from the row's vertex (the pixel centre first, then the
+      // difference, as the reference rounds them), and the exponent's
+      // coefficients
+      float dz = (__builtin_fmaf((float)col0, g.pixel_width, g.tl_z_half_w) -
+                  v.cz) -
+                 s.tdy;
+      const float qa = s.qd * kExp2Scale;
+      const float qb = 0.0f;
       const float* src = win + (col0 - c0) * H + r;
       float acc = 0.0f;
       int k = 0;
@@ -592,10 +596,14 @@ __device__ __forceinline__ void backward_task(const BandInstance& d,
       if (!SPANS && G > 1 && (!valid || v.inv == 0.0f)) s.n = 0;
       const int nq = (s.n > q) ? (s.n - q + LR - 1) / LR : 0;
       const int col0 = s.lo + q;
-      float dz = __builtin_fmaf((float)col0, g.pixel_width,
-                                g.tl_z_half_w - v.cz);
-      const float qa = s.rowA * kExp2Scale;
-      const float qb = s.bdy * kExp2Scale;
+      // offset from the row's vertex (the pixel centre first, then the
+      // difference, as the reference rounds them), and the exponent's
+      // coefficients
+      float dz = (__builtin_fmaf((float)col0, g.pixel_width, g.tl_z_half_w) -
+                  v.cz) -
+                 s.tdy;
+      const float qa = s.qd * kExp2Scale;
+      const float qb = 0.0f;
       const float scale = s.pre * v.inv;
       float* w = win + (col0 - c0) * H + r;
       int k = 0;

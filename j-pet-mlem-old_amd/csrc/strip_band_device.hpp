@@ -77,7 +77,23 @@ struct RowSpan {
   float rowA;  // (A dy) dy
   float bdy;   // B dy
   float pre;   // c / sqrt(a . S^-1 . a) of this row
+  float tdy;   // tan dy: the row's Q is least at dz = tan dy
+  float qd;    // 4 id sec^2 dy^2: the row's least Q
 };
+
+// The kernel's exponent along a row in vertex form,
+//   Q(dz) = C (dz - tan dy)^2 + 4 id sec^2 dy^2,
+// the sum of squares b . S^-1 . b of gaussian_kernel.h:51-79 (C = 2 iz).
+// Both terms are non-negative, so FP32 keeps its relative precision where
+// the expanded form A dy^2 + B dy dz + C dz^2 cancels (thin, steep
+// ellipses).  tan = -B / (2C) and sec^2 = 1 + tan^2 from the event's B.
+__device__ inline void row_vertex(const DeviceGeometry& g, float B, float dy,
+                                  float& tdy, float& qd) {
+  const float t = -B * g.inv_2c;
+  const float ad = 4.0f * g.inv_pow_sigma_dl * __builtin_fmaf(t, t, 1.0f);
+  tdy = t * dy;
+  qd = ad * dy * dy;
+}
 
 __device__ inline RowSpan row_span(const DeviceGeometry& g, float cz,
                                    float cy, float A, float B, float P,
@@ -90,10 +106,13 @@ __device__ inline RowSpan row_span(const DeviceGeometry& g, float cz,
   s.rowA = 0.f;
   s.bdy = 0.f;
   s.pre = 0.f;
+  s.tdy = 0.f;
+  s.qd = 0.f;
   if (iy < y0 || iy >= y1) return s;
   const float py = pixel_center_y(g, iy);
   const float dy = py - cy;
   row_terms_exact(A, B, dy, s.rowA, s.bdy);
+  row_vertex(g, B, dy, s.tdy, s.qd);
   const float yp = dy + cy;
   s.pre = __builtin_amdgcn_rsqf(__builtin_fmaf(P, yp * yp, Q0));
   const float disc = s.bdy * s.bdy - 4.0f * C * (s.rowA - 9.0f);
@@ -147,6 +166,7 @@ __device__ inline RowSpan stored_span(const DeviceGeometry& g, uint32_t word,
   const float py = pixel_center_y(g, iy);
   const float dy = py - cy;
   row_terms_exact(A, B, dy, s.rowA, s.bdy);
+  row_vertex(g, B, dy, s.tdy, s.qd);
   const float yp = dy + cy;
   s.pre = __builtin_amdgcn_rsqf(__builtin_fmaf(P, yp * yp, Q0));
   return s;

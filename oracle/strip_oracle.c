@@ -12,7 +12,13 @@
  *   - the reference itself, compiled from its headers into oracle/_ref
  *     (ref_harness.cpp) where /root/reference is present;
  *   - closed forms and invariants: Q identity, mass conservation,
- *     single-event closed form, linearity, brute-force support enumeration.
+ *     single-event closed form, linearity, brute-force support enumeration;
+ *   - the legacy --gpu composition (warp_granularity.cuh:53-152, not
+ *     buildable here): its single-event closed form with the analytic kernel
+ *     and the 5-point sensitivity in the denominator, the NaN of its
+ *     unguarded 1/D, and the pixels its float-reciprocal row split visits;
+ *   - the FP64 differentiation checker: central finite differences
+ *     (tests/test_oracle_diff.py).
  */
 #define _GNU_SOURCE
 #include "strip_oracle.h"

@@ -238,8 +238,8 @@ def test_phantom_command_full_and_tan_bins(tmp_path, phantom_mod):
          str(w.radius), "-l", str(w.length), "-p", str(w.pixel),
          "--n-z-pixels", str(w.n_z), "--n-y-pixels", str(w.n_y), "--s-z",
          str(w.sigma_z), "--s-dl", str(w.sigma_dl), "-e", "5000",
-         "--detected", "--full", "--tan-bins", "4", "--kernel-point", "0",
-         "10", "-o", str(out), "--seed", "3"],
+         "--detected", "--full", "--tan-bins", "4", "--kernel-point",
+         "0,10", "-o", str(out), "--seed", "3"],
         capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0
