@@ -232,6 +232,25 @@ int strip_mlem_kernel_evaluations(strip_mlem_ctx* ctx, int64_t* count);
  * current image (host array of n_events floats). */
 int strip_mlem_forward(strip_mlem_ctx* ctx, float* denominators);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc): strip_mlem_forward's projections D_e of device-resident
+ * responses (n_events AoS float triples z_u, z_d, dl) through a
+ * device-resident row-major n_y x n_z image into device_denominators, on
+ * `stream` (a hipStream_t; NULL = the default stream) of the current device,
+ * asynchronously.  It calls no hipMalloc / hipFree: the per-event parameters
+ * live in a fixed device array of STRIP_MLEM_NOALLOC_MAX_EVENTS events
+ * (14 MiB) that every GPU keeps until the process ends, so n_events above it
+ * is refused, and calls that overlap on one GPU race on that array.  Modes
+ * without a sensitivity map only (sensitivity 0); NULL mode = the CPU
+ * semantics. */
+#define STRIP_MLEM_NOALLOC_MAX_EVENTS (1 << 18)
+int strip_mlem_forward_noalloc(const strip_mlem_geometry* geometry,
+                               const strip_mlem_mode* mode,
+                               const float* device_responses, int64_t n_events,
+                               const float* device_rho,
+                               float* device_denominators, void* stream);
+
 /* ---- differentiation (no counterpart in the reference; the adjoint form of
  * the same kernels).  Defined for the CPU-path semantics (Gaussian kernel, no
  * sensitivity, guarded 1/D) on the row-band algorithm; other modes return

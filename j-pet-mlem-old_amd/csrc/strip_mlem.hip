@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -2320,6 +2321,108 @@ int strip_mlem_batch_autotune(strip_mlem_batch* b, double* best_ms) {
     if ((rc = strip_mlem_set_rho(b->items[i], saved[i].data()))) return rc;
   if (best_ms) *best_ms = best;
   strip::g_last_error.clear();
+  return STRIP_MLEM_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Forward projection that allocates nothing
+// ---------------------------------------------------------------------------
+
+namespace {
+
+// Scratch of strip_mlem_forward_noalloc: the per-event parameters of up to
+// STRIP_MLEM_NOALLOC_MAX_EVENTS events, as four arrays (a, b, c: float4 each,
+// clamped bbox: uint2; 56 bytes per event, 14 MiB).  Every GPU the library
+// runs on holds its own copy until the process ends; host code reaches it
+// only through the per-device address looked up below.
+constexpr int64_t kNoallocEvents = STRIP_MLEM_NOALLOC_MAX_EVENTS;
+static __device__ __align__(256) unsigned char
+    scratch[kNoallocEvents * (3 * sizeof(float4) + sizeof(uint2))];
+
+constexpr int kNoallocDevices = 64;
+std::mutex g_scratch_mutex;
+unsigned char* g_scratch_ptr[kNoallocDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) and kept.
+int noalloc_scratch(int device, unsigned char** out) {
+  if (device < 0 || device >= kNoallocDevices)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return strip::fail(STRIP_MLEM_ERR_HIP,
+                         std::string("hipGetSymbolAddress(scratch): ") +
+                             hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = g_scratch_ptr[device];
+  return STRIP_MLEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int strip_mlem_forward_noalloc(const strip_mlem_geometry* geometry,
+                               const strip_mlem_mode* mode,
+                               const float* device_responses, int64_t n_events,
+                               const float* device_rho,
+                               float* device_denominators, void* stream) {
+  strip::g_last_error.clear();
+  int rc = check_geometry(geometry);
+  if (rc) return rc;
+  strip_mlem_mode m = {0, 0, 0, 1};
+  if (mode) {
+    if ((rc = check_mode(mode))) return rc;
+    m = *mode;
+  }
+  if (m.sensitivity != 0)
+    return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                       "the allocation-free forward projection has no "
+                       "sensitivity map: use a context for sensitivity modes");
+  if (n_events < 0 || n_events > kNoallocEvents)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "n_events must be 0.." +
+                           std::to_string(kNoallocEvents) +
+                           " (the allocation-free scratch holds that many)");
+  if (n_events == 0) return STRIP_MLEM_OK;
+  if (!device_responses || !device_rho || !device_denominators)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "NULL device pointer");
+  int device = 0;
+  HIP_TRY(hipGetDevice(&device));
+  unsigned char* base = nullptr;
+  if ((rc = noalloc_scratch(device, &base))) return rc;
+  float4* a = reinterpret_cast<float4*>(base);
+  float4* b = a + kNoallocEvents;
+  float4* c = b + kNoallocEvents;
+  uint2* box = reinterpret_cast<uint2*>(c + kNoallocEvents);
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                device));
+  const strip::DeviceGeometry g = strip::make_device_geometry(*geometry);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(setup_events_kernel,
+                     dim3((unsigned)((n_events + 255) / 256)), dim3(256), 0, s,
+                     g, device_responses, n_events, a, b, c, box);
+  HIP_TRY(hipGetLastError());
+  // one wavefront per event, four per workgroup, at most 16 workgroups/CU
+  const int grid = (int)std::max<int64_t>(
+      1, std::min<int64_t>((n_events + 3) / 4,
+                           (int64_t)std::max(cus, 1) * 16));
+  if (is_fast_mode(m))
+    hipLaunchKernelGGL((bbox_event_kernel<true, false>), dim3(grid),
+                       dim3(256), 0, s, g, mode_flags(m), a, b, box, n_events,
+                       device_rho, nullptr, nullptr, device_denominators);
+  else
+    hipLaunchKernelGGL((bbox_event_kernel<false, false>), dim3(grid),
+                       dim3(256), 0, s, g, mode_flags(m), a, b, box, n_events,
+                       device_rho, nullptr, nullptr, device_denominators);
+  HIP_TRY(hipGetLastError());
   return STRIP_MLEM_OK;
 }
 

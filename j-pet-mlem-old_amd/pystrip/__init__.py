@@ -226,6 +226,9 @@ def load_library():
         "strip_mlem_kernel_evaluations": (ctypes.c_int,
                                           [P, ctypes.POINTER(i64)]),
         "strip_mlem_forward": (ctypes.c_int, [P, fptr]),
+        "strip_mlem_forward_noalloc": (ctypes.c_int, [
+            ctypes.POINTER(Geometry), ctypes.POINTER(Mode), ctypes.c_void_p,
+            i64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
         "strip_mlem_set_timing": (ctypes.c_int, [P, ctypes.c_int]),
         "strip_mlem_get_timing": (ctypes.c_int, [P,
                                                  ctypes.POINTER(ctypes.c_double),
@@ -287,6 +290,7 @@ def exported_symbols():
         "strip_mlem_set_subsets", "strip_mlem_image_total",
         "strip_mlem_set_plan_events",
         "strip_mlem_kernel_evaluations", "strip_mlem_forward",
+        "strip_mlem_forward_noalloc",
         "strip_mlem_set_timing", "strip_mlem_get_timing",
         "strip_mlem_get_timing_detail",
         "strip_mlem_device_name", "strip_mlem_last_error", "strip_mlem_run",
@@ -594,6 +598,23 @@ class Reconstruction:
         buf = ctypes.create_string_buffer(256)
         _check(self._lib.strip_mlem_device_name(self._ctx, buf, 256))
         return buf.value.decode(errors="replace")
+
+
+NOALLOC_MAX_EVENTS = 1 << 18  # STRIP_MLEM_NOALLOC_MAX_EVENTS
+
+
+def forward_noalloc(geometry: Geometry, responses_ptr: int, n: int,
+                    rho_ptr: int, denominators_ptr: int, mode=None,
+                    stream: int = 0):
+    """strip_mlem_forward_noalloc: forward projections of n device-resident
+    responses through a device-resident row-major image into device memory,
+    asynchronously on `stream` of the current device, allocating nothing."""
+    lib = load_library()
+    m = None if mode is None else ctypes.byref(Mode(*mode))
+    _check(lib.strip_mlem_forward_noalloc(
+        ctypes.byref(geometry), m, ctypes.c_void_p(responses_ptr), int(n),
+        ctypes.c_void_p(rho_ptr), ctypes.c_void_p(denominators_ptr),
+        ctypes.c_void_p(stream) if stream else None))
 
 
 def run(geometry: Geometry, responses, rho, n_blocks: int,
