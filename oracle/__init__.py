@@ -356,3 +356,118 @@ def osem(g, events, n_subsets, rho0=None, iterations=1, mode=MODE_CPU,
     if rc:
         raise RuntimeError("oracle_osem failed")
     return rho
+
+
+# ---------------------------------------------------------------------------
+# 3D hybrid reconstruction
+# ---------------------------------------------------------------------------
+
+def _geometry_args(geo):
+    return (np.ascontiguousarray(geo.seg, np.float32),
+            np.ascontiguousarray(geo.widths, np.float32),
+            np.ascontiguousarray(geo.pixels, np.int16),
+            np.ascontiguousarray(geo.positions, np.float32),
+            np.ascontiguousarray(geo.weights, np.float32),
+            np.ascontiguousarray(geo.info_begin, np.int64),
+            np.ascontiguousarray(geo.info_end, np.int64))
+
+
+def ref_hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
+                    sens_mode=0, use_3d_sensitivity=False, sens_in=None,
+                    n_threads=0):
+    """The reference's own PET3D::Hybrid::Reconstruction (oracle/_ref) on
+    the geometry arrays of pystrip.hybrid.HybridGeometry.  Returns
+    (rho, n_used_events, sensitivity, normalised weights)."""
+    L = ref()
+    fn = L.ref_hybrid_mlem
+    fp, i16p, i64p = (ctypes.POINTER(ctypes.c_float),
+                      ctypes.POINTER(ctypes.c_int16),
+                      ctypes.POINTER(ctypes.c_int64))
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, fp, fp, ctypes.c_int64, i16p, fp, fp,
+                   ctypes.c_int, i64p, i64p, ctypes.c_int, ctypes.c_int,
+                   ctypes.c_float, ctypes.c_int, ctypes.c_float,
+                   ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, fp, ctypes.c_int,
+                   fp, ctypes.c_int64, fp, ctypes.c_int, ctypes.c_int, i64p,
+                   fp, fp]
+    seg, widths, pixels, positions, weights, begin, end = _geometry_args(geo)
+    ev = np.ascontiguousarray(events, np.float32)
+    rho = (np.ones(geo.shape, np.float32) if rho0 is None
+           else np.ascontiguousarray(rho0, np.float32).copy())
+    depth = (geo.n_planes_half if geo.n_planes_half > 1
+             else (geo.n_planes // 2 if use_3d_sensitivity else 1))
+    sens = np.empty((depth, geo.n_rows, geo.n_columns), np.float32)
+    sin = (np.ascontiguousarray(sens_in, np.float32) if sens_in is not None
+           else None)
+    wout = np.empty_like(weights)
+    used = ctypes.c_int64()
+    rc = fn(geo.n_detectors, _p(seg, ctypes.c_float),
+            _p(widths, ctypes.c_float), geo.n_infos,
+            _p(pixels, ctypes.c_int16), _p(positions, ctypes.c_float),
+            _p(weights, ctypes.c_float), geo.n_planes_half,
+            _p(begin, ctypes.c_int64), _p(end, ctypes.c_int64),
+            geo.n_columns, geo.n_rows, geo.pixel_size, geo.n_planes,
+            geo.z_left, geo.sigma_z, geo.sigma_dl, geo.length, weights_mode,
+            sens_mode, int(bool(use_3d_sensitivity)),
+            _p(sin, ctypes.c_float) if sin is not None else None, depth,
+            _p(ev, ctypes.c_float), ev.shape[0], _p(rho, ctypes.c_float),
+            iterations, n_threads, ctypes.byref(used),
+            _p(sens, ctypes.c_float), _p(wout, ctypes.c_float))
+    if rc:
+        raise RuntimeError("ref_hybrid_mlem failed")
+    return rho, used.value, sens, wout
+
+
+class OracleHybrid(ctypes.Structure):
+    _fields_ = [("n_detectors", ctypes.c_int), ("n_infos", ctypes.c_int64),
+                ("n_planes_half", ctypes.c_int),
+                ("seg", ctypes.c_void_p), ("widths", ctypes.c_void_p),
+                ("pixels", ctypes.c_void_p), ("positions", ctypes.c_void_p),
+                ("info_begin", ctypes.c_void_p), ("info_end", ctypes.c_void_p),
+                ("n_columns", ctypes.c_int), ("n_rows", ctypes.c_int),
+                ("pixel_size", ctypes.c_double), ("n_planes", ctypes.c_int),
+                ("z_left", ctypes.c_double), ("sigma_z", ctypes.c_double),
+                ("sigma_dl", ctypes.c_double), ("length", ctypes.c_double)]
+
+
+def hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
+                sens_mode=0, sens_in=None, n_threads=0, dtype=np.float32):
+    """The C restatement of the hybrid reconstruction (strip_oracle.c,
+    hybrid_oracle_impl.inc).  Returns (rho, n_used, sensitivity, weights)."""
+    keep = _geometry_args(geo)
+    seg, widths, pixels, positions, weights, begin, end = keep
+    h = OracleHybrid(geo.n_detectors, geo.n_infos, geo.n_planes_half,
+                     seg.ctypes.data, widths.ctypes.data, pixels.ctypes.data,
+                     positions.ctypes.data, begin.ctypes.data,
+                     end.ctypes.data, geo.n_columns, geo.n_rows,
+                     float(np.float32(geo.pixel_size)), geo.n_planes,
+                     float(np.float32(geo.z_left)),
+                     float(np.float32(geo.sigma_z)),
+                     float(np.float32(geo.sigma_dl)),
+                     float(np.float32(geo.length)))
+    real = ctypes.c_float if dtype == np.float32 else ctypes.c_double
+    fn = (lib().oracle_hybrid_mlem_f32 if dtype == np.float32
+          else lib().oracle_hybrid_mlem_f64)
+    fp = ctypes.POINTER(ctypes.c_float)
+    rp = ctypes.POINTER(real)
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.POINTER(OracleHybrid), fp, ctypes.c_int,
+                   ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_int64, rp,
+                   ctypes.c_int, ctypes.c_int,
+                   ctypes.POINTER(ctypes.c_int64), rp, rp]
+    ev = np.ascontiguousarray(events, np.float32)
+    rho = (np.ones(geo.shape, dtype) if rho0 is None
+           else np.ascontiguousarray(rho0, dtype).copy())
+    sens = np.empty((geo.n_rows, geo.n_columns), dtype)
+    wout = np.empty(geo.n_infos, dtype)
+    sin = (np.ascontiguousarray(sens_in, np.float32) if sens_in is not None
+           else None)
+    used = ctypes.c_int64()
+    rc = fn(ctypes.byref(h), _p(weights, ctypes.c_float), weights_mode,
+            sens_mode, 0, _p(sin, ctypes.c_float) if sin is not None else None,
+            _p(ev, ctypes.c_float), ev.shape[0], _p(rho, real), iterations,
+            n_threads, ctypes.byref(used), _p(sens, real), _p(wout, real))
+    if rc:
+        raise RuntimeError(f"oracle_hybrid_mlem failed ({rc})")
+    return rho, used.value, sens, wout

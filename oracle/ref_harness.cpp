@@ -11,12 +11,16 @@
 //   PET2D::Strip::Scanner<F, short>                      scanner.h:26-291
 //   PET2D::Strip::GaussianKernel<F>, AnalyticKernel<F>   gaussian_kernel.h, analytic_kernel.h
 //   util::text_parser (fast .txt event loading)          util/text_parser.h:29-175
+//   PET3D::Hybrid::Reconstruction<Scanner, GaussianKernel<float>>
+//                                                        3d/hybrid/reconstruction.h:32-549
+//   PET2D::Barrel::GeometrySOA (filled from caller arrays) 2d/barrel/geometry_soa.h:31-287
 //   PET2D::PixelMap text/binary streams, util::nrrd_writer,
 //   util::obstream / ibstream (image outputs and --rho inputs)
 //                                                        pixel_map.h:113-157, nrrd_writer.h, bstream.h
 #include <cstdint>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -28,6 +32,10 @@
 #include "common/types.h"
 #include "util/bstream.h"
 #include "util/nrrd_writer.h"
+#include "2d/barrel/square_detector.h"
+#include "2d/barrel/generic_scanner.h"
+#include "3d/hybrid/scanner.h"
+#include "3d/hybrid/reconstruction.h"
 
 #include "strip_oracle.h"  // oracle_geometry only (plain struct)
 
@@ -240,6 +248,98 @@ int ref_read_start_image(const oracle_geometry* g, const char* fn, int is_txt,
     }
   }
   for (int i = 0; i < rho.size; ++i) out[i] = rho[i];
+  return 0;
+}
+
+// 3D hybrid reconstruction (reconstruction_cmd.cpp:459-667 without the file
+// handling) over a GeometrySOA filled from the caller's arrays:
+//   seg: 4 floats per LOR (start x, y = the second detector's centre, end
+//   x, y = the first's), widths: 1 per LOR, pixels: 2 int16 per pixel info,
+//   positions: 1 per info, weights: n_infos x n_planes_half, info_begin /
+//   info_end: per LOR (LOR index first (first + 1) / 2 + second).
+// weights_mode 1 calls calculate_weight (no system matrix); sens_mode 0
+// calculate_sensitivity, 1 set_sensitivity_to_one, 2 copies sens_in (depth
+// planes); then normalize_geometry_weights, the events (first, second,
+// z_up, z_dn, dl per event) through translate_to_frame and the grid filter,
+// and n_iterations of operator().  rho (n_columns x n_rows x n_planes) is
+// the start image and receives the result; sens_out (may be null) the
+// sensitivity, weights_out (may be null) the normalised weights.
+int ref_hybrid_mlem(int n_detectors, const float* seg, const float* widths,
+                    int64_t n_infos, const int16_t* pixels,
+                    const float* positions, const float* weights,
+                    int n_planes_half, const int64_t* info_begin,
+                    const int64_t* info_end, int n_columns, int n_rows,
+                    float pixel_size, int n_planes, float z_left,
+                    float sigma_z, float sigma_dl, float length,
+                    int weights_mode, int sens_mode, int use_3d_sensitivity,
+                    const float* sens_in, int sens_depth,
+                    const float* events, int64_t n_events, float* rho,
+                    int n_iterations, int n_threads, int64_t* n_used,
+                    float* sens_out, float* weights_out) {
+#if _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+  using Detector = PET2D::Barrel::SquareDetector<float>;
+  using Scanner2D = PET2D::Barrel::GenericScanner<Detector, short>;
+  using Scanner = PET3D::Hybrid::Scanner<Scanner2D>;
+  using Kernel = PET2D::Strip::GaussianKernel<float>;
+  using Rec = PET3D::Hybrid::Reconstruction<Scanner, Kernel>;
+  Scanner2D barrel;
+  Scanner scanner(barrel, length);
+  scanner.set_sigmas(sigma_z, sigma_dl);
+  PET2D::PixelGrid<float, short> pixel_grid((short)n_columns, (short)n_rows,
+                                            pixel_size);
+  Rec::Grid grid(pixel_grid, z_left, (short)n_planes);
+  Rec::Geometry geo((short)n_detectors, (size_t)n_infos,
+                    (size_t)n_planes_half);
+  for (size_t l = 0; l < geo.n_lors; ++l) {
+    geo.lor_line_segments[l] = PET2D::LineSegment<float>(
+        PET2D::Point<float>(seg[4 * l], seg[4 * l + 1]),
+        PET2D::Point<float>(seg[4 * l + 2], seg[4 * l + 3]));
+    geo.lor_widths[l] = widths[l];
+    geo.lor_pixel_info_begin[l] = (size_t)info_begin[l];
+    geo.lor_pixel_info_end[l] = (size_t)info_end[l];
+  }
+  for (int64_t i = 0; i < n_infos; ++i) {
+    geo.pixels[i] = PET2D::Pixel<short>(pixels[2 * i], pixels[2 * i + 1]);
+    geo.pixel_positions[i] = positions[i];
+  }
+  for (int64_t i = 0; i < n_infos * n_planes_half; ++i)
+    geo.pixel_weights[i] = weights[i];
+  Rec rec(scanner, grid, geo, use_3d_sensitivity != 0);
+  if (weights_mode == 1) rec.calculate_weight();
+  if (sens_mode == 2) {
+    const int64_t n = (int64_t)rec.sensitivity.size;
+    for (int64_t i = 0; i < n; ++i) rec.sensitivity[i] = sens_in[i];
+    (void)sens_depth;
+  } else if (sens_mode == 1) {
+    rec.set_sensitivity_to_one();
+  } else {
+    rec.calculate_sensitivity();
+  }
+  rec.normalize_geometry_weights();
+  std::stringstream in;
+  for (int64_t e = 0; e < n_events; ++e) {
+    // Response read as text: "first second z_up z_dn dl"
+    // (3d/hybrid/scanner.h Response(std::istream&))
+    in << (int)events[5 * e] << ' ' << (int)events[5 * e + 1] << ' ';
+    in.precision(9);
+    in << events[5 * e + 2] << ' ' << events[5 * e + 3] << ' '
+       << events[5 * e + 4] << '\n';
+  }
+  rec << in;
+  const int64_t nv = (int64_t)grid.n_voxels;
+  for (int64_t i = 0; i < nv; ++i) rec.rho[i] = rho[i];
+  int used = 0;
+  for (int it = 0; it < n_iterations; ++it) used = rec();
+  for (int64_t i = 0; i < nv; ++i) rho[i] = rec.rho[i];
+  if (n_used) *n_used = n_iterations ? used : rec.n_events();
+  if (sens_out)
+    for (int i = 0; i < rec.sensitivity.size; ++i)
+      sens_out[i] = rec.sensitivity[i];
+  if (weights_out)
+    for (int64_t i = 0; i < n_infos * n_planes_half; ++i)
+      weights_out[i] = geo.pixel_weights[i];
   return 0;
 }
 

@@ -43,6 +43,7 @@
 #define EXP expf
 #define ATAN atanf
 #include "strip_oracle_impl.inc"
+#include "hybrid_oracle_impl.inc"
 #undef REAL
 #undef SFX
 #undef SQRT
@@ -55,6 +56,7 @@
 #define EXP exp
 #define ATAN atan
 #include "strip_oracle_impl.inc"
+#include "hybrid_oracle_impl.inc"
 #undef REAL
 #undef SFX
 #undef SQRT

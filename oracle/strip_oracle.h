@@ -158,6 +158,54 @@ int oracle_mlem_loss_f64(const oracle_geometry* g, const double* events,
                          const double* d_sigma, double* loss, double* jvp,
                          double* rho_out);
 
+/* ---- 3D hybrid reconstruction (src/3d/hybrid/reconstruction.h:32-549) ----
+ * Geometry in the reference's GeometrySOA layout (geometry_soa.h:262-287):
+ * per LOR (index first (first + 1) / 2 + second, lor.h:36-38) the segment
+ * from the second detector's centre to the first's (4 floats), its width,
+ * and the range [info_begin, info_end) of its pixel infos (pixel x, y as
+ * int16 pairs, scaled position along the segment, sorted).  The voxel grid
+ * is the pixel grid centred on the origin (pixel_grid.h:33-40) times
+ * n_planes planes from z_left (voxel_grid.h:22-59). */
+typedef struct {
+  int n_detectors;
+  int64_t n_infos;
+  int n_planes_half;
+  const float* seg;
+  const float* widths;
+  const int16_t* pixels;
+  const float* positions;
+  const int64_t* info_begin;
+  const int64_t* info_end;
+  int n_columns, n_rows;
+  double pixel_size;
+  int n_planes;
+  double z_left;
+  double sigma_z, sigma_dl;
+  double length;
+} oracle_hybrid;
+
+/* The CLI's preparation (reconstruction_cmd.cpp:459-560) and n_iterations of
+ * Reconstruction::operator() (reconstruction.h:274-445): weights (n_infos x
+ * n_planes_half) as given (weights_mode 0) or from calculate_weight (1,
+ * plane 0); sensitivity from calculate_sensitivity (sens_mode 0), ones (1)
+ * or sens_in (2, depth planes); normalize_geometry_weights; events (first,
+ * second, z_up, z_dn, dl per event) through translate_to_frame and
+ * bb_intersects_grid_with_positive_weight.  rho: n_planes x n_rows x
+ * n_columns voxels, in and out.  n_used: events kept; sens_out (depth
+ * planes) and weights_out may be NULL. */
+int oracle_hybrid_mlem_f32(const oracle_hybrid* h, const float* weights,
+                           int weights_mode, int sens_mode,
+                           int use_3d_sensitivity, const float* sens_in,
+                           const float* events, int64_t n_events, float* rho,
+                           int n_iterations, int n_threads, int64_t* n_used,
+                           float* sens_out, float* weights_out);
+int oracle_hybrid_mlem_f64(const oracle_hybrid* h, const float* weights,
+                           int weights_mode, int sens_mode,
+                           int use_3d_sensitivity, const float* sens_in,
+                           const float* events, int64_t n_events, double* rho,
+                           int n_iterations, int n_threads, int64_t* n_used,
+                           double* sens_out, double* weights_out);
+
 #ifdef __cplusplus
 }
 #endif
