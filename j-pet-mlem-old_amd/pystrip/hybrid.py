@@ -11,7 +11,7 @@ projection t along the segment (geometry_soa.h:146-170), stored LOR after
 LOR in index order, followed by one padding info (the grid's centre pixel,
 weight 0, position 2) that belongs to no LOR: the reference's event setup
 takes the info after the first position above the box's far end
-(reconstruction.h:132-137), which for the last LOR with infos is one past
+(reconstruction.h:131-138), which for the last LOR with infos is one past
 the arrays.  The reference builds this from a Monte-Carlo system
 matrix or a Boost.Geometry LOR-pixel intersection (2d_barrel_matrix,
 2d_barrel_geometry), neither of which is part of this build; the tube of
@@ -78,6 +78,64 @@ def lor_index(first, second):
     return first * (first + 1) // 2 + second
 
 
+def _tube_infos(seg, lor_width, n_pixels, pixel_size, fov, chunk=2048):
+    """Pixel infos of the LORs seg (m, 4) float32 (start, end): for each LOR
+    the pixels whose centre is inside the field of view and within half the
+    width of the LOR's line, with their scaled projection t along the
+    segment.  Candidates come from walking the LOR's main axis (at most
+    ceil(width / (|cos| pixel)) + 3 pixels across per step); the test is
+    applied in FP64.  Returns (lor, raster index, x, y, t float32) arrays."""
+    n, ps = n_pixels, pixel_size
+    ll = -n * ps / 2
+    c0 = ll + ps / 2
+    axis = np.arange(n)
+    centre = ll + ps / 2 + axis * ps
+    out = []
+    for lo in range(0, len(seg), chunk):
+        sg = seg[lo:lo + chunk]
+        sx, sy = sg[:, 0].astype(np.float64), sg[:, 1].astype(np.float64)
+        d = (sg[:, 2:] - sg[:, :2]).astype(np.float64)
+        length = np.hypot(d[:, 0], d[:, 1])
+        ok = length > 0
+        length = np.where(ok, length, 1.0)
+        ux, uy = d[:, 0] / length, d[:, 1] / length
+        horiz = np.abs(ux) >= np.abs(uy)
+        # main-axis coordinate a (column for horizontal LORs, row else);
+        # the line's cross coordinate there, and the half band across
+        main = np.where(horiz, ux, uy)
+        safe = np.where(np.abs(main) > 0, main, 1.0)
+        cross_at = np.where(horiz[:, None],
+                            sy[:, None] + (centre[None] - sx[:, None]) *
+                            (uy / safe)[:, None],
+                            sx[:, None] + (centre[None] - sy[:, None]) *
+                            (ux / safe)[:, None])
+        half = lor_width / (2 * np.abs(safe))
+        k = int(np.ceil(2 * half.max() / ps)) + 3
+        b0 = np.floor((cross_at - half[:, None] - c0) / ps).astype(np.int64) - 1
+        cand = b0[:, :, None] + np.arange(k)[None, None, :]
+        a_idx = np.broadcast_to(axis[None, :, None], cand.shape)
+        ix = np.where(horiz[:, None, None], a_idx, cand)
+        iy = np.where(horiz[:, None, None], cand, a_idx)
+        valid = (ix >= 0) & (ix < n) & (iy >= 0) & (iy < n) & ok[:, None, None]
+        ixc, iyc = np.clip(ix, 0, n - 1), np.clip(iy, 0, n - 1)
+        cx = ll + ps / 2 + ixc * ps
+        cy = ll + ps / 2 + iyc * ps
+        dist = (cx - sx[:, None, None]) * (-uy)[:, None, None] + \
+            (cy - sy[:, None, None]) * ux[:, None, None]
+        sel = valid & (cx * cx + cy * cy < fov * fov) & \
+            (np.abs(dist) < lor_width / 2)
+        li = np.broadcast_to(np.arange(lo, lo + len(sg))[:, None, None],
+                             cand.shape)[sel]
+        t = (((cx - sx[:, None, None]) * ux[:, None, None] +
+              (cy - sy[:, None, None]) * uy[:, None, None]) /
+             length[:, None, None])[sel].astype(np.float32)
+        out.append((li, iyc[sel] * n + ixc[sel], ixc[sel], iyc[sel], t))
+    cat = [np.concatenate([o[i] for o in out]) for i in range(5)]
+    lor, raster, x, y, t = cat
+    order = np.lexsort((raster, t, lor))
+    return lor[order], x[order], y[order], t[order]
+
+
 def ring_geometry(n_detectors=48, ring_radius=0.4, lor_width=0.02,
                   n_pixels=80, pixel_size=0.005, n_planes=None,
                   fov_radius=None, sigma_z=0.015, sigma_dl=0.06, length=0.5,
@@ -91,45 +149,17 @@ def ring_geometry(n_detectors=48, ring_radius=0.4, lor_width=0.02,
     centres = np.stack([ring_radius * np.cos(ang),
                         ring_radius * np.sin(ang)], 1).astype(np.float32)
     n_lors = (n_detectors + 1) * n_detectors // 2
-    seg = np.zeros((n_lors, 4), np.float32)
+    first = np.repeat(np.arange(n_detectors), np.arange(1, n_detectors + 1))
+    second = np.concatenate([np.arange(f + 1) for f in range(n_detectors)])
+    seg = np.concatenate([centres[second], centres[first]], 1)
     widths = np.full(n_lors, lor_width, np.float32)
-    # pixel centres of the grid (PixelGrid::center_at, pixel_grid.h:54-57)
-    ll = -n_pixels * pixel_size / 2
-    ix, iy = np.meshgrid(np.arange(n_pixels), np.arange(n_pixels))
-    cx = (ll + pixel_size / 2 + ix * pixel_size).ravel()
-    cy = (ll + pixel_size / 2 + iy * pixel_size).ravel()
-    in_fov = cx * cx + cy * cy < fov * fov
-    pix_all = np.stack([ix.ravel(), iy.ravel()], 1).astype(np.int16)
-    begin = np.zeros(n_lors, np.int64)
-    end = np.zeros(n_lors, np.int64)
-    pixels, positions = [], []
-    pos = 0
-    for first in range(n_detectors):
-        for second in range(first + 1):
-            li = lor_index(first, second)
-            s, e = centres[second], centres[first]
-            seg[li] = (s[0], s[1], e[0], e[1])
-            begin[li] = pos
-            if first != second:
-                d = (e - s).astype(np.float64)
-                length_ = float(np.hypot(*d))
-                u = d / length_
-                nrm = np.array([-u[1], u[0]])
-                dist = (cx - s[0]) * nrm[0] + (cy - s[1]) * nrm[1]
-                sel = in_fov & (np.abs(dist) < lor_width / 2)
-                if sel.any():
-                    t = ((cx[sel] - s[0]) * u[0] + (cy[sel] - s[1]) * u[1]) \
-                        / length_
-                    t = t.astype(np.float32)
-                    order = np.argsort(t, kind="stable")
-                    pixels.append(pix_all[sel][order])
-                    positions.append(t[order])
-                    pos += int(sel.sum())
-            end[li] = pos
-    pixels.append(np.array([[n_pixels // 2, n_pixels // 2]], np.int16))
-    positions.append(np.array([2.0], np.float32))
-    pixels = np.concatenate(pixels)
-    positions = np.concatenate(positions)
+    lor, x, y, t = _tube_infos(seg, lor_width, n_pixels, pixel_size, fov)
+    counts = np.bincount(lor, minlength=n_lors)
+    end = np.cumsum(counts).astype(np.int64)
+    begin = end - counts
+    pixels = np.concatenate([np.stack([x, y], 1),
+                             [[n_pixels // 2, n_pixels // 2]]]).astype(np.int16)
+    positions = np.concatenate([t, np.array([2.0], np.float32)])
     rng = np.random.default_rng(weight_seed)
     weights = rng.uniform(0.2, 1.0, size=(n_planes_half, len(positions))
                           ).astype(np.float32)
@@ -205,3 +235,231 @@ def hybrid_events(geo: HybridGeometry, n, seed=0, radius=0.12, half_z=0.1,
         out[sl, 4] = dl[idx] + geo.sigma_dl * g[idx, 2]
         filled += take
     return out
+
+
+# ---------------------------------------------------------------------------
+# Binding of include/hybrid_mlem.h (lib/libstrip_mlem.so)
+# ---------------------------------------------------------------------------
+
+import ctypes  # noqa: E402
+
+HYBRID_MLEM_CPU = 0
+HYBRID_MLEM_LEGACY_GPU = 1
+
+RESPONSE_DTYPE = np.dtype([("first", "<i2"), ("second", "<i2"),
+                           ("z_up", "<f4"), ("z_dn", "<f4"), ("dl", "<f4")])
+
+
+class GeometryC(ctypes.Structure):
+    """hybrid_mlem_geometry."""
+    _fields_ = [("n_detectors", ctypes.c_int32),
+                ("n_pixel_infos", ctypes.c_int64),
+                ("n_planes_half", ctypes.c_int32),
+                ("lor_line_segments", ctypes.c_void_p),
+                ("lor_widths", ctypes.c_void_p),
+                ("pixels", ctypes.c_void_p),
+                ("pixel_positions", ctypes.c_void_p),
+                ("pixel_weights", ctypes.c_void_p),
+                ("lor_pixel_info_begin", ctypes.c_void_p),
+                ("lor_pixel_info_end", ctypes.c_void_p),
+                ("n_columns", ctypes.c_int32), ("n_rows", ctypes.c_int32),
+                ("n_planes", ctypes.c_int32),
+                ("pixel_size", ctypes.c_double), ("z_left", ctypes.c_double),
+                ("sigma_z", ctypes.c_double), ("sigma_dl", ctypes.c_double),
+                ("length", ctypes.c_double)]
+
+
+_bound = None
+
+
+def exported_symbols():
+    """Names of the hybrid C-ABI entry points this binding uses."""
+    return ["hybrid_mlem_create", "hybrid_mlem_destroy",
+            "hybrid_mlem_prepare_weights", "hybrid_mlem_add_responses",
+            "hybrid_mlem_clear_events", "hybrid_mlem_n_events",
+            "hybrid_mlem_get_frame_events", "hybrid_mlem_set_rho",
+            "hybrid_mlem_get_rho", "hybrid_mlem_get_sensitivity",
+            "hybrid_mlem_get_weights", "hybrid_mlem_iterate",
+            "hybrid_mlem_last_iteration_ms", "hybrid_mlem_voxel_visits"]
+
+
+def load_library():
+    """lib/libstrip_mlem.so with the hybrid entry points' signatures set
+    (raises if the library has not been built)."""
+    global _bound
+    if _bound is not None:
+        return _bound
+    from . import load_library as _load
+    lib = _load()
+    P = ctypes.c_void_p
+    i64 = ctypes.c_int64
+    fp = ctypes.POINTER(ctypes.c_float)
+    sigs = {
+        "hybrid_mlem_create": (ctypes.c_int, [ctypes.POINTER(GeometryC),
+                                              ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(P)]),
+        "hybrid_mlem_destroy": (None, [P]),
+        "hybrid_mlem_prepare_weights": (ctypes.c_int, [P, ctypes.c_int,
+                                                       ctypes.c_int, fp]),
+        "hybrid_mlem_add_responses": (ctypes.c_int, [P, P, i64]),
+        "hybrid_mlem_clear_events": (ctypes.c_int, [P]),
+        "hybrid_mlem_n_events": (i64, [P]),
+        "hybrid_mlem_get_frame_events": (ctypes.c_int, [P, P, P]),
+        "hybrid_mlem_set_rho": (ctypes.c_int, [P, fp]),
+        "hybrid_mlem_get_rho": (ctypes.c_int, [P, fp]),
+        "hybrid_mlem_get_sensitivity": (ctypes.c_int, [P, fp]),
+        "hybrid_mlem_get_weights": (ctypes.c_int, [P, fp]),
+        "hybrid_mlem_iterate": (ctypes.c_int, [P, ctypes.c_int]),
+        "hybrid_mlem_last_iteration_ms": (ctypes.c_int, [P, fp]),
+        "hybrid_mlem_voxel_visits": (ctypes.c_int, [P, ctypes.POINTER(i64)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _bound = lib
+    return lib
+
+
+def responses(events) -> np.ndarray:
+    """(n, 5) events (first, second, z_up, z_dn, dl) as the C ABI's
+    hybrid_mlem_response records (the binary event file's layout)."""
+    ev = np.asarray(events)
+    out = np.empty(ev.shape[0], RESPONSE_DTYPE)
+    out["first"] = ev[:, 0]
+    out["second"] = ev[:, 1]
+    out["z_up"] = ev[:, 2]
+    out["z_dn"] = ev[:, 3]
+    out["dl"] = ev[:, 4]
+    return out
+
+
+class HybridReconstruction:
+    """The GPU hybrid reconstruction of one geometry (include/hybrid_mlem.h),
+    used like the reference's command line drives
+    PET3D::Hybrid::Reconstruction (reconstruction_cmd.cpp:459-560)."""
+
+    def __init__(self, geo: HybridGeometry, semantics=HYBRID_MLEM_CPU,
+                 device=0):
+        from . import StripMLEMError
+        self._err = StripMLEMError
+        self._lib = load_library()
+        self.geo = geo
+        keep = [np.ascontiguousarray(geo.seg, np.float32),
+                np.ascontiguousarray(geo.widths, np.float32),
+                np.ascontiguousarray(geo.pixels, np.int16),
+                np.ascontiguousarray(geo.positions, np.float32),
+                np.ascontiguousarray(geo.weights[0], np.float32),
+                np.ascontiguousarray(geo.info_begin, np.int64),
+                np.ascontiguousarray(geo.info_end, np.int64)]
+        g = GeometryC(geo.n_detectors, geo.n_infos, geo.n_planes_half,
+                      *[a.ctypes.data for a in keep], geo.n_columns,
+                      geo.n_rows, geo.n_planes,
+                      float(np.float32(geo.pixel_size)),
+                      float(np.float32(geo.z_left)),
+                      float(np.float32(geo.sigma_z)),
+                      float(np.float32(geo.sigma_dl)),
+                      float(np.float32(geo.length)))
+        self._ctx = ctypes.c_void_p()
+        self._check(self._lib.hybrid_mlem_create(ctypes.byref(g), semantics,
+                                                 device,
+                                                 ctypes.byref(self._ctx)))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._lib.strip_mlem_last_error().decode(errors="replace")
+            raise self._err(f"status {rc}: {msg}")
+
+    def close(self):
+        if self._ctx:
+            self._lib.hybrid_mlem_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def prepare_weights(self, calculate_weight=True, sensitivity_mode=0,
+                        sensitivity=None):
+        s = None
+        if sensitivity is not None:
+            s = np.ascontiguousarray(sensitivity, np.float32)
+            if s.size != self.geo.n_rows * self.geo.n_columns:
+                raise ValueError("sensitivity must have n_rows x n_columns "
+                                 "values")
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._lib.hybrid_mlem_prepare_weights(
+            self._ctx, int(bool(calculate_weight)), int(sensitivity_mode),
+            s.ctypes.data_as(fp) if s is not None else None))
+
+    def add_responses(self, events):
+        r = responses(events)
+        self._check(self._lib.hybrid_mlem_add_responses(
+            self._ctx, r.ctypes.data if len(r) else None, len(r)))
+
+    def clear_events(self):
+        self._check(self._lib.hybrid_mlem_clear_events(self._ctx))
+
+    @property
+    def n_events(self):
+        return int(self._lib.hybrid_mlem_n_events(self._ctx))
+
+    def frame_events(self):
+        n = self.n_events
+        ints = np.empty((n, 4), np.int32)
+        floats = np.empty((n, 4), np.float32)
+        self._check(self._lib.hybrid_mlem_get_frame_events(
+            self._ctx, ints.ctypes.data, floats.ctypes.data))
+        return ints, floats
+
+    @property
+    def rho(self):
+        out = np.empty(self.geo.shape, np.float32)
+        self._check(self._lib.hybrid_mlem_get_rho(
+            self._ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    @rho.setter
+    def rho(self, value):
+        a = np.ascontiguousarray(value, np.float32)
+        if a.shape != self.geo.shape:
+            raise ValueError(f"rho must have shape {self.geo.shape}")
+        self._check(self._lib.hybrid_mlem_set_rho(
+            self._ctx, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    @property
+    def sensitivity(self):
+        out = np.empty((self.geo.n_rows, self.geo.n_columns), np.float32)
+        self._check(self._lib.hybrid_mlem_get_sensitivity(
+            self._ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    @property
+    def weights(self):
+        out = np.empty(self.geo.n_infos, np.float32)
+        self._check(self._lib.hybrid_mlem_get_weights(
+            self._ctx, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def iterate(self, n=1):
+        self._check(self._lib.hybrid_mlem_iterate(self._ctx, int(n)))
+
+    def last_iteration_ms(self):
+        ms = ctypes.c_float()
+        self._check(self._lib.hybrid_mlem_last_iteration_ms(
+            self._ctx, ctypes.byref(ms)))
+        return ms.value
+
+    def voxel_visits(self):
+        v = ctypes.c_int64()
+        self._check(self._lib.hybrid_mlem_voxel_visits(self._ctx,
+                                                       ctypes.byref(v)))
+        return v.value

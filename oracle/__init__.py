@@ -431,10 +431,7 @@ class OracleHybrid(ctypes.Structure):
                 ("sigma_dl", ctypes.c_double), ("length", ctypes.c_double)]
 
 
-def hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
-                sens_mode=0, sens_in=None, n_threads=0, dtype=np.float32):
-    """The C restatement of the hybrid reconstruction (strip_oracle.c,
-    hybrid_oracle_impl.inc).  Returns (rho, n_used, sensitivity, weights)."""
+def _oracle_hybrid(geo):
     keep = _geometry_args(geo)
     seg, widths, pixels, positions, weights, begin, end = keep
     h = OracleHybrid(geo.n_detectors, geo.n_infos, geo.n_planes_half,
@@ -446,6 +443,40 @@ def hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
                      float(np.float32(geo.sigma_z)),
                      float(np.float32(geo.sigma_dl)),
                      float(np.float32(geo.length)))
+    return h, keep
+
+
+def hybrid_frames(geo, events, weights):
+    """translate_to_frame + the box filter (FP32) of every event against
+    normalised weights: (ints (n, 5): lor, info begin, info end, plane
+    begin, plane end; floats (n, 4): up, right, tan, sec; keep (n,))."""
+    h, keep_alive = _oracle_hybrid(geo)
+    ev = np.ascontiguousarray(events, np.float32)
+    w = np.ascontiguousarray(weights, np.float32)
+    n = ev.shape[0]
+    ints = np.empty((n, 5), np.int64)
+    floats = np.empty((n, 4), np.float32)
+    keep = np.empty(n, np.int8)
+    fn = lib().oracle_hybrid_frames
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.POINTER(OracleHybrid), ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                   ctypes.c_void_p, ctypes.c_void_p]
+    if fn(ctypes.byref(h), w.ctypes.data, ev.ctypes.data, n, ints.ctypes.data,
+          floats.ctypes.data, keep.ctypes.data):
+        raise RuntimeError("oracle_hybrid_frames failed")
+    return ints, floats, keep.astype(bool)
+
+
+def hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
+                sens_mode=0, sens_in=None, n_threads=0, dtype=np.float32,
+                legacy=False):
+    """The C restatement of the hybrid reconstruction (strip_oracle.c,
+    hybrid_oracle_impl.inc).  Returns (rho, n_used, sensitivity, weights).
+    legacy: each visit of a voxel adds its own weight (the CUDA kernel's
+    update) instead of the voxel's last cached weight (the CPU path's)."""
+    h, keep = _oracle_hybrid(geo)
+    seg, widths, pixels, positions, weights, begin, end = keep
     real = ctypes.c_float if dtype == np.float32 else ctypes.c_double
     fn = (lib().oracle_hybrid_mlem_f32 if dtype == np.float32
           else lib().oracle_hybrid_mlem_f64)
@@ -455,7 +486,7 @@ def hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
     fn.argtypes = [ctypes.POINTER(OracleHybrid), fp, ctypes.c_int,
                    ctypes.c_int, ctypes.c_int, fp, fp, ctypes.c_int64, rp,
                    ctypes.c_int, ctypes.c_int,
-                   ctypes.POINTER(ctypes.c_int64), rp, rp]
+                   ctypes.POINTER(ctypes.c_int64), rp, rp, ctypes.c_int]
     ev = np.ascontiguousarray(events, np.float32)
     rho = (np.ones(geo.shape, dtype) if rho0 is None
            else np.ascontiguousarray(rho0, dtype).copy())
@@ -467,7 +498,8 @@ def hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
     rc = fn(ctypes.byref(h), _p(weights, ctypes.c_float), weights_mode,
             sens_mode, 0, _p(sin, ctypes.c_float) if sin is not None else None,
             _p(ev, ctypes.c_float), ev.shape[0], _p(rho, real), iterations,
-            n_threads, ctypes.byref(used), _p(sens, real), _p(wout, real))
+            n_threads, ctypes.byref(used), _p(sens, real), _p(wout, real),
+            int(bool(legacy)))
     if rc:
         raise RuntimeError(f"oracle_hybrid_mlem failed ({rc})")
     return rho, used.value, sens, wout

@@ -42,8 +42,10 @@
 #define SQRT sqrtf
 #define EXP expf
 #define ATAN atanf
+#define HY_FRAMES_ONCE 1
 #include "strip_oracle_impl.inc"
 #include "hybrid_oracle_impl.inc"
+#undef HY_FRAMES_ONCE
 #undef REAL
 #undef SFX
 #undef SQRT

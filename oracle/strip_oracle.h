@@ -192,19 +192,30 @@ typedef struct {
  * second, z_up, z_dn, dl per event) through translate_to_frame and
  * bb_intersects_grid_with_positive_weight.  rho: n_planes x n_rows x
  * n_columns voxels, in and out.  n_used: events kept; sens_out (depth
- * planes) and weights_out may be NULL. */
+ * planes) and weights_out may be NULL.  legacy = 0: the CPU path's update
+ * (each visit adds the voxel's cached weight, the last one written);
+ * legacy = 1: the CUDA kernel's (each visit adds its own weight,
+ * cuda/reconstruction/warp_granularity.cuh:136-168). */
 int oracle_hybrid_mlem_f32(const oracle_hybrid* h, const float* weights,
                            int weights_mode, int sens_mode,
                            int use_3d_sensitivity, const float* sens_in,
                            const float* events, int64_t n_events, float* rho,
                            int n_iterations, int n_threads, int64_t* n_used,
-                           float* sens_out, float* weights_out);
+                           float* sens_out, float* weights_out, int legacy);
 int oracle_hybrid_mlem_f64(const oracle_hybrid* h, const float* weights,
                            int weights_mode, int sens_mode,
                            int use_3d_sensitivity, const float* sens_in,
                            const float* events, int64_t n_events, double* rho,
                            int n_iterations, int n_threads, int64_t* n_used,
-                           double* sens_out, double* weights_out);
+                           double* sens_out, double* weights_out, int legacy);
+
+/* translate_to_frame and the box filter of every event (FP32), against
+ * normalised weights: per event lor, info begin, info end (the reference's
+ * upper_bound + 1, not clipped), plane begin, plane end (ints, 5 per
+ * event), up, right, tan, sec (4 floats) and the filter's verdict. */
+int oracle_hybrid_frames(const oracle_hybrid* h, const float* weights,
+                         const float* events, int64_t n_events,
+                         int64_t* ints, float* floats, int8_t* keep);
 
 #ifdef __cplusplus
 }

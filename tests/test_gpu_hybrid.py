@@ -1,0 +1,186 @@
+"""3D hybrid reconstruction on the GPU (include/hybrid_mlem.h) against the
+oracle's restatement (oracle/hybrid_oracle_impl.inc) and, where it was
+built, the reference's own PET3D::Hybrid::Reconstruction (oracle/_ref).
+
+Geometry: a 48-detector ring, 80 x 80 pixels of 5 mm, 80 planes, LOR tubes
+2 cm wide (pystrip.hybrid.ring_geometry).  Bounds: weights and sensitivity
+within 1e-5 / 1e-6 of the oracle (single expf of the weight and the same
+per-pixel summation order); frame events bit-exact (integer box and the
+FP32 frame parameters, evaluated without contraction on both sides); images
+rel-L2 <= 1e-4 (the GPU sums each event's denominator over a wavefront and
+adds its updates with float atomics, in another order than the reference's
+sequential loop).
+"""
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hybrid_mod(pystrip_mod):
+    from pystrip import hybrid
+    return hybrid
+
+
+@pytest.fixture(scope="module")
+def ring(hybrid_mod):
+    return hybrid_mod.ring_geometry()
+
+
+def _rec(hybrid_mod, geo, semantics=0, calculate_weight=True, sens_mode=0,
+         sens=None):
+    rec = hybrid_mod.HybridReconstruction(geo, semantics)
+    rec.prepare_weights(calculate_weight, sens_mode, sens)
+    return rec
+
+
+@pytest.mark.parametrize("weights_mode,sens_mode",
+                         [(1, 0), (0, 0), (1, 1), (0, 2)])
+def test_weights_and_sensitivity(oracle_mod, hybrid_mod, ring, weights_mode,
+                                 sens_mode):
+    print_case("hybrid weights/sensitivity", weights_mode, sens_mode)
+    sens_in = None
+    if sens_mode == 2:
+        sens_in = np.random.default_rng(4).uniform(
+            0.5, 2.0, (ring.n_rows, ring.n_columns)).astype(np.float32)
+    rec = _rec(hybrid_mod, ring, 0, weights_mode, sens_mode, sens_in)
+    _, _, s_ref, w_ref = oracle_mod.hybrid_mlem(
+        ring, np.zeros((0, 5), np.float32), iterations=0,
+        weights_mode=weights_mode, sens_mode=sens_mode, sens_in=sens_in)
+    s, w = rec.sensitivity, rec.weights
+    print("sensitivity max rel", np.max(np.abs(s - s_ref)) / s_ref.max(),
+          flush=True)
+    assert np.max(np.abs(s - s_ref)) <= 1e-6 * s_ref.max()
+    nz = w_ref != 0
+    assert np.array_equal(w != 0, nz)
+    assert np.max(np.abs(w - w_ref)[nz] / np.abs(w_ref[nz])) <= 1e-5
+    rec.close()
+
+
+def test_frame_events_bit_exact(oracle_mod, hybrid_mod, ring):
+    print_case("hybrid frame events")
+    ev = hybrid_mod.hybrid_events(ring, 20000, seed=11)
+    # boxes beyond the axial grid are filtered out; a box far before the
+    # LOR's first info still holds that info (upper_bound + 1 of a box
+    # below every position is the first info) and is kept
+    ev[:500, 2:4] += 0.6
+    ev[500:1000, 4] = 3.0
+    rec = _rec(hybrid_mod, ring)
+    rec.add_responses(ev)
+    ints, floats = rec.frame_events()
+    o_ints, o_floats, keep = oracle_mod.hybrid_frames(ring, ev, rec.weights)
+    print("kept", keep.sum(), "of", len(ev), flush=True)
+    assert rec.n_events == int(keep.sum())
+    assert not keep[:500].any()
+    far = o_ints[500:1000]
+    assert np.all(far[:, 2] - far[:, 1] == 1)
+    assert np.array_equal(far[:, 1], ring.info_begin[far[:, 0]])
+    k = o_ints[keep]
+    assert np.array_equal(ints[:, 0], k[:, 0])
+    assert np.array_equal(ints[:, 1], k[:, 1])
+    assert np.array_equal(ints[:, 2], np.minimum(k[:, 2], ring.n_infos))
+    assert np.array_equal(ints[:, 3] & 0xffff, k[:, 3])
+    assert np.array_equal(ints[:, 3] >> 16, k[:, 4])
+    assert np.array_equal(floats, o_floats[keep])
+    assert rec.voxel_visits() == int(((k[:, 2] - k[:, 1]) *
+                                      (k[:, 4] - k[:, 3])).sum())
+    # appending keeps the order of the kept events
+    rec.add_responses(ev[:3000])
+    assert rec.n_events == int(keep.sum() + keep[:3000].sum())
+    ints2, _ = rec.frame_events()
+    assert np.array_equal(ints2[len(ints):, :3],
+                          o_ints[:3000][keep[:3000]][:, :3])
+    rec.close()
+
+
+@pytest.mark.parametrize("weights_mode,sens_mode", [(1, 0), (0, 1)])
+def test_mlem_parity(oracle_mod, hybrid_mod, ring, weights_mode, sens_mode):
+    print_case("hybrid MLEM", weights_mode, sens_mode)
+    ev = hybrid_mod.hybrid_events(ring, 50000, seed=12)
+    rec = _rec(hybrid_mod, ring, 0, weights_mode, sens_mode)
+    rec.add_responses(ev)
+    rec.iterate(3)
+    gpu = rec.rho
+    ref, n_used, _, _ = oracle_mod.hybrid_mlem(
+        ring, ev, iterations=3, weights_mode=weights_mode,
+        sens_mode=sens_mode)
+    assert rec.n_events == n_used
+    e = oracle_mod.rel_l2(gpu, ref)
+    msg = f"rel-L2 vs oracle {e:.3e}"
+    if oracle_mod.ref_available():
+        r, nr, _, _ = oracle_mod.ref_hybrid_mlem(
+            ring, ev, iterations=3, weights_mode=weights_mode,
+            sens_mode=sens_mode)
+        assert nr == n_used
+        er = oracle_mod.rel_l2(gpu, r)
+        msg += f", vs reference build {er:.3e}"
+        assert er <= 1e-4
+    print(msg, "event kernel ms", rec.last_iteration_ms(), flush=True)
+    assert e <= 1e-4
+    assert np.all(np.isfinite(gpu))
+    rec.close()
+
+
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_repeated_visit(oracle_mod, hybrid_mod, ring, semantics):
+    from hybrid_cases import repeated_pixel_events, repeated_pixel_geometry
+    print_case("hybrid repeated voxel visit, semantics", semantics)
+    geo, li, first, second = repeated_pixel_geometry(ring)
+    ev = np.concatenate([
+        repeated_pixel_events(geo, li, first, second, 300, seed=5),
+        hybrid_mod.hybrid_events(geo, 3000, seed=6)])
+    rec = _rec(hybrid_mod, geo, semantics)
+    rec.add_responses(ev)
+    rec.iterate(2)
+    gpu = rec.rho
+    ref = oracle_mod.hybrid_mlem(geo, ev, iterations=2,
+                                 legacy=bool(semantics))[0]
+    other = oracle_mod.hybrid_mlem(geo, ev, iterations=2,
+                                   legacy=not semantics)[0]
+    e, e_other = oracle_mod.rel_l2(gpu, ref), oracle_mod.rel_l2(gpu, other)
+    print("rel-L2", e, "against the other semantics", e_other, flush=True)
+    assert e <= 1e-5
+    assert e_other > 1e-3
+    rec.close()
+
+
+def test_start_image_and_layout(oracle_mod, pystrip_mod, hybrid_mod, ring):
+    print_case("hybrid start image")
+    rng = np.random.default_rng(9)
+    rho0 = rng.uniform(0.5, 2.0, ring.shape).astype(np.float32)
+    ev = hybrid_mod.hybrid_events(ring, 20000, seed=13)
+    rec = _rec(hybrid_mod, ring)
+    rec.rho = rho0
+    assert np.array_equal(rec.rho, rho0)
+    rec.add_responses(ev)
+    rec.iterate(2)
+    ref = oracle_mod.hybrid_mlem(ring, ev, iterations=2, rho0=rho0)[0]
+    assert oracle_mod.rel_l2(rec.rho, ref) <= 1e-4
+    # a LOR index outside the geometry is refused, the events held stay
+    n = rec.n_events
+    bad = ev[:10].copy()
+    bad[3, 0] = 400
+    with pytest.raises(pystrip_mod.StripMLEMError):
+        rec.add_responses(bad)
+    assert rec.n_events == n
+    rec.close()
+
+
+def test_update_preserves_event_count_at_scale(hybrid_mod, ring):
+    # from rho = 1, one update spreads each event's unit over its voxels in
+    # proportion to weight x sensitivity: sum_v rho'(v) s(pixel(v)) = number
+    # of events (none of these repeats a voxel), at 2e6 events
+    print_case("hybrid mass identity, 2e6 events")
+    ev = hybrid_mod.hybrid_events(ring, 2_000_000, seed=14)
+    rec = _rec(hybrid_mod, ring)
+    rec.add_responses(ev)
+    rec.iterate(1)
+    mass = (rec.rho.astype(np.float64) *
+            rec.sensitivity.astype(np.float64)[None]).sum()
+    print("events", rec.n_events, "mass", mass, "visits", rec.voxel_visits(),
+          "event kernel ms", rec.last_iteration_ms(), flush=True)
+    assert abs(mass - rec.n_events) <= 1e-4 * rec.n_events
+    rec.close()

@@ -95,7 +95,7 @@ def test_calculate_weight_and_sensitivity(oracle_mod, ring):
     # calculate_weight: gauss_norm_w exp(-d^2 / (2 sigma_w^2)) with
     # sigma_w = 0.3 width, d the pixel centre's distance from the LOR's
     # line; sensitivity: the sum of each pixel's weights; the normalised
-    # weights divide by it (reconstruction.h:214-268)
+    # weights divide by it (reconstruction.h:218-271)
     geo = ring
     _, _, sens, w = oracle_mod.hybrid_mlem(geo, np.zeros((0, 5), np.float32),
                                            iterations=0, dtype=np.float64)
@@ -132,7 +132,7 @@ def test_update_preserves_event_count(oracle_mod, hybrid_mod, ring):
 
 
 def test_plane_and_info_ranges_follow_the_box(oracle_mod, hybrid_mod, ring):
-    # bb_intersects_grid_with_positive_weight (reconstruction.h:150-161):
+    # bb_intersects_grid_with_positive_weight (reconstruction.h:151-162):
     # an event whose box lies beyond the axial grid is dropped, one inside
     # is kept
     geo = ring
@@ -141,3 +141,44 @@ def test_plane_and_info_ranges_follow_the_box(oracle_mod, hybrid_mod, ring):
     _, n_in, _, _ = oracle_mod.hybrid_mlem(geo, inside, iterations=1)
     _, n_out, _, _ = oracle_mod.hybrid_mlem(geo, beyond, iterations=1)
     assert (n_in, n_out) == (1, 0)
+
+
+def test_repeated_visit_follows_the_kernel_cache(oracle_mod, hybrid_mod,
+                                                  ring):
+    # an event visits the pixel of its LOR's last info twice (the extra info
+    # past the box is the next LOR's first and repeats it): the reference's
+    # CPU path adds the voxel's last cached weight on both visits
+    from hybrid_cases import repeated_pixel_events, repeated_pixel_geometry
+    geo, li, first, second = repeated_pixel_geometry(ring)
+    ev = repeated_pixel_events(geo, li, first, second, 200, seed=7)
+    ints, _, keep = oracle_mod.hybrid_frames(
+        geo, ev, oracle_mod.hybrid_mlem(geo, ev[:0], iterations=0)[3])
+    assert keep.all()
+    assert np.all(ints[:, 2] == geo.info_end[li] + 1)
+    assert np.all(ints[:, 1] < geo.info_end[li])
+    a, na, _, w = oracle_mod.hybrid_mlem(geo, ev, iterations=1)
+    b, nb, _, _ = oracle_mod.hybrid_mlem(geo, ev, iterations=1, legacy=True)
+    assert na == nb == len(ev)
+    if oracle_mod.ref_available():
+        r, nr, _, _ = oracle_mod.ref_hybrid_mlem(geo, ev, iterations=1)
+        assert nr == na
+        assert oracle_mod.rel_l2(a, r) < 2e-6
+        assert oracle_mod.rel_l2(b, r) > 1e-3
+    # the two updates differ only on the repeated pixel's voxels, where the
+    # cached update is the legacy one with the first visit's weight
+    # replaced by the extra info's: a / b = 2 w_x / (w_j + w_x)
+    x, y = geo.pixels[geo.info_end[li]]
+    j = geo.info_end[li] - 1
+    diff = np.argwhere(np.abs(a - b) > 1e-6 * np.abs(b).max())
+    assert len(diff) and np.all(diff[:, 1] == y) and np.all(diff[:, 2] == x)
+    wx, wj = float(w[j + 1]), float(w[j])
+    ratio = a[:, y, x][b[:, y, x] > 0] / b[:, y, x][b[:, y, x] > 0]
+    assert np.allclose(ratio, 2 * wx / (wj + wx), rtol=1e-5)
+
+
+def test_legacy_update_equals_the_cached_one_without_repeats(
+        oracle_mod, hybrid_mod, ring):
+    ev = hybrid_mod.hybrid_events(ring, 5000, seed=8)
+    a = oracle_mod.hybrid_mlem(ring, ev, iterations=2)[0]
+    b = oracle_mod.hybrid_mlem(ring, ev, iterations=2, legacy=True)[0]
+    assert oracle_mod.rel_l2(a, b) < 1e-6
