@@ -292,6 +292,80 @@ def batched_timing(pystrip, phantom, n_members, iterations):
     return out
 
 
+def hybrid_timing(name, steps, warmup, cpu_sample, with_cpu, algorithm=0):
+    """The 3D hybrid reconstruction (include/hybrid_mlem.h) on a named
+    workload: weights and sensitivity prepared on the device, events
+    translated and filtered on the device, then `steps` timed MLEM
+    iterations after `warmup` untimed ones.  The CPU baseline is the
+    reference's own Reconstruction (oracle/_ref) on `cpu_sample` events of
+    the same workload: one iteration's time, the difference of runs with
+    one and with zero iterations."""
+    from pystrip import hybrid
+    w = hybrid.WORKLOADS[name]
+    t = time.perf_counter()
+    geo = w.geometry()
+    events = w.events(geo)
+    t_gen = time.perf_counter() - t
+    rec = hybrid.HybridReconstruction(geo)
+    rec.set_algorithm(algorithm)
+    t = time.perf_counter()
+    rec.prepare_weights(True, 0)
+    rec.add_responses(events)
+    t_setup = time.perf_counter() - t
+    visits = rec.voxel_visits()
+    t = time.perf_counter()
+    rec.iterate(warmup)
+    t_first = time.perf_counter() - t
+    t = time.perf_counter()
+    rec.iterate(steps)
+    elapsed = time.perf_counter() - t
+    kernel_ms = rec.last_iteration_ms()
+    n = rec.n_events
+    used_algorithm = rec.algorithm
+    mass = float((rec.rho.astype(np.float64)).sum())
+    rec.close()
+    out = {
+        "workload": name,
+        "n_events": w.n_events,
+        "n_events_kept": n,
+        "n_pixel_infos": geo.n_infos,
+        "voxels": list(geo.shape),
+        "algorithm": used_algorithm,
+        "steps": steps,
+        "warmup": warmup,
+        "event_updates_per_s": n * steps / elapsed,
+        "ms_per_iteration": elapsed * 1e3 / steps,
+        "event_kernel_ms_last_iteration": kernel_ms,
+        "voxel_visits_per_iteration": visits,
+        "voxel_visits_per_s": visits / (kernel_ms * 1e-3),
+        "timing_s": {"geometry_and_events": t_gen,
+                     "weights_sensitivity_and_frames": t_setup,
+                     "warmup_iterations_and_plan": t_first},
+        "check": {"image_mass": mass},
+        "cpu_baseline": None,
+    }
+    if with_cpu:
+        import oracle
+        if oracle.ref_available():
+            sample = events[:cpu_sample]
+            cores = usable_cores()
+            t = time.perf_counter()
+            oracle.ref_hybrid_mlem(geo, sample, iterations=0, n_threads=cores)
+            t0 = time.perf_counter() - t
+            t = time.perf_counter()
+            _, used, _, _ = oracle.ref_hybrid_mlem(geo, sample, iterations=1,
+                                                   n_threads=cores)
+            t1 = time.perf_counter() - t
+            out["cpu_baseline"] = {
+                "value": used / max(t1 - t0, 1e-9),
+                "unit": "event-updates/s", "cores": cores,
+                "kind": "reference",
+                "sample": f"{len(sample)} events of {name}, one iteration "
+                          f"(run with 1 iteration {t1:.2f} s minus run "
+                          f"with 0 {t0:.2f} s)"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +385,12 @@ def main():
     ap.add_argument("--no-osem", action="store_true")
     ap.add_argument("--no-parity", action="store_true",
                     help="skip the comparison with the reference's image")
+    ap.add_argument("--hybrid-workload",
+                    default="hybrid-ring384-160x160x100-5mm-1e7")
+    ap.add_argument("--hybrid-steps", type=int, default=5)
+    ap.add_argument("--hybrid-cpu-sample", type=int, default=200_000)
+    ap.add_argument("--hybrid-algorithm", type=int, default=0)
+    ap.add_argument("--no-hybrid", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -444,6 +524,12 @@ def main():
         rec.close()
         line["osem_largest"] = osem_timing(pystrip, phantom, args.osem_workload,
                                            16, 5, 10_000_000)
+    if not args.no_hybrid:
+        rec.close()
+        line["hybrid"] = hybrid_timing(args.hybrid_workload,
+                                       args.hybrid_steps, 1,
+                                       args.hybrid_cpu_sample,
+                                       not args.no_cpu, args.hybrid_algorithm)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, args.cpu_sample)
     rec.close()

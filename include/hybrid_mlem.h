@@ -144,6 +144,23 @@ int hybrid_mlem_get_weights(hybrid_mlem_ctx* ctx, float* weights);
  * 276-447; the CUDA loop body, cuda/reconstruction.cu:110-176). */
 int hybrid_mlem_iterate(hybrid_mlem_ctx* ctx, int n_iterations);
 
+/* How an iteration runs (no counterpart in the reference, whose CUDA
+ * kernel is one warp per event with global atomics,
+ * cuda/reconstruction/warp_granularity.cuh:28-173):
+ *   1: one wavefront per event: D_e from the full kernel per visit, the
+ *      update added to the image with float atomics;
+ *   2: LOR space: the frame sensitivity and pixel weight tabulated per
+ *      (info, plane) once (n_pixel_infos x n_planes floats, with as many
+ *      again for the back-projection's LOR-space sums), one workgroup per
+ *      LOR taking its events, their updates summed in LDS over the LOR's
+ *      infos x planes, then gathered per voxel over the pixel's infos;
+ *   0: automatic (default): 2 where its tables fit in device memory, the
+ *      largest LOR's infos x planes fit in LDS with room for two
+ *      wavefronts, and no event spans more than 256 infos, else 1.
+ * hybrid_mlem_get_algorithm returns the algorithm of the last iteration. */
+int hybrid_mlem_set_algorithm(hybrid_mlem_ctx* ctx, int algorithm);
+int hybrid_mlem_get_algorithm(hybrid_mlem_ctx* ctx);
+
 /* Device time of the last hybrid_mlem_iterate's event kernels, in ms. */
 int hybrid_mlem_last_iteration_ms(hybrid_mlem_ctx* ctx, float* ms);
 

@@ -405,6 +405,395 @@ __global__ __launch_bounds__(256) void event_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// LOR-space iteration
+// ---------------------------------------------------------------------------
+//
+// The frame sensitivity and the pixel weight of a visit depend on the info
+// (its pixel and LOR) and the plane, not on the event, so
+//   G(i, z) = pw_i / S(z, up_i; R_lor, L)
+// is tabulated once per geometry (n_infos x n_planes, plane-fastest), with
+// 0 where kernel2d would be skipped (S negative).  An event's kernel splits
+// into a per-info factor and a Gaussian in z:
+//   K_e(i, z) = A_ei exp(-iz b^2),  b = (z - right) - dy tan,
+//   A_ei = c / sqrt(a.S^-1.a) exp(-id (2 dy sec)^2 / 2),  dy = up_i - up
+// (gaussian_kernel.h:51-79 with b_ic_b = 2 iz b^2 + id (2 dy sec)^2).
+//   forward:  D_e = sum K_e G rho s + the extra info's visits
+//   backward: bp(i, z) = sum_e K_e(i, z) / D_e, accumulated per LOR in LDS
+//   gather:   rho'(p, z) = rho(p, z) sum_{i of pixel p} G(i, z) bp(i, z)
+//                          + extra(p, z)
+// The extra info past a LOR's own infos (the range's last, when it lies
+// beyond the LOR) is evaluated in full and added to extra(p, z) with
+// atomics; so is a repeated visit's update under the kernel-cache
+// semantics.
+
+constexpr int kEventInfoCap = 256;  // infos of one event held in LDS
+constexpr size_t kLdsBytes = 160 * 1024;
+constexpr size_t kWaveLdsBytes = 2 * kEventInfoCap * sizeof(float);
+
+// A_ei and dy tan of one info of an event.
+__device__ inline void info_factors(const Kernel& k, const float4& ef,
+                                    float R, float up_i, float& A,
+                                    float& t) {
+  const float y = ef.x, tan = ef.z, sec = ef.w;
+  const float dy = up_i - y;
+  const float sec_sq = sec * sec;
+  const float yp = dy + y;
+  const float ap = -(yp - R) * sec_sq;
+  const float aq = -(yp + R) * sec_sq;
+  const float ar = -2.0f * yp * sec * tan;
+  const float a_ic_a = ap * k.iz * ap + aq * k.iz * aq + ar * k.id * ar;
+  const float br = -2.0f * dy * sec;
+  A = (k.norm_c / sqrtf(a_ic_a)) * expf(-0.5f * (br * k.id * br));
+  t = dy * tan;
+}
+
+__device__ inline float plane_centre(const Grid& g, int plane) {
+#pragma clang fp contract(off)
+  return ((float)plane + 0.5f) * g.pixel_size + g.z_left;
+}
+
+// info -> LOR, and per-info pixel index (-1 outside the grid) and up, the
+// projection of the pixel centre on its LOR relative to the middle.
+__global__ __launch_bounds__(256) void info_lor_kernel(
+    Grid g, const float4* __restrict__ seg_a, const int2* __restrict__ ranges,
+    const int* __restrict__ pix, int64_t n_lors, int* __restrict__ info_lor,
+    int* __restrict__ ipix, float* __restrict__ upi) {
+  const int lane = threadIdx.x & 63;
+  const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+  for (int64_t l = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       l < n_lors; l += n_waves) {
+    const int2 r = ranges[l];
+    const float4 sa = seg_a[l];
+    for (int i = r.x + lane; i < r.y; i += 64) {
+      const int p = pix[i];
+      const int x = pixel_x(p), y = pixel_y(p);
+      float cx, cy;
+      pixel_centre(g, x, y, cx, cy);
+      float up;
+      {
+#pragma clang fp contract(off)
+        up = (cx - sa.x) * sa.z + (cy - sa.y) * sa.w;
+      }
+      info_lor[i] = (int)l;
+      ipix[i] = in_grid(g, x, y) ? y * g.n_columns + x : -1;
+      upi[i] = up;
+    }
+  }
+}
+
+// G(i, z) for every info of a LOR (0 for the others).
+__global__ __launch_bounds__(256) void frame_table_kernel(
+    Grid g, Kernel k, const float4* __restrict__ seg_b,
+    const int* __restrict__ info_lor, const int* __restrict__ ipix,
+    const float* __restrict__ upi, const float* __restrict__ w, int n_infos,
+    float* __restrict__ G) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)n_infos * g.n_planes) return;
+  const int i = (int)(idx / g.n_planes);
+  const int z = (int)(idx - (int64_t)i * g.n_planes);
+  const int l = info_lor[i];
+  float v = 0.0f;
+  if (l >= 0 && ipix[i] >= 0) {
+    const float S =
+        frame_sensitivity(plane_centre(g, z), upi[i], seg_b[l].w, k.length);
+    // kernel2d = un / S is skipped where <= 0 (S negative or -0)
+    v = signbit(S) ? 0.0f : w[i] / S;
+  }
+  G[idx] = v;
+}
+
+// Orders this wavefront's LDS accesses before and after (a wavefront's
+// LDS instructions complete in issue order; the fences keep the compiler
+// from moving them across).
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The (info, plane) walk of an event's box, plane-fastest: lane `lane`
+// starts at pair `lane` and steps 64 pairs at a time.
+struct BoxWalk {
+  int q, r, sq, sr, n_pl;
+  __device__ BoxWalk(int lane, int n_pl_) : n_pl(n_pl_) {
+    sq = n_pl > 0 ? 64 / n_pl : 0;
+    sr = 64 - sq * n_pl;
+    q = n_pl > 0 ? lane / n_pl : 0;
+    r = lane - q * n_pl;
+  }
+  __device__ void step() {
+    q += sq;
+    r += sr;
+    if (r >= n_pl) {
+      r -= n_pl;
+      ++q;
+    }
+  }
+};
+
+// Forward projection in LOR space: one single-wave block per event
+// (grid-stride, events in LOR order).  D_e over the box from G rho s plus
+// the extra info in full; inv_out[e] = 1 / D_e (0 where D_e is not > 0);
+// the extra info's update (twice, for the visit it repeats, under the
+// kernel-cache semantics) goes to the image-space extra sums.
+template <bool LAST_WRITE>
+__global__ __launch_bounds__(64) void lor_forward_kernel(
+    Grid g, Kernel k, float kexp, const float4* __restrict__ seg_a,
+    const float4* __restrict__ seg_b, const int2* __restrict__ ranges,
+    const int* __restrict__ pix, const int* __restrict__ ipix,
+    const float* __restrict__ upi, const float* __restrict__ w,
+    const float* __restrict__ sens, const float* __restrict__ G,
+    const float* __restrict__ rho, const int4* __restrict__ ev_i,
+    const float4* __restrict__ ev_f, const int* __restrict__ ev_dup,
+    int64_t n_events, float* __restrict__ inv_out,
+    float* __restrict__ extra) {
+  __shared__ float sA[kEventInfoCap], sT[kEventInfoCap];
+  __shared__ int sP[kEventInfoCap];
+  const int lane = threadIdx.x;
+  const int64_t np = g.n_planes;
+  for (int64_t e = blockIdx.x; e < n_events; e += gridDim.x) {
+    const int4 ei = ev_i[e];
+    const float4 ef = ev_f[e];
+    const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+    const int n_pl = pe - pb;
+    const int2 lr = ranges[ei.x];
+    const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+    const int n_info = ie_l - ei.y;
+    const float R = seg_b[ei.x].w;
+    __syncthreads();  // the previous event's factors are no longer read
+    for (int t = lane; t < n_info; t += 64) {
+      const int i = ei.y + t;
+      float A, T;
+      info_factors(k, ef, R, upi[i], A, T);
+      const int p = ipix[i];
+      sA[t] = p >= 0 ? A * sens[p] : 0.0f;
+      sT[t] = T;
+      sP[t] = p;
+    }
+    __syncthreads();
+    float den0 = 0.0f, den1 = 0.0f;
+    const int total = n_info * (n_pl > 0 ? n_pl : 0);
+    {
+      // two pairs per step keep two gathers of G and rho in flight
+      BoxWalk a(lane, n_pl), b(lane, n_pl);
+      b.step();
+      for (int idx = lane; idx < total; idx += 128) {
+        {
+          const int p = sP[a.q];
+          if (p >= 0) {
+            const int z = pb + a.r;
+            const float d = (plane_centre(g, z) - ef.y) - sT[a.q];
+            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+            den0 += sA[a.q] * K * G[(int64_t)(ei.y + a.q) * np + z] *
+                    rho[(int64_t)p * np + z];
+          }
+        }
+        if (idx + 64 < total) {
+          const int p = sP[b.q];
+          if (p >= 0) {
+            const int z = pb + b.r;
+            const float d = (plane_centre(g, z) - ef.y) - sT[b.q];
+            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+            den1 += sA[b.q] * K * G[(int64_t)(ei.y + b.q) * np + z] *
+                    rho[(int64_t)p * np + z];
+          }
+        }
+        a.step();
+        a.step();
+        b.step();
+        b.step();
+      }
+    }
+    float den = den0 + den1;
+    const bool has_extra = ei.z > lr.y && n_pl > 0;
+    int xx = 0, yy = 0, xpixel = -1;
+    float4 sa = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (has_extra) {
+      const int pxy = pix[ei.z - 1];
+      xx = pixel_x(pxy);
+      yy = pixel_y(pxy);
+      if (in_grid(g, xx, yy)) xpixel = yy * g.n_columns + xx;
+      sa = seg_a[ei.x];
+    }
+    if (xpixel >= 0) {
+      for (int z = pb + lane; z < pe; z += 64) {
+        const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
+        if (!(kv <= 0.0f))
+          den += kv * w[ei.z - 1] * rho[(int64_t)xpixel * np + z] *
+                 sens[xpixel];
+      }
+    }
+    den = strip::wave_sum(den);
+    const float inv = den > 0.0f ? 1.0f / den : 0.0f;
+    if (lane == 0) inv_out[e] = inv;
+    if (xpixel >= 0 && inv > 0.0f) {
+      const float mult = (LAST_WRITE && ev_dup[e] >= 0) ? 2.0f : 1.0f;
+      for (int z = pb + lane; z < pe; z += 64) {
+        const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
+        if (!(kv <= 0.0f)) {
+          const int64_t v = (int64_t)xpixel * np + z;
+          atomicAdd(&extra[v], mult * (kv * w[ei.z - 1] * rho[v] * inv));
+        }
+      }
+    }
+  }
+}
+
+// Back-projection in LOR space: one block per LOR with events, its waves
+// taking the LOR's events in turn (the next event's parameters loaded while
+// the current one is summed); K_e / D_e added into the block's LDS slab
+// (the LOR's infos x all planes, laid out as its rows of bp), which is
+// written to bp at the end.  Under the kernel-cache semantics the visit
+// that the extra info repeats is left out (its update went to the extra
+// sums with the extra info's weight).
+template <bool LAST_WRITE>
+__global__ __launch_bounds__(1024) void lor_backward_kernel(
+    Grid g, Kernel k, float kexp, const float4* __restrict__ seg_b,
+    const int2* __restrict__ ranges, const int* __restrict__ ipix,
+    const float* __restrict__ upi, const int4* __restrict__ ev_i,
+    const float4* __restrict__ ev_f, const int* __restrict__ ev_dup,
+    const float* __restrict__ inv, const int* __restrict__ lor_events,
+    float* __restrict__ bp) {
+  extern __shared__ float lds[];
+  const int lor = blockIdx.x;
+  const int e0 = lor_events[lor], e1 = lor_events[lor + 1];
+  if (e0 == e1) return;  // the rows of a LOR without events stay zero
+  const int2 lr = ranges[lor];
+  const int n_lor = lr.y - lr.x;
+  const int np = g.n_planes;
+  const int n_waves = blockDim.x >> 6;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float* slab = lds;
+  float* sA = lds + (size_t)n_lor * np + (size_t)wave * 2 * kEventInfoCap;
+  float* sT = sA + kEventInfoCap;
+  for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) slab[t] = 0.0f;
+  __syncthreads();
+  const float R = seg_b[lor].w;
+  int e = e0 + wave;
+  int4 ei = make_int4(0, 0, 0, 0);
+  float4 ef = make_float4(0.f, 0.f, 0.f, 0.f);
+  float iv = 0.0f;
+  int dup = -1;
+  if (e < e1) {
+    ei = ev_i[e];
+    ef = ev_f[e];
+    iv = inv[e];
+    if (LAST_WRITE) dup = ev_dup[e];
+  }
+  while (e < e1) {
+    const int en = e + n_waves;
+    int4 ein = make_int4(0, 0, 0, 0);
+    float4 efn = make_float4(0.f, 0.f, 0.f, 0.f);
+    float ivn = 0.0f;
+    int dupn = -1;
+    if (en < e1) {
+      ein = ev_i[en];
+      efn = ev_f[en];
+      ivn = inv[en];
+      if (LAST_WRITE) dupn = ev_dup[en];
+    }
+    if (iv > 0.0f) {
+      const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+      const int n_pl = pe - pb;
+      const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+      const int n_info = ie_l - ei.y;
+      for (int t = lane; t < n_info; t += 64) {
+        float A, T;
+        info_factors(k, ef, R, upi[ei.y + t], A, T);
+        sA[t] = ipix[ei.y + t] >= 0 && !(LAST_WRITE && ei.y + t == dup)
+                    ? A * iv
+                    : 0.0f;
+        sT[t] = T;
+      }
+      wave_lds_sync();
+      const int total = n_info * (n_pl > 0 ? n_pl : 0);
+      BoxWalk a(lane, n_pl);
+      for (int idx = lane; idx < total; idx += 64) {
+        const float f = sA[a.q];
+        if (f != 0.0f) {
+          const int z = pb + a.r;
+          const float d = (plane_centre(g, z) - ef.y) - sT[a.q];
+          const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+          atomicAdd(&slab[(ei.y + a.q - lr.x) * np + z], f * K);
+        }
+        a.step();
+      }
+      wave_lds_sync();  // this event's factors are read before the next's
+    }
+    e = en;
+    ei = ein;
+    ef = efn;
+    iv = ivn;
+    dup = dupn;
+  }
+  __syncthreads();
+  float* rows = bp + (int64_t)lr.x * np;
+  for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) rows[t] = slab[t];
+}
+
+// rho'(p, z) = rho(p, z) sum_i G(i, z) bp(i, z) + extra(p, z) over the
+// infos of pixel p (in info order): one wavefront per pixel, lanes over
+// planes.
+__global__ __launch_bounds__(256) void lor_gather_kernel(
+    const int* __restrict__ offsets, const int* __restrict__ order,
+    const float* __restrict__ G, const float* __restrict__ bp,
+    const float* __restrict__ rho, const float* __restrict__ extra,
+    int n_pixels, int n_planes, float* __restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int n_waves = gridDim.x * (blockDim.x >> 6);
+  const int64_t np = n_planes;
+  for (int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+       p < n_pixels; p += n_waves) {
+    const int k0 = offsets[p], k1 = offsets[p + 1];
+    for (int z = lane; z < n_planes; z += 64) {
+      float acc = 0.0f;
+      for (int k = k0; k < k1; ++k) {
+        const int64_t i = order[k];
+        acc += G[i * np + z] * bp[i * np + z];
+      }
+      const int64_t v = (int64_t)p * np + z;
+      out[v] = rho[v] * acc + extra[v];
+    }
+  }
+}
+
+// Plan: LOR keys of the events, per-LOR event offsets, the events whose
+// range has an extra info, and the largest per-event info count.
+__global__ __launch_bounds__(256) void event_keys_kernel(
+    const int4* __restrict__ ev_i, int64_t n, unsigned* __restrict__ keys,
+    int* __restrict__ idx) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  keys[e] = (unsigned)ev_i[e].x;
+  idx[e] = (int)e;
+}
+
+__global__ __launch_bounds__(256) void permute_events_kernel(
+    const int* __restrict__ order, int64_t n, const int4* __restrict__ a_i,
+    const float4* __restrict__ a_f, const int* __restrict__ a_dup,
+    const int2* __restrict__ ranges, int4* __restrict__ b_i,
+    float4* __restrict__ b_f, int* __restrict__ b_dup,
+    int* __restrict__ lor_count, int* __restrict__ max_infos) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int src = order[e];
+  const int4 ei = a_i[src];
+  b_i[e] = ei;
+  b_f[e] = a_f[src];
+  b_dup[e] = a_dup[src];
+  atomicAdd(&lor_count[ei.x], 1);
+  const int2 lr = ranges[ei.x];
+  const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+  atomicMax(max_infos, ie_l - ei.y);
+}
+
+__global__ __launch_bounds__(256) void iota_kernel(int* __restrict__ p,
+                                                   int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int)i;
+}
+
 // Reference voxel order (z n_rows + y) n_columns + x <-> plane-fastest
 // pixel n_planes + z.
 __global__ __launch_bounds__(256) void to_device_order_kernel(
@@ -475,12 +864,40 @@ struct hybrid_mlem_ctx {
   float* stage = nullptr;  // reference-order staging
   Events ev;
   float last_ms = 0.0f;
+  // per-info LOR, pixel index and up; infos sorted by pixel (CSR)
+  int* info_lor = nullptr;
+  int* ipix = nullptr;
+  float* upi = nullptr;
+  int* csr_offsets = nullptr;
+  int* csr_order = nullptr;
+  int max_lor_infos = 0;
+  // LOR-space tables (n_infos x n_planes) and the image-space extra sums
+  float* G = nullptr;
+  float* bp = nullptr;
+  float* extra = nullptr;
+  bool tables = false;
+  int lor_waves = 0;  // waves per block of the LOR-space event kernel
+  size_t lor_lds = 0;
+  // algorithm: 0 automatic, 1 wavefront per event with image atomics,
+  // 2 LOR space
+  int algorithm_request = 0;
+  int algorithm = 1;
+  // plan of the LOR-space iteration: events sorted by LOR
+  bool plan_valid = false;
+  int4* pl_i = nullptr;
+  float4* pl_f = nullptr;
+  int* pl_dup = nullptr;
+  float* inv = nullptr;
+  int* lor_events = nullptr;
+  int max_event_infos = 0;
+  int64_t pl_capacity = 0;
 
   ~hybrid_mlem_ctx() {
     if (stream) (void)hipStreamSynchronize(stream);
     void* ptrs[] = {segments, widths, seg_a, seg_b, ranges, pix, pos,
                     w_given, w, sens, rho, out, stage, ev.ev_i, ev.ev_f,
-                    ev.ev_dup};
+                    ev.ev_dup, info_lor, ipix, upi, csr_offsets, csr_order,
+                    G, bp, extra, pl_i, pl_f, pl_dup, inv, lor_events};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (t0) (void)hipEventDestroy(t0);
@@ -545,47 +962,45 @@ int grow_events(hybrid_mlem_ctx* c, int64_t need) {
   return STRIP_MLEM_OK;
 }
 
-// calculate_sensitivity on the device: the infos inside the grid sorted
-// stably by pixel (radix sort of (pixel, info) pairs), per-pixel offsets
-// from a count and a scan, then one thread per pixel sums its weights in
-// info order, the order of the reference's loop.
-int sensitivity_from_weights(hybrid_mlem_ctx* c) {
+// Infos inside the grid sorted stably by pixel (radix sort of (pixel,
+// info) pairs) with per-pixel offsets: the order of calculate_sensitivity's
+// per-pixel sums (reconstruction.h:240-251) and the gather's index.
+int build_pixel_index(hybrid_mlem_ctx* c) {
   const Grid& g = c->grid;
   const int n = c->n_infos;
+  int rc = STRIP_MLEM_OK;
+  if ((rc = device_alloc(&c->csr_offsets, (size_t)g.n_pixels + 1)) ||
+      (rc = device_alloc(&c->csr_order, n)))
+    return rc;
   if (n == 0) {
-    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(g.n_pixels)), dim3(256),
-                       0, c->stream, c->sens, (int64_t)g.n_pixels, 0.0f);
-    HY_TRY(hipGetLastError());
+    HY_TRY(hipMemsetAsync(c->csr_offsets, 0,
+                          sizeof(int) * ((size_t)g.n_pixels + 1), c->stream));
     return STRIP_MLEM_OK;
   }
-  int *count = nullptr, *offsets = nullptr;
+  int* count = nullptr;
   unsigned *keys = nullptr, *keys_sorted = nullptr;
-  int *order = nullptr, *iota = nullptr;
+  int* iota = nullptr;
   char* tmp = nullptr;
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
-    void* ps[] = {count, offsets, keys, keys_sorted, order, iota, tmp};
+    void* ps[] = {count, keys, keys_sorted, iota, tmp};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   };
-  int rc = STRIP_MLEM_OK;
   size_t sort_bytes = 0, scan_bytes = 0;
-  hipError_t e = hipSuccess;
-  std::vector<int> seq(n);
-  for (int i = 0; i < n; ++i) seq[i] = i;
   if ((rc = device_alloc(&count, (size_t)g.n_pixels + 1)) ||
-      (rc = device_alloc(&offsets, (size_t)g.n_pixels + 1)) ||
       (rc = device_alloc(&keys, n)) || (rc = device_alloc(&keys_sorted, n)) ||
-      (rc = device_alloc(&order, n)) || (rc = device_alloc(&iota, n))) {
+      (rc = device_alloc(&iota, n))) {
     cleanup();
     return rc;
   }
-  e = hipcub::DeviceRadixSort::SortPairs(nullptr, sort_bytes, keys,
-                                         keys_sorted, iota, order, n, 0, 32,
-                                         c->stream);
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+      nullptr, sort_bytes, keys, keys_sorted, iota, c->csr_order, n, 0, 32,
+      c->stream);
   if (e == hipSuccess)
-    e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count, offsets,
-                                         g.n_pixels + 1, c->stream);
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, count,
+                                         c->csr_offsets, g.n_pixels + 1,
+                                         c->stream);
   if (e == hipSuccess &&
       (rc = device_alloc(&tmp, std::max(sort_bytes, scan_bytes)))) {
     cleanup();
@@ -594,31 +1009,170 @@ int sensitivity_from_weights(hybrid_mlem_ctx* c) {
   if (e == hipSuccess)
     e = hipMemsetAsync(count, 0, sizeof(int) * ((size_t)g.n_pixels + 1),
                        c->stream);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(iota, seq.data(), sizeof(int) * n,
-                       hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) {
+    hipLaunchKernelGGL(iota_kernel, dim3(blocks_for(n)), dim3(256), 0,
+                       c->stream, iota, (int64_t)n);
     hipLaunchKernelGGL(pixel_count_kernel, dim3(blocks_for(n)), dim3(256), 0,
                        c->stream, g, c->pix, (int64_t)n, count, keys);
     e = hipGetLastError();
   }
   if (e == hipSuccess)
     e = hipcub::DeviceRadixSort::SortPairs(tmp, sort_bytes, keys, keys_sorted,
-                                           iota, order, n, 0, 32, c->stream);
+                                           iota, c->csr_order, n, 0, 32,
+                                           c->stream);
   if (e == hipSuccess)
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, count, offsets,
-                                         g.n_pixels + 1, c->stream);
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(sensitivity_sum_kernel, dim3(blocks_for(g.n_pixels)),
-                       dim3(256), 0, c->stream, offsets, order, c->w,
-                       g.n_pixels, c->sens);
-    e = hipGetLastError();
-  }
+    e = hipcub::DeviceScan::ExclusiveSum(tmp, scan_bytes, count,
+                                         c->csr_offsets, g.n_pixels + 1,
+                                         c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   cleanup();
   if (e != hipSuccess)
     return strip::fail(STRIP_MLEM_ERR_HIP,
-                       std::string("sensitivity: ") + hipGetErrorString(e));
+                       std::string("pixel index: ") + hipGetErrorString(e));
+  return STRIP_MLEM_OK;
+}
+
+// LOR-space tables: G (recomputed with every weight preparation) and bp
+// (rows of LORs without events stay zero), the extra accumulator, and the
+// back-projection's block: the largest LOR's slab (infos x planes) plus
+// 2 KiB of per-event factors per wave, within 160 KiB of LDS, 2 to 16
+// waves.
+// Where that does not fit, or the tables do not fit in device memory, the
+// context stays on the wavefront-per-event algorithm.
+int prepare_tables(hybrid_mlem_ctx* c) {
+  const Grid& g = c->grid;
+  const size_t slab = sizeof(float) * (size_t)c->max_lor_infos * g.n_planes;
+  const int waves =
+      slab < kLdsBytes ? (int)std::min<size_t>(16, (kLdsBytes - slab) /
+                                                       kWaveLdsBytes)
+                       : 0;
+  if (waves < 2) {
+    c->tables = false;
+    return STRIP_MLEM_OK;
+  }
+  c->lor_waves = waves;
+  c->lor_lds = slab + waves * kWaveLdsBytes;
+  const int64_t cells = (int64_t)c->n_infos * g.n_planes;
+  if (!c->G) {
+    if (device_alloc(&c->G, cells) != STRIP_MLEM_OK ||
+        device_alloc(&c->bp, cells) != STRIP_MLEM_OK ||
+        device_alloc(&c->extra, c->n_voxels()) != STRIP_MLEM_OK) {
+      if (c->G) (void)hipFree(c->G);
+      if (c->bp) (void)hipFree(c->bp);
+      c->G = c->bp = nullptr;
+      c->tables = false;
+      return STRIP_MLEM_OK;
+    }
+    HY_TRY(hipMemsetAsync(c->bp, 0, sizeof(float) * cells, c->stream));
+    HY_TRY(hipFuncSetAttribute(
+        (const void*)lor_backward_kernel<true>,
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+    HY_TRY(hipFuncSetAttribute(
+        (const void*)lor_backward_kernel<false>,
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
+  }
+  if (cells)
+    hipLaunchKernelGGL(frame_table_kernel, dim3(blocks_for(cells)),
+                       dim3(256), 0, c->stream, g, c->kernel, c->seg_b,
+                       c->info_lor, c->ipix, c->upi, c->w, c->n_infos, c->G);
+  HY_TRY(hipGetLastError());
+  c->tables = true;
+  return STRIP_MLEM_OK;
+}
+
+// Events sorted by LOR (a stable radix sort), per-LOR offsets and the
+// largest per-event info count within a LOR.  Rows of bp whose LOR loses
+// all its events are cleared.
+int build_plan(hybrid_mlem_ctx* c) {
+  const int64_t n = c->ev.n;
+  int rc = STRIP_MLEM_OK;
+  if (n > c->pl_capacity) {
+    void* ps[] = {c->pl_i, c->pl_f, c->pl_dup, c->inv};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+    c->pl_i = nullptr;
+    c->pl_f = nullptr;
+    c->pl_dup = nullptr;
+    c->inv = nullptr;
+    c->pl_capacity = 0;
+    if ((rc = device_alloc(&c->pl_i, n)) || (rc = device_alloc(&c->pl_f, n)) ||
+        (rc = device_alloc(&c->pl_dup, n)) || (rc = device_alloc(&c->inv, n)))
+      return rc;
+    c->pl_capacity = n;
+  }
+  if (!c->lor_events &&
+      (rc = device_alloc(&c->lor_events, (size_t)c->n_lors + 1)))
+    return rc;
+  unsigned *keys = nullptr, *keys_sorted = nullptr;
+  int *idx = nullptr, *order = nullptr, *count = nullptr, *stats = nullptr;
+  char* tmp = nullptr;
+  auto cleanup = [&]() {
+    (void)hipStreamSynchronize(c->stream);
+    void* ps[] = {keys, keys_sorted, idx, order, count, stats, tmp};
+    for (void* p : ps)
+      if (p) (void)hipFree(p);
+  };
+  const int64_t nn = n > 0 ? n : 1;
+  if ((rc = device_alloc(&keys, nn)) || (rc = device_alloc(&keys_sorted, nn)) ||
+      (rc = device_alloc(&idx, nn)) || (rc = device_alloc(&order, nn)) ||
+      (rc = device_alloc(&count, (size_t)c->n_lors + 1)) ||
+      (rc = device_alloc(&stats, 4))) {
+    cleanup();
+    return rc;
+  }
+  size_t b1 = 0, b2 = 0;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(
+      nullptr, b1, keys, keys_sorted, idx, order, (int)nn, 0, 32, c->stream);
+  if (e == hipSuccess)
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, b2, count, c->lor_events,
+                                         (int)(c->n_lors + 1), c->stream);
+  const size_t tb = std::max(b1, b2);
+  if (e == hipSuccess && (rc = device_alloc(&tmp, tb))) {
+    cleanup();
+    return rc;
+  }
+  if (e == hipSuccess)
+    e = hipMemsetAsync(count, 0, sizeof(int) * ((size_t)c->n_lors + 1),
+                       c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(stats, 0, sizeof(int) * 4, c->stream);
+  if (e == hipSuccess && n > 0) {
+    hipLaunchKernelGGL(event_keys_kernel, dim3(blocks_for(n)), dim3(256), 0,
+                       c->stream, c->ev.ev_i, n, keys, idx);
+    e = hipGetLastError();
+    size_t t1 = tb, t2 = tb;
+    if (e == hipSuccess)
+      e = hipcub::DeviceRadixSort::SortPairs(tmp, t1, keys, keys_sorted, idx,
+                                             order, (int)n, 0, 32, c->stream);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(permute_events_kernel, dim3(blocks_for(n)),
+                         dim3(256), 0, c->stream, order, n, c->ev.ev_i,
+                         c->ev.ev_f, c->ev.ev_dup, c->ranges, c->pl_i,
+                         c->pl_f, c->pl_dup, count, stats);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+      e = hipcub::DeviceScan::ExclusiveSum(tmp, t2, count, c->lor_events,
+                                           (int)(c->n_lors + 1), c->stream);
+  } else if (e == hipSuccess) {
+    e = hipMemsetAsync(c->lor_events, 0,
+                       sizeof(int) * ((size_t)c->n_lors + 1), c->stream);
+  }
+  // a LOR whose events are gone keeps no stale rows
+  if (e == hipSuccess && c->bp)
+    e = hipMemsetAsync(c->bp, 0,
+                       sizeof(float) * (size_t)c->n_infos * c->grid.n_planes,
+                       c->stream);
+  int host_stats[4] = {0, 0, 0, 0};
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host_stats, stats, sizeof(int) * 4,
+                       hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  cleanup();
+  if (e != hipSuccess)
+    return strip::fail(STRIP_MLEM_ERR_HIP,
+                       std::string("event plan: ") + hipGetErrorString(e));
+  c->max_event_infos = host_stats[0];
+  c->plan_valid = true;
   return STRIP_MLEM_OK;
 }
 
@@ -769,6 +1323,26 @@ int hybrid_mlem_create(const hybrid_mlem_geometry* G, int semantics,
   if (he != hipSuccess || hipGetLastError() != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return bail(strip::fail(STRIP_MLEM_ERR_HIP, "geometry setup"));
+  for (int64_t l = 0; l < n_lors; ++l)
+    c->max_lor_infos = std::max(c->max_lor_infos, ranges[l].y - ranges[l].x);
+  if ((rc = device_alloc(&c->info_lor, n_infos)) ||
+      (rc = device_alloc(&c->ipix, n_infos)) ||
+      (rc = device_alloc(&c->upi, n_infos)))
+    return bail(rc);
+  if (n_infos) {
+    // infos outside every LOR: LOR -1, pixel -1, up 0
+    if (hipMemsetAsync(c->info_lor, 0xff, sizeof(int) * n_infos, c->stream) ||
+        hipMemsetAsync(c->ipix, 0xff, sizeof(int) * n_infos, c->stream) ||
+        hipMemsetAsync(c->upi, 0, sizeof(float) * n_infos, c->stream))
+      return bail(strip::fail(STRIP_MLEM_ERR_HIP, "per-info arrays"));
+    const int64_t waves = std::min<int64_t>(n_lors, 65536);
+    hipLaunchKernelGGL(info_lor_kernel, dim3(blocks_for(waves, 4)), dim3(256),
+                       0, c->stream, g, c->seg_a, c->ranges, c->pix, n_lors,
+                       c->info_lor, c->ipix, c->upi);
+    if (hipGetLastError() != hipSuccess)
+      return bail(strip::fail(STRIP_MLEM_ERR_HIP, "per-info arrays"));
+  }
+  if ((rc = build_pixel_index(c))) return bail(rc);
   *out = c;
   return STRIP_MLEM_OK;
 }
@@ -806,14 +1380,17 @@ int hybrid_mlem_prepare_weights(hybrid_mlem_ctx* c, int calculate_weight,
     hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(g.n_pixels)), dim3(256),
                        0, c->stream, c->sens, (int64_t)g.n_pixels, 1.0f);
   } else {
-    // stable sort of the infos by pixel; each pixel then sums its weights
-    // in info order
-    HY_CHECK(sensitivity_from_weights(c));
+    // each pixel sums its weights in info order
+    hipLaunchKernelGGL(sensitivity_sum_kernel, dim3(blocks_for(g.n_pixels)),
+                       dim3(256), 0, c->stream, c->csr_offsets, c->csr_order,
+                       c->w, g.n_pixels, c->sens);
+    HY_TRY(hipGetLastError());
   }
   if (n)
     hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(n)), dim3(256), 0,
                        c->stream, g, c->pix, c->sens, (int64_t)n, c->w);
   HY_TRY(hipGetLastError());
+  HY_CHECK(prepare_tables(c));
   HY_TRY(hipStreamSynchronize(c->stream));
   c->prepared = true;
   return STRIP_MLEM_OK;
@@ -919,12 +1496,14 @@ int hybrid_mlem_add_responses(hybrid_mlem_ctx* c,
                        std::string("event compaction: ") +
                            hipGetErrorString(e));
   c->ev.n += host_flags[1];
+  c->plan_valid = false;
   return STRIP_MLEM_OK;
 }
 
 int hybrid_mlem_clear_events(hybrid_mlem_ctx* c) {
   if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
   c->ev.n = 0;
+  c->plan_valid = false;
   return STRIP_MLEM_OK;
 }
 
@@ -996,6 +1575,95 @@ int hybrid_mlem_get_weights(hybrid_mlem_ctx* c, float* weights) {
   return STRIP_MLEM_OK;
 }
 
+int hybrid_mlem_set_algorithm(hybrid_mlem_ctx* c, int algorithm) {
+  if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (algorithm < 0 || algorithm > 2)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "unknown algorithm");
+  c->algorithm_request = algorithm;
+  return STRIP_MLEM_OK;
+}
+
+int hybrid_mlem_get_algorithm(hybrid_mlem_ctx* c) {
+  if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  return c->algorithm;
+}
+
+namespace {
+
+// The algorithm the next iteration uses: the LOR-space one where its tables
+// exist and every event's infos fit the per-event LDS factors, else one
+// wavefront per event.
+int choose_algorithm(hybrid_mlem_ctx* c) {
+  if (c->algorithm_request == 1) {
+    c->algorithm = 1;
+    return STRIP_MLEM_OK;
+  }
+  if (!c->tables) {
+    if (c->algorithm_request == 2)
+      return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                         "LOR-space tables unavailable (memory or a LOR "
+                         "with too many infos for the LDS slab)");
+    c->algorithm = 1;
+    return STRIP_MLEM_OK;
+  }
+  if (!c->plan_valid) HY_CHECK(build_plan(c));
+  if (c->max_event_infos > kEventInfoCap) {
+    if (c->algorithm_request == 2)
+      return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
+                         "an event spans more than " +
+                             std::to_string(kEventInfoCap) + " infos");
+    c->algorithm = 1;
+    return STRIP_MLEM_OK;
+  }
+  c->algorithm = 2;
+  return STRIP_MLEM_OK;
+}
+
+int iterate_lor_space(hybrid_mlem_ctx* c) {
+  const Grid& g = c->grid;
+  const int64_t nv = c->n_voxels();
+  const float kexp = -c->kernel.iz * 1.44269504088896340736f;
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(nv)), dim3(256), 0,
+                     c->stream, c->extra, nv, 0.0f);
+  const int64_t n = c->ev.n;
+  if (n && c->n_lors) {
+    const bool cpu = c->semantics == HYBRID_MLEM_CPU;
+    const unsigned fblocks = (unsigned)std::min<int64_t>(n, 65536);
+    if (cpu)
+      hipLaunchKernelGGL(lor_forward_kernel<true>, dim3(fblocks), dim3(64), 0,
+                         c->stream, g, c->kernel, kexp, c->seg_a, c->seg_b,
+                         c->ranges, c->pix, c->ipix, c->upi, c->w, c->sens,
+                         c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup, n, c->inv,
+                         c->extra);
+    else
+      hipLaunchKernelGGL(lor_forward_kernel<false>, dim3(fblocks), dim3(64),
+                         0, c->stream, g, c->kernel, kexp, c->seg_a, c->seg_b,
+                         c->ranges, c->pix, c->ipix, c->upi, c->w, c->sens,
+                         c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup, n, c->inv,
+                         c->extra);
+    const dim3 grid((unsigned)c->n_lors), block(64 * c->lor_waves);
+    if (cpu)
+      hipLaunchKernelGGL(lor_backward_kernel<true>, grid, block, c->lor_lds,
+                         c->stream, g, c->kernel, kexp, c->seg_b, c->ranges,
+                         c->ipix, c->upi, c->pl_i, c->pl_f, c->pl_dup, c->inv,
+                         c->lor_events, c->bp);
+    else
+      hipLaunchKernelGGL(lor_backward_kernel<false>, grid, block, c->lor_lds,
+                         c->stream, g, c->kernel, kexp, c->seg_b, c->ranges,
+                         c->ipix, c->upi, c->pl_i, c->pl_f, c->pl_dup, c->inv,
+                         c->lor_events, c->bp);
+  }
+  const unsigned gblocks =
+      (unsigned)std::min<int64_t>((g.n_pixels + 3) / 4, 65536);
+  hipLaunchKernelGGL(lor_gather_kernel, dim3(gblocks), dim3(256), 0,
+                     c->stream, c->csr_offsets, c->csr_order, c->G, c->bp,
+                     c->rho, c->extra, g.n_pixels, g.n_planes, c->out);
+  HY_TRY(hipGetLastError());
+  return STRIP_MLEM_OK;
+}
+
+}  // namespace
+
 int hybrid_mlem_iterate(hybrid_mlem_ctx* c, int n_iterations) {
   if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
   if (n_iterations < 0)
@@ -1004,15 +1672,19 @@ int hybrid_mlem_iterate(hybrid_mlem_ctx* c, int n_iterations) {
     return strip::fail(STRIP_MLEM_ERR_STATE,
                        "hybrid_mlem_prepare_weights must come first");
   HY_TRY(hipSetDevice(c->device));
+  HY_CHECK(choose_algorithm(c));
   const int64_t nv = c->n_voxels();
   const int64_t n = c->ev.n;
   const unsigned blocks =
       (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, 16384));
   for (int it = 0; it < n_iterations; ++it) {
-    hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(nv)), dim3(256), 0,
-                       c->stream, c->out, nv, 0.0f);
+    if (c->algorithm == 1)
+      hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(nv)), dim3(256), 0,
+                         c->stream, c->out, nv, 0.0f);
     HY_TRY(hipEventRecord(c->t0, c->stream));
-    if (n) {
+    if (c->algorithm == 2) {
+      HY_CHECK(iterate_lor_space(c));
+    } else if (n) {
       if (c->semantics == HYBRID_MLEM_CPU)
         hipLaunchKernelGGL(event_kernel<true>, dim3(blocks), dim3(256), 0,
                            c->stream, c->grid, c->kernel, c->seg_a, c->seg_b,

@@ -170,6 +170,50 @@ def ring_geometry(n_detectors=48, ring_radius=0.4, lor_width=0.02,
                           begin, end)
 
 
+@dataclass(frozen=True)
+class HybridWorkload:
+    """A named hybrid reconstruction workload: ring geometry parameters,
+    the emission phantom of hybrid_events and the event count."""
+    name: str
+    n_detectors: int
+    ring_radius: float
+    lor_width: float
+    n_pixels: int
+    pixel_size: float
+    n_planes: int
+    fov_radius: float
+    sigma_z: float
+    sigma_dl: float
+    length: float
+    radius: float
+    half_z: float
+    n_events: int
+
+    def geometry(self):
+        return ring_geometry(self.n_detectors, self.ring_radius,
+                             self.lor_width, self.n_pixels, self.pixel_size,
+                             self.n_planes, self.fov_radius, self.sigma_z,
+                             self.sigma_dl, self.length)
+
+    def events(self, geo, n=None, seed=0):
+        return hybrid_events(geo, self.n_events if n is None else n, seed,
+                             radius=self.radius, half_z=self.half_z)
+
+
+WORKLOADS = {w.name: w for w in [
+    # the test ring: 48 detectors, tubes 2 cm wide
+    HybridWorkload("hybrid-ring48-80x80x80-5mm-5e4", 48, 0.4, 0.02, 80,
+                   0.005, 80, 0.2, 0.015, 0.06, 0.5, 0.12, 0.1, 50_000),
+    # a fully packed J-PET ring (config/full_ring.cfg: 384 detectors 7 mm
+    # wide at R 0.4278 m, 0.5 m long, FOV radius 0.4 m), 160 x 160 x 100
+    # voxels of 5 mm, sigma_z 1.5 cm, sigma_dl 6 cm (the command line's
+    # defaults), a 25 cm cylinder phantom, 1e7 events
+    HybridWorkload("hybrid-ring384-160x160x100-5mm-1e7", 384, 0.4278, 0.007,
+                   160, 0.005, 100, 0.4, 0.015, 0.06, 0.5, 0.25, 0.2,
+                   10_000_000),
+]}
+
+
 def hybrid_events(geo: HybridGeometry, n, seed=0, radius=0.12, half_z=0.1,
                   hot=(0.04, 0.03, 0.0, 0.03, 0.3), max_tan=None,
                   batch=1 << 18):
@@ -280,7 +324,8 @@ def exported_symbols():
             "hybrid_mlem_get_frame_events", "hybrid_mlem_set_rho",
             "hybrid_mlem_get_rho", "hybrid_mlem_get_sensitivity",
             "hybrid_mlem_get_weights", "hybrid_mlem_iterate",
-            "hybrid_mlem_last_iteration_ms", "hybrid_mlem_voxel_visits"]
+            "hybrid_mlem_last_iteration_ms", "hybrid_mlem_voxel_visits",
+            "hybrid_mlem_set_algorithm", "hybrid_mlem_get_algorithm"]
 
 
 def load_library():
@@ -312,6 +357,8 @@ def load_library():
         "hybrid_mlem_iterate": (ctypes.c_int, [P, ctypes.c_int]),
         "hybrid_mlem_last_iteration_ms": (ctypes.c_int, [P, fp]),
         "hybrid_mlem_voxel_visits": (ctypes.c_int, [P, ctypes.POINTER(i64)]),
+        "hybrid_mlem_set_algorithm": (ctypes.c_int, [P, ctypes.c_int]),
+        "hybrid_mlem_get_algorithm": (ctypes.c_int, [P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -451,6 +498,19 @@ class HybridReconstruction:
 
     def iterate(self, n=1):
         self._check(self._lib.hybrid_mlem_iterate(self._ctx, int(n)))
+
+    def set_algorithm(self, algorithm):
+        """0 automatic, 1 wavefront per event, 2 LOR space."""
+        self._check(self._lib.hybrid_mlem_set_algorithm(self._ctx,
+                                                        int(algorithm)))
+
+    @property
+    def algorithm(self):
+        """The algorithm of the last iteration (1 or 2)."""
+        a = self._lib.hybrid_mlem_get_algorithm(self._ctx)
+        if a < 0:
+            self._check(a)
+        return a
 
     def last_iteration_ms(self):
         ms = ctypes.c_float()

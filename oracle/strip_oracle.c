@@ -1,5 +1,6 @@
 /*
- * CPU oracle for the J-PET 2D strip list-mode TOF MLEM path.
+ * CPU oracle for the J-PET 2D strip list-mode TOF MLEM path and the 3D
+ * hybrid reconstruction built on its kernel.
  * TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py as the checker; the product never links it.
  *
@@ -18,7 +19,15 @@
  *     and the 5-point sensitivity in the denominator, the NaN of its
  *     unguarded 1/D, and the pixels its float-reciprocal row split visits;
  *   - the FP64 differentiation checker: central finite differences
- *     (tests/test_oracle_diff.py).
+ *     (tests/test_oracle_diff.py);
+ *   - the 3D hybrid restatement (hybrid_oracle_impl.inc): the reference's
+ *     own PET3D::Hybrid::Reconstruction built into oracle/_ref, on the same
+ *     geometry arrays and events (weights, sensitivity, kept events, images
+ *     including a repeated voxel visit); calculate_weight against an FP64
+ *     closed form; the event-count identity of one update
+ *     (tests/test_hybrid_oracle.py).  Its legacy (CUDA kernel) update is
+ *     pinned by equality with the cached one where no visit repeats and by
+ *     the closed-form ratio 2 w_x / (w_j + w_x) on the repeated voxel.
  */
 #define _GNU_SOURCE
 #include "strip_oracle.h"

@@ -31,9 +31,10 @@ def ring(hybrid_mod):
 
 
 def _rec(hybrid_mod, geo, semantics=0, calculate_weight=True, sens_mode=0,
-         sens=None):
+         sens=None, algorithm=0):
     rec = hybrid_mod.HybridReconstruction(geo, semantics)
     rec.prepare_weights(calculate_weight, sens_mode, sens)
+    rec.set_algorithm(algorithm)
     return rec
 
 
@@ -96,13 +97,18 @@ def test_frame_events_bit_exact(oracle_mod, hybrid_mod, ring):
     rec.close()
 
 
+@pytest.mark.parametrize("algorithm", [1, 2])
 @pytest.mark.parametrize("weights_mode,sens_mode", [(1, 0), (0, 1)])
-def test_mlem_parity(oracle_mod, hybrid_mod, ring, weights_mode, sens_mode):
-    print_case("hybrid MLEM", weights_mode, sens_mode)
+def test_mlem_parity(oracle_mod, hybrid_mod, ring, weights_mode, sens_mode,
+                     algorithm):
+    print_case("hybrid MLEM", weights_mode, sens_mode, "algorithm",
+               algorithm)
     ev = hybrid_mod.hybrid_events(ring, 50000, seed=12)
-    rec = _rec(hybrid_mod, ring, 0, weights_mode, sens_mode)
+    rec = _rec(hybrid_mod, ring, 0, weights_mode, sens_mode,
+               algorithm=algorithm)
     rec.add_responses(ev)
     rec.iterate(3)
+    assert rec.algorithm == algorithm
     gpu = rec.rho
     ref, n_used, _, _ = oracle_mod.hybrid_mlem(
         ring, ev, iterations=3, weights_mode=weights_mode,
@@ -124,17 +130,20 @@ def test_mlem_parity(oracle_mod, hybrid_mod, ring, weights_mode, sens_mode):
     rec.close()
 
 
+@pytest.mark.parametrize("algorithm", [1, 2])
 @pytest.mark.parametrize("semantics", [0, 1])
-def test_repeated_visit(oracle_mod, hybrid_mod, ring, semantics):
+def test_repeated_visit(oracle_mod, hybrid_mod, ring, semantics, algorithm):
     from hybrid_cases import repeated_pixel_events, repeated_pixel_geometry
-    print_case("hybrid repeated voxel visit, semantics", semantics)
+    print_case("hybrid repeated voxel visit, semantics", semantics,
+               "algorithm", algorithm)
     geo, li, first, second = repeated_pixel_geometry(ring)
     ev = np.concatenate([
         repeated_pixel_events(geo, li, first, second, 300, seed=5),
         hybrid_mod.hybrid_events(geo, 3000, seed=6)])
-    rec = _rec(hybrid_mod, geo, semantics)
+    rec = _rec(hybrid_mod, geo, semantics, algorithm=algorithm)
     rec.add_responses(ev)
     rec.iterate(2)
+    assert rec.algorithm == algorithm
     gpu = rec.rho
     ref = oracle_mod.hybrid_mlem(geo, ev, iterations=2,
                                  legacy=bool(semantics))[0]
@@ -147,12 +156,14 @@ def test_repeated_visit(oracle_mod, hybrid_mod, ring, semantics):
     rec.close()
 
 
-def test_start_image_and_layout(oracle_mod, pystrip_mod, hybrid_mod, ring):
-    print_case("hybrid start image")
+@pytest.mark.parametrize("algorithm", [1, 2])
+def test_start_image_and_layout(oracle_mod, pystrip_mod, hybrid_mod, ring,
+                                algorithm):
+    print_case("hybrid start image, algorithm", algorithm)
     rng = np.random.default_rng(9)
     rho0 = rng.uniform(0.5, 2.0, ring.shape).astype(np.float32)
     ev = hybrid_mod.hybrid_events(ring, 20000, seed=13)
-    rec = _rec(hybrid_mod, ring)
+    rec = _rec(hybrid_mod, ring, algorithm=algorithm)
     rec.rho = rho0
     assert np.array_equal(rec.rho, rho0)
     rec.add_responses(ev)
@@ -169,18 +180,44 @@ def test_start_image_and_layout(oracle_mod, pystrip_mod, hybrid_mod, ring):
     rec.close()
 
 
-def test_update_preserves_event_count_at_scale(hybrid_mod, ring):
+@pytest.mark.parametrize("algorithm", [1, 2])
+def test_update_preserves_event_count_at_scale(hybrid_mod, ring, algorithm):
     # from rho = 1, one update spreads each event's unit over its voxels in
     # proportion to weight x sensitivity: sum_v rho'(v) s(pixel(v)) = number
     # of events (none of these repeats a voxel), at 2e6 events
-    print_case("hybrid mass identity, 2e6 events")
+    print_case("hybrid mass identity, 2e6 events, algorithm", algorithm)
     ev = hybrid_mod.hybrid_events(ring, 2_000_000, seed=14)
-    rec = _rec(hybrid_mod, ring)
+    rec = _rec(hybrid_mod, ring, algorithm=algorithm)
     rec.add_responses(ev)
     rec.iterate(1)
+    assert rec.algorithm == algorithm
     mass = (rec.rho.astype(np.float64) *
             rec.sensitivity.astype(np.float64)[None]).sum()
     print("events", rec.n_events, "mass", mass, "visits", rec.voxel_visits(),
           "event kernel ms", rec.last_iteration_ms(), flush=True)
     assert abs(mass - rec.n_events) <= 1e-4 * rec.n_events
     rec.close()
+
+
+def test_full_ring_workload(oracle_mod, hybrid_mod):
+    # the bench workload's geometry (384 detectors, 9.2e6 pixel infos,
+    # 160 x 160 x 100 voxels) with 2e5 of its events: both algorithms
+    # against the oracle
+    w = hybrid_mod.WORKLOADS["hybrid-ring384-160x160x100-5mm-1e7"]
+    print_case("hybrid full ring", w.name, "2e5 events")
+    geo = w.geometry()
+    ev = w.events(geo, 200_000, seed=15)
+    ref, n_used, _, _ = oracle_mod.hybrid_mlem(geo, ev, iterations=2)
+    images = {}
+    for algorithm in (1, 2):
+        rec = _rec(hybrid_mod, geo, algorithm=algorithm)
+        rec.add_responses(ev)
+        rec.iterate(2)
+        assert rec.algorithm == algorithm and rec.n_events == n_used
+        images[algorithm] = rec.rho
+        e = oracle_mod.rel_l2(images[algorithm], ref)
+        print("algorithm", algorithm, "rel-L2", e, "event kernels ms",
+              rec.last_iteration_ms(), flush=True)
+        assert e <= 1e-4
+        rec.close()
+    assert oracle_mod.rel_l2(images[1], images[2]) <= 1e-4
