@@ -105,7 +105,13 @@ void hybrid_mlem_destroy(hybrid_mlem_ctx* ctx);
  *     floats, row-major, the --sensitivity file);
  * then every weight is divided by its pixel's sensitivity
  * (normalize_geometry_weights, reconstruction.h:253-271).  Sums run in info
- * order per pixel, as the reference's loop does. */
+ * order per pixel, as the reference's loop does.
+ *   sensitivity_mode 3 (HYBRID_MLEM_SENSITIVITY_NORMALISED): `sensitivity`
+ *     as given and the weights given to hybrid_mlem_create taken as already
+ *     normalised (no division): the state in which the reference's CLI
+ *     hands geometry and sensitivity to its GPU run
+ *     (reconstruction_cmd.cpp:596-600). */
+#define HYBRID_MLEM_SENSITIVITY_NORMALISED 3
 int hybrid_mlem_prepare_weights(hybrid_mlem_ctx* ctx, int calculate_weight,
                                 int sensitivity_mode,
                                 const float* sensitivity);
@@ -121,6 +127,30 @@ int hybrid_mlem_add_responses(hybrid_mlem_ctx* ctx,
                               int64_t n_responses);
 int hybrid_mlem_clear_events(hybrid_mlem_ctx* ctx);
 int64_t hybrid_mlem_n_events(const hybrid_mlem_ctx* ctx);
+
+/* A frame event as the reference's event reader leaves it (FrameEvent,
+ * reconstruction.h:53-63): LOR, up, right, tan, sec, pixel-info range and
+ * plane range. */
+typedef struct hybrid_mlem_frame_event {
+  int16_t first, second;
+  float up, right, tan, sec;
+  int64_t info_begin, info_end;
+  int16_t plane_begin, plane_end;
+} hybrid_mlem_frame_event;
+
+/* Replaces the events held by frame events computed elsewhere (the
+ * reference's GPU run receives them from its CPU reader,
+ * cuda/reconstruction.cu:36-57); no filter is applied.  An info end one
+ * past the arrays (the reference's upper_bound + 1 on the last LOR with
+ * infos) stops at the arrays' end; a LOR, info or plane range outside the
+ * geometry is an argument error. */
+int hybrid_mlem_set_frame_events(hybrid_mlem_ctx* ctx,
+                                 const hybrid_mlem_frame_event* events,
+                                 int64_t n_events);
+
+/* Name of the context's device (the run's info callback,
+ * cuda/reconstruction.cu:69-71). */
+int hybrid_mlem_device_name(hybrid_mlem_ctx* ctx, char* name, int size);
 
 /* Frame events (FrameEvent, reconstruction.h:53-63) of the events held:
  * per event lor index, info begin, info end, plane begin | plane end << 16

@@ -16,7 +16,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -27,6 +29,13 @@
 namespace strip {
 int fail(int code, const std::string& message);
 }
+
+// the C ABI's records, as the binding lays them out
+static_assert(sizeof(hybrid_mlem_response) == 16, "response layout");
+static_assert(sizeof(hybrid_mlem_frame_event) == 48 &&
+                  offsetof(hybrid_mlem_frame_event, info_begin) == 24 &&
+                  offsetof(hybrid_mlem_frame_event, plane_begin) == 40,
+              "frame event layout");
 
 namespace hybrid {
 namespace {
@@ -306,6 +315,47 @@ __global__ __launch_bounds__(256) void frame_events_kernel(
   ev_dup[i] = dup;
 }
 
+// Frame events given by the caller: checked against the geometry, the
+// info end clipped to the arrays, the repeated-visit index found as in
+// frame_events_kernel.
+__global__ __launch_bounds__(256) void frame_import_kernel(
+    Grid g, const hybrid_mlem_frame_event* __restrict__ in, int64_t n,
+    int64_t n_lors, const int2* __restrict__ ranges,
+    const int* __restrict__ pix, int n_infos, int4* __restrict__ ev_i,
+    float4* __restrict__ ev_f, int* __restrict__ ev_dup,
+    int* __restrict__ bad) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const hybrid_mlem_frame_event f = in[i];
+  const int first = f.first, second = f.second;
+  const int64_t lor = (int64_t)(first * (first + 1) / 2 + second);
+  const int64_t ie = f.info_end <= n_infos ? f.info_end : n_infos;
+  if (first < 0 || second < 0 || lor >= n_lors || f.info_begin < 0 ||
+      f.info_begin > ie || f.plane_begin < 0 || f.plane_end > g.n_planes) {
+    atomicOr(bad, 1);
+    ev_i[i] = make_int4(0, 0, 0, 0);
+    ev_f[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    ev_dup[i] = -1;
+    return;
+  }
+  const int2 be = ranges[lor];
+  const int ib = (int)f.info_begin, iend = (int)ie;
+  int dup = -1;
+  const int last = iend - 1;
+  if (last >= ib && last >= be.y) {
+    const int px = pix[last];
+    for (int j = ib; j < last; ++j)
+      if (pix[j] == px) {
+        dup = j;
+        break;
+      }
+  }
+  ev_i[i] = make_int4((int)lor, ib, iend,
+                      (int)f.plane_begin | ((int)f.plane_end << 16));
+  ev_f[i] = make_float4(f.up, f.right, f.tan, f.sec);
+  ev_dup[i] = dup;
+}
+
 __global__ __launch_bounds__(256) void visits_kernel(
     const int4* __restrict__ ev_i, int64_t n,
     unsigned long long* __restrict__ total) {
@@ -428,8 +478,9 @@ __global__ __launch_bounds__(256) void event_kernel(
 // semantics.
 
 constexpr int kEventInfoCap = 256;  // infos of one event held in LDS
-constexpr size_t kLdsBytes = 160 * 1024;
-constexpr size_t kWaveLdsBytes = 2 * kEventInfoCap * sizeof(float);
+// dynamic LDS of the back-projection: 160 KiB less room for its static
+// queue slot
+constexpr size_t kLdsBytes = 160 * 1024 - 256;
 
 // A_ei and dy tan of one info of an event.
 __device__ inline void info_factors(const Kernel& k, const float4& ef,
@@ -532,13 +583,59 @@ struct BoxWalk {
   }
 };
 
-// Forward projection in LOR space: one single-wave block per event
-// (grid-stride, events in LOR order).  D_e over the box from G rho s plus
-// the extra info in full; inv_out[e] = 1 / D_e (0 where D_e is not > 0);
-// the extra info's update (twice, for the visit it repeats, under the
-// kernel-cache semantics) goes to the image-space extra sums.
+// Event parameters of 64 events, one per lane, read back for one event at a
+// time with readlane (wave-uniform values in scalar registers).
+struct EventBatch {
+  int4 i;
+  float4 f;
+  float inv;
+  int dup;
+  __device__ void load(const int4* ev_i, const float4* ev_f,
+                       const float* inv_in, const int* ev_dup, int e,
+                       bool valid) {
+    if (valid) {
+      i = ev_i[e];
+      f = ev_f[e];
+      inv = inv_in ? inv_in[e] : 0.0f;
+      dup = ev_dup[e];
+    } else {
+      i = make_int4(0, 0, 0, 0);
+      f = make_float4(0.f, 0.f, 0.f, 0.f);
+      inv = 0.0f;
+      dup = -1;
+    }
+  }
+};
+
+__device__ inline int lane_int(int v, int j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+__device__ inline float lane_float(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// One LOR-space iteration's event work.  A persistent grid, one block per
+// CU, takes LORs from a queue (LORs with events, most events first).  For
+// each LOR the block's LDS slab (the LOR's infos x all planes, laid out as
+// its rows of bp) first holds P = G rho s of every (info, plane).
+//   Forward: the waves take the LOR's events in turn (parameters read 64 at
+//   a time, one per lane); per event the per-info factors go to the wave's
+//   LDS, D_e = sum A_ei exp(-iz b^2) P over the box (plus the extra info in
+//   full), then 1 / D_e and the back-projection's factors F = A_ei / D_e
+//   and dy tan are written to global memory, and the extra info's update
+//   to the image-space extra sums (twice, for the visit it repeats, under
+//   the kernel-cache semantics, which also leaves that visit's F at 0).
+//   Backward: the slab is cleared; its rows are dealt to the waves in
+//   blocks of kRowBlock (block b to wave b mod n_waves); every wave reads
+//   the events 64 at a time, picks by ballot those whose box meets its
+//   rows, and adds F exp(-iz b^2) over its rows of each box.  Each slab
+//   word has one writer, so the sums are plain LDS read-modify-writes (LDS
+//   float atomics run at about one lane per clock).
+// The slab is finally written to bp.  n_waves must be a power of two.
+constexpr int kRowBlock = 4;
+
 template <bool LAST_WRITE>
-__global__ __launch_bounds__(64) void lor_forward_kernel(
+__global__ __launch_bounds__(1024) void lor_iteration_kernel(
     Grid g, Kernel k, float kexp, const float4* __restrict__ seg_a,
     const float4* __restrict__ seg_b, const int2* __restrict__ ranges,
     const int* __restrict__ pix, const int* __restrict__ ipix,
@@ -546,190 +643,231 @@ __global__ __launch_bounds__(64) void lor_forward_kernel(
     const float* __restrict__ sens, const float* __restrict__ G,
     const float* __restrict__ rho, const int4* __restrict__ ev_i,
     const float4* __restrict__ ev_f, const int* __restrict__ ev_dup,
-    int64_t n_events, float* __restrict__ inv_out,
+    const int64_t* __restrict__ fac_off, const int* __restrict__ lor_events,
+    const int* __restrict__ lor_order, int n_active, int* __restrict__ queue,
+    float* __restrict__ inv, float2* __restrict__ fac, float* __restrict__ bp,
     float* __restrict__ extra) {
-  __shared__ float sA[kEventInfoCap], sT[kEventInfoCap];
-  __shared__ int sP[kEventInfoCap];
-  const int lane = threadIdx.x;
-  const int64_t np = g.n_planes;
-  for (int64_t e = blockIdx.x; e < n_events; e += gridDim.x) {
-    const int4 ei = ev_i[e];
-    const float4 ef = ev_f[e];
-    const int pb = ei.w & 0xffff, pe = ei.w >> 16;
-    const int n_pl = pe - pb;
-    const int2 lr = ranges[ei.x];
-    const int ie_l = ei.z < lr.y ? ei.z : lr.y;
-    const int n_info = ie_l - ei.y;
-    const float R = seg_b[ei.x].w;
-    __syncthreads();  // the previous event's factors are no longer read
-    for (int t = lane; t < n_info; t += 64) {
-      const int i = ei.y + t;
-      float A, T;
-      info_factors(k, ef, R, upi[i], A, T);
-      const int p = ipix[i];
-      sA[t] = p >= 0 ? A * sens[p] : 0.0f;
-      sT[t] = T;
-      sP[t] = p;
-    }
-    __syncthreads();
-    float den0 = 0.0f, den1 = 0.0f;
-    const int total = n_info * (n_pl > 0 ? n_pl : 0);
-    {
-      // two pairs per step keep two gathers of G and rho in flight
-      BoxWalk a(lane, n_pl), b(lane, n_pl);
-      b.step();
-      for (int idx = lane; idx < total; idx += 128) {
-        {
-          const int p = sP[a.q];
-          if (p >= 0) {
-            const int z = pb + a.r;
-            const float d = (plane_centre(g, z) - ef.y) - sT[a.q];
-            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-            den0 += sA[a.q] * K * G[(int64_t)(ei.y + a.q) * np + z] *
-                    rho[(int64_t)p * np + z];
-          }
-        }
-        if (idx + 64 < total) {
-          const int p = sP[b.q];
-          if (p >= 0) {
-            const int z = pb + b.r;
-            const float d = (plane_centre(g, z) - ef.y) - sT[b.q];
-            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-            den1 += sA[b.q] * K * G[(int64_t)(ei.y + b.q) * np + z] *
-                    rho[(int64_t)p * np + z];
-          }
-        }
-        a.step();
-        a.step();
-        b.step();
-        b.step();
-      }
-    }
-    float den = den0 + den1;
-    const bool has_extra = ei.z > lr.y && n_pl > 0;
-    int xx = 0, yy = 0, xpixel = -1;
-    float4 sa = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (has_extra) {
-      const int pxy = pix[ei.z - 1];
-      xx = pixel_x(pxy);
-      yy = pixel_y(pxy);
-      if (in_grid(g, xx, yy)) xpixel = yy * g.n_columns + xx;
-      sa = seg_a[ei.x];
-    }
-    if (xpixel >= 0) {
-      for (int z = pb + lane; z < pe; z += 64) {
-        const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
-        if (!(kv <= 0.0f))
-          den += kv * w[ei.z - 1] * rho[(int64_t)xpixel * np + z] *
-                 sens[xpixel];
-      }
-    }
-    den = strip::wave_sum(den);
-    const float inv = den > 0.0f ? 1.0f / den : 0.0f;
-    if (lane == 0) inv_out[e] = inv;
-    if (xpixel >= 0 && inv > 0.0f) {
-      const float mult = (LAST_WRITE && ev_dup[e] >= 0) ? 2.0f : 1.0f;
-      for (int z = pb + lane; z < pe; z += 64) {
-        const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
-        if (!(kv <= 0.0f)) {
-          const int64_t v = (int64_t)xpixel * np + z;
-          atomicAdd(&extra[v], mult * (kv * w[ei.z - 1] * rho[v] * inv));
-        }
-      }
-    }
-  }
-}
-
-// Back-projection in LOR space: one block per LOR with events, its waves
-// taking the LOR's events in turn (the next event's parameters loaded while
-// the current one is summed); K_e / D_e added into the block's LDS slab
-// (the LOR's infos x all planes, laid out as its rows of bp), which is
-// written to bp at the end.  Under the kernel-cache semantics the visit
-// that the extra info repeats is left out (its update went to the extra
-// sums with the extra info's weight).
-template <bool LAST_WRITE>
-__global__ __launch_bounds__(1024) void lor_backward_kernel(
-    Grid g, Kernel k, float kexp, const float4* __restrict__ seg_b,
-    const int2* __restrict__ ranges, const int* __restrict__ ipix,
-    const float* __restrict__ upi, const int4* __restrict__ ev_i,
-    const float4* __restrict__ ev_f, const int* __restrict__ ev_dup,
-    const float* __restrict__ inv, const int* __restrict__ lor_events,
-    float* __restrict__ bp) {
   extern __shared__ float lds[];
-  const int lor = blockIdx.x;
-  const int e0 = lor_events[lor], e1 = lor_events[lor + 1];
-  if (e0 == e1) return;  // the rows of a LOR without events stay zero
-  const int2 lr = ranges[lor];
-  const int n_lor = lr.y - lr.x;
+  __shared__ int s_next;
   const int np = g.n_planes;
   const int n_waves = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  float* slab = lds;
-  float* sA = lds + (size_t)n_lor * np + (size_t)wave * 2 * kEventInfoCap;
-  float* sT = sA + kEventInfoCap;
-  for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) slab[t] = 0.0f;
-  __syncthreads();
-  const float R = seg_b[lor].w;
-  int e = e0 + wave;
-  int4 ei = make_int4(0, 0, 0, 0);
-  float4 ef = make_float4(0.f, 0.f, 0.f, 0.f);
-  float iv = 0.0f;
-  int dup = -1;
-  if (e < e1) {
-    ei = ev_i[e];
-    ef = ev_f[e];
-    iv = inv[e];
-    if (LAST_WRITE) dup = ev_dup[e];
-  }
-  while (e < e1) {
-    const int en = e + n_waves;
-    int4 ein = make_int4(0, 0, 0, 0);
-    float4 efn = make_float4(0.f, 0.f, 0.f, 0.f);
-    float ivn = 0.0f;
-    int dupn = -1;
-    if (en < e1) {
-      ein = ev_i[en];
-      efn = ev_f[en];
-      ivn = inv[en];
-      if (LAST_WRITE) dupn = ev_dup[en];
+  const int span = kRowBlock * n_waves;
+  for (;;) {
+    if (threadIdx.x == 0) s_next = atomicAdd(queue, 1);
+    __syncthreads();
+    const int qi = s_next;
+    __syncthreads();  // every thread has read s_next
+    if (qi >= n_active) break;
+    const int lor = lor_order[qi];
+    const int e0 = lor_events[lor], e1 = lor_events[lor + 1];
+    const int2 lr = ranges[lor];
+    const int n_lor = lr.y - lr.x;
+    float* slab = lds;
+    float* s_up = slab + (size_t)n_lor * np;
+    float* wA = s_up + n_lor + (size_t)wave * 2 * kEventInfoCap;
+    float* wT = wA + kEventInfoCap;
+    const float4 sa = seg_a[lor];
+    const float R = seg_b[lor].w;
+    // the rows' pixel and sensitivity, staged in the (still unused) factor
+    // area of the waves
+    int* s_pix = (int*)(s_up + n_lor);
+    float* s_sens = (float*)(s_pix + n_lor);
+    for (int t = threadIdx.x; t < n_lor; t += blockDim.x) {
+      s_up[t] = upi[lr.x + t];
+      const int p = ipix[lr.x + t];
+      s_pix[t] = p;
+      s_sens[t] = p >= 0 ? sens[p] : 0.0f;
     }
-    if (iv > 0.0f) {
-      const int pb = ei.w & 0xffff, pe = ei.w >> 16;
-      const int n_pl = pe - pb;
-      const int ie_l = ei.z < lr.y ? ei.z : lr.y;
-      const int n_info = ie_l - ei.y;
-      for (int t = lane; t < n_info; t += 64) {
-        float A, T;
-        info_factors(k, ef, R, upi[ei.y + t], A, T);
-        sA[t] = ipix[ei.y + t] >= 0 && !(LAST_WRITE && ei.y + t == dup)
-                    ? A * iv
-                    : 0.0f;
-        sT[t] = T;
-      }
-      wave_lds_sync();
-      const int total = n_info * (n_pl > 0 ? n_pl : 0);
-      BoxWalk a(lane, n_pl);
-      for (int idx = lane; idx < total; idx += 64) {
-        const float f = sA[a.q];
-        if (f != 0.0f) {
-          const int z = pb + a.r;
-          const float d = (plane_centre(g, z) - ef.y) - sT[a.q];
-          const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-          atomicAdd(&slab[(ei.y + a.q - lr.x) * np + z], f * K);
+    __syncthreads();
+    // P = G rho s, eight cells per thread in flight
+    {
+      const int n_cells = n_lor * np;
+      const float* Grows = G + lr.x * np;
+      const float inv_np = 1.0f / (float)np;
+      for (int c0 = threadIdx.x; c0 < n_cells; c0 += 8 * blockDim.x) {
+        float gv[8], rv[8], sv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = c0 + u * blockDim.x;
+          gv[u] = rv[u] = sv[u] = 0.0f;
+          if (c < n_cells) {
+            int row = (int)((float)c * inv_np);
+            if (row * np > c) --row;
+            if ((row + 1) * np <= c) ++row;
+            const int p = s_pix[row];
+            if (p >= 0) {
+              gv[u] = Grows[c];
+              rv[u] = rho[p * np + (c - row * np)];
+              sv[u] = s_sens[row];
+            }
+          }
         }
-        a.step();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int c = c0 + u * blockDim.x;
+          if (c < n_cells) slab[c] = gv[u] * rv[u] * sv[u];
+        }
       }
-      wave_lds_sync();  // this event's factors are read before the next's
     }
-    e = en;
-    ei = ein;
-    ef = efn;
-    iv = ivn;
-    dup = dupn;
+    __syncthreads();
+
+    // forward: wave w takes events e0 + w, e0 + w + n_waves, ...
+    for (int base = e0 + wave; base < e1; base += 64 * n_waves) {
+      EventBatch bt;
+      const int mine = base + lane * n_waves;
+      bt.load(ev_i, ev_f, nullptr, ev_dup, mine, mine < e1);
+      const int cnt = min(64, (e1 - base + n_waves - 1) / n_waves);
+      for (int j = 0; j < cnt; ++j) {
+        const int e = base + j * n_waves;
+        const int4 ei = make_int4(lane_int(bt.i.x, j), lane_int(bt.i.y, j),
+                                  lane_int(bt.i.z, j), lane_int(bt.i.w, j));
+        const float4 ef = make_float4(lane_float(bt.f.x, j),
+                                      lane_float(bt.f.y, j),
+                                      lane_float(bt.f.z, j),
+                                      lane_float(bt.f.w, j));
+        const int dup = LAST_WRITE ? lane_int(bt.dup, j) : -1;
+        const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+        const int n_pl = pe - pb;
+        const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+        const int a0 = ei.y - lr.x;
+        const int n_info = ie_l - ei.y;
+        for (int t = lane; t < n_info; t += 64) {
+          float A, T;
+          info_factors(k, ef, R, s_up[a0 + t], A, T);
+          wA[t] = A;
+          wT[t] = T;
+        }
+        wave_lds_sync();
+        float den = 0.0f;
+        const int total = n_info * (n_pl > 0 ? n_pl : 0);
+        BoxWalk wk(lane, n_pl);
+        for (int idx = lane; idx < total; idx += 64) {
+          const int z = pb + wk.r;
+          const float d = (plane_centre(g, z) - ef.y) - wT[wk.q];
+          const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+          den += wA[wk.q] * K * slab[(a0 + wk.q) * np + z];
+          wk.step();
+        }
+        int xx = 0, yy = 0, xpixel = -1;
+        if (ei.z > lr.y && n_pl > 0) {
+          const int pxy = pix[ei.z - 1];
+          xx = pixel_x(pxy);
+          yy = pixel_y(pxy);
+          if (in_grid(g, xx, yy)) xpixel = yy * g.n_columns + xx;
+        }
+        if (xpixel >= 0) {
+          for (int z = pb + lane; z < pe; z += 64) {
+            const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
+            if (!(kv <= 0.0f))
+              den += kv * w[ei.z - 1] * rho[xpixel * np + z] * sens[xpixel];
+          }
+        }
+        den = strip::wave_sum(den);
+        const float iv = den > 0.0f ? 1.0f / den : 0.0f;
+        if (lane == 0) inv[e] = iv;
+        if (iv > 0.0f) {
+          float2* f = fac + fac_off[e];
+          for (int t = lane; t < n_info; t += 64)
+            f[t] = make_float2(
+                (LAST_WRITE && ei.y + t == dup) ? 0.0f : wA[t] * iv, wT[t]);
+          if (xpixel >= 0) {
+            const float mult = (LAST_WRITE && dup >= 0) ? 2.0f : 1.0f;
+            for (int z = pb + lane; z < pe; z += 64) {
+              const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
+              if (!(kv <= 0.0f)) {
+                const int v = xpixel * np + z;
+                atomicAdd(&extra[v], mult * (kv * w[ei.z - 1] * rho[v] * iv));
+              }
+            }
+          }
+        }
+        wave_lds_sync();  // this event's factors are read before the next's
+      }
+    }
+    // every 1 / D_e and factor is written and P is no longer read; other
+    // waves of this block read them back from global memory, which the
+    // barrier's workgroup-scope release and acquire order (one CU, one L1)
+    __syncthreads();
+    for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) slab[t] = 0.0f;
+    __syncthreads();
+
+    // back-projection
+    for (int base = e0; base < e1; base += 64) {
+      const int e = base + lane;
+      int4 ei = make_int4(0, 0, 0, 0);
+      float iv = 0.0f, right = 0.0f;
+      int64_t off = 0;
+      if (e < e1) {
+        ei = ev_i[e];
+        iv = inv[e];
+        right = ev_f[e].y;
+        off = fac_off[e];
+      }
+      const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+      const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+      const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
+      int blk = a0 / kRowBlock;
+      blk += (wave - blk) & (n_waves - 1);
+      const bool mine = e < e1 && iv > 0.0f && pe > pb && a1 > a0 &&
+                        blk * kRowBlock < a1;
+      uint64_t mask = __ballot(mine);
+      while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const int ja0 = lane_int(a0, j), ja1 = lane_int(a1, j);
+        const int jpb = lane_int(pb, j), jpe = lane_int(pe, j);
+        const int jblk = lane_int(blk, j);
+        const float jright = lane_float(right, j);
+        const int64_t joff =
+            (int64_t)(((uint64_t)(uint32_t)lane_int((int)(off >> 32), j)
+                       << 32) |
+                      (uint32_t)lane_int((int)off, j));
+        const int n_pl = jpe - jpb;
+        BoxWalk start(lane, n_pl);
+        for (int r0 = jblk * kRowBlock; r0 < ja1; r0 += span) {
+          const int lo = r0 > ja0 ? r0 : ja0;
+          const int hi = r0 + kRowBlock < ja1 ? r0 + kRowBlock : ja1;
+          const int rows = hi - lo;
+          float2 f = make_float2(0.f, 0.f);
+          if (lane < rows) f = fac[joff + (lo - ja0) + lane];
+          const int total = rows * n_pl;
+          BoxWalk wk = start;
+          // a uniform loop: every lane takes part in the shuffles, which
+          // read lanes 0 .. rows - 1
+          for (int step = 0; step < total; step += 64) {
+            const float F = __shfl(f.x, wk.q & 63, 64);
+            const float T = __shfl(f.y, wk.q & 63, 64);
+            if (step + lane < total && F != 0.0f) {
+              const int z = jpb + wk.r;
+              const float d = (plane_centre(g, z) - jright) - T;
+              const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+              slab[(lo + wk.q) * np + z] += F * K;
+            }
+            wk.step();
+          }
+        }
+      }
+    }
+    __syncthreads();
+    float* rows_out = bp + (int64_t)lr.x * np;
+    for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x)
+      rows_out[t] = slab[t];
+    __syncthreads();  // the slab is read before the next LOR fills it
   }
-  __syncthreads();
-  float* rows = bp + (int64_t)lr.x * np;
-  for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) rows[t] = slab[t];
+}
+
+// LORs with events, most events first: (count, LOR) sorted descending.
+__global__ __launch_bounds__(256) void lor_count_keys_kernel(
+    const int* __restrict__ lor_events, int64_t n_lors,
+    unsigned* __restrict__ keys, int* __restrict__ idx,
+    int* __restrict__ n_active) {
+  int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n_lors) return;
+  const int c = lor_events[l + 1] - lor_events[l];
+  keys[l] = (unsigned)c;
+  idx[l] = (int)l;
+  if (c > 0) atomicAdd(n_active, 1);
 }
 
 // rho'(p, z) = rho(p, z) sum_i G(i, z) bp(i, z) + extra(p, z) over the
@@ -774,7 +912,8 @@ __global__ __launch_bounds__(256) void permute_events_kernel(
     const float4* __restrict__ a_f, const int* __restrict__ a_dup,
     const int2* __restrict__ ranges, int4* __restrict__ b_i,
     float4* __restrict__ b_f, int* __restrict__ b_dup,
-    int* __restrict__ lor_count, int* __restrict__ max_infos) {
+    int* __restrict__ lor_count, int* __restrict__ max_infos,
+    int64_t* __restrict__ n_info) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const int src = order[e];
@@ -786,6 +925,7 @@ __global__ __launch_bounds__(256) void permute_events_kernel(
   const int2 lr = ranges[ei.x];
   const int ie_l = ei.z < lr.y ? ei.z : lr.y;
   atomicMax(max_infos, ie_l - ei.y);
+  n_info[e] = ie_l - ei.y;
 }
 
 __global__ __launch_bounds__(256) void iota_kernel(int* __restrict__ p,
@@ -889,6 +1029,13 @@ struct hybrid_mlem_ctx {
   int* pl_dup = nullptr;
   float* inv = nullptr;
   int* lor_events = nullptr;
+  int64_t* fac_off = nullptr;  // per event: its first (event, info) factor
+  float2* fac = nullptr;       // per (event, info): A / D_e, dy tan
+  int64_t fac_capacity = 0;
+  int* lor_order = nullptr;  // LORs with events, most events first
+  int* queue = nullptr;      // the back-projection's LOR counter
+  int n_active = 0;
+  int n_cus = 0;
   int max_event_infos = 0;
   int64_t pl_capacity = 0;
 
@@ -897,7 +1044,8 @@ struct hybrid_mlem_ctx {
     void* ptrs[] = {segments, widths, seg_a, seg_b, ranges, pix, pos,
                     w_given, w, sens, rho, out, stage, ev.ev_i, ev.ev_f,
                     ev.ev_dup, info_lor, ipix, upi, csr_offsets, csr_order,
-                    G, bp, extra, pl_i, pl_f, pl_dup, inv, lor_events};
+                    G, bp, extra, pl_i, pl_f, pl_dup, inv, lor_events,
+                    lor_order, queue, fac_off, fac};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (t0) (void)hipEventDestroy(t0);
@@ -1034,42 +1182,50 @@ int build_pixel_index(hybrid_mlem_ctx* c) {
 
 // LOR-space tables: G (recomputed with every weight preparation) and bp
 // (rows of LORs without events stay zero), the extra accumulator, and the
-// back-projection's block: the largest LOR's slab (infos x planes) plus
-// 2 KiB of per-event factors per wave, within 160 KiB of LDS, 2 to 16
-// waves.
+// iteration kernel's block: the largest LOR's slab (infos x planes) and
+// its infos' up with 2 KiB of per-event factors per wave, within 160 KiB
+// of LDS, 16 waves (or 8, 4, 2 where LDS is short); the tables' cell
+// indices must fit in 31 bits.
 // Where that does not fit, or the tables do not fit in device memory, the
 // context stays on the wavefront-per-event algorithm.
 int prepare_tables(hybrid_mlem_ctx* c) {
   const Grid& g = c->grid;
-  const size_t slab = sizeof(float) * (size_t)c->max_lor_infos * g.n_planes;
-  const int waves =
-      slab < kLdsBytes ? (int)std::min<size_t>(16, (kLdsBytes - slab) /
-                                                       kWaveLdsBytes)
-                       : 0;
-  if (waves < 2) {
+  const size_t slab =
+      sizeof(float) * (size_t)c->max_lor_infos * ((size_t)g.n_planes + 1);
+  const size_t per_wave = sizeof(float) * 2 * kEventInfoCap;
+  int waves = 16;  // a power of two (the row blocks' wave index)
+  while (waves > 2 && slab + waves * per_wave > kLdsBytes) waves /= 2;
+  // the factor area also stages the rows' pixel and sensitivity
+  if (slab + waves * per_wave > kLdsBytes ||
+      2 * sizeof(float) * (size_t)c->max_lor_infos > waves * per_wave) {
     c->tables = false;
     return STRIP_MLEM_OK;
   }
   c->lor_waves = waves;
-  c->lor_lds = slab + waves * kWaveLdsBytes;
+  c->lor_lds = slab + waves * per_wave;
   const int64_t cells = (int64_t)c->n_infos * g.n_planes;
   if (!c->G) {
-    if (device_alloc(&c->G, cells) != STRIP_MLEM_OK ||
-        device_alloc(&c->bp, cells) != STRIP_MLEM_OK ||
-        device_alloc(&c->extra, c->n_voxels()) != STRIP_MLEM_OK) {
-      if (c->G) (void)hipFree(c->G);
-      if (c->bp) (void)hipFree(c->bp);
-      c->G = c->bp = nullptr;
+    const bool ok =
+        cells < ((int64_t)1 << 31) &&
+        hipFuncSetAttribute((const void*)lor_iteration_kernel<true>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kLdsBytes) == hipSuccess &&
+        hipFuncSetAttribute((const void*)lor_iteration_kernel<false>,
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)kLdsBytes) == hipSuccess &&
+        device_alloc(&c->G, cells) == STRIP_MLEM_OK &&
+        device_alloc(&c->bp, cells) == STRIP_MLEM_OK &&
+        device_alloc(&c->extra, c->n_voxels()) == STRIP_MLEM_OK;
+    if (!ok) {
+      (void)hipGetLastError();
+      void* ps[] = {c->G, c->bp, c->extra};
+      for (void* q : ps)
+        if (q) (void)hipFree(q);
+      c->G = c->bp = c->extra = nullptr;
       c->tables = false;
       return STRIP_MLEM_OK;
     }
     HY_TRY(hipMemsetAsync(c->bp, 0, sizeof(float) * cells, c->stream));
-    HY_TRY(hipFuncSetAttribute(
-        (const void*)lor_backward_kernel<true>,
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
-    HY_TRY(hipFuncSetAttribute(
-        (const void*)lor_backward_kernel<false>,
-        hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBytes));
   }
   if (cells)
     hipLaunchKernelGGL(frame_table_kernel, dim3(blocks_for(cells)),
@@ -1087,16 +1243,18 @@ int build_plan(hybrid_mlem_ctx* c) {
   const int64_t n = c->ev.n;
   int rc = STRIP_MLEM_OK;
   if (n > c->pl_capacity) {
-    void* ps[] = {c->pl_i, c->pl_f, c->pl_dup, c->inv};
+    void* ps[] = {c->pl_i, c->pl_f, c->pl_dup, c->inv, c->fac_off};
     for (void* p : ps)
       if (p) (void)hipFree(p);
     c->pl_i = nullptr;
     c->pl_f = nullptr;
     c->pl_dup = nullptr;
     c->inv = nullptr;
+    c->fac_off = nullptr;
     c->pl_capacity = 0;
     if ((rc = device_alloc(&c->pl_i, n)) || (rc = device_alloc(&c->pl_f, n)) ||
-        (rc = device_alloc(&c->pl_dup, n)) || (rc = device_alloc(&c->inv, n)))
+        (rc = device_alloc(&c->pl_dup, n)) || (rc = device_alloc(&c->inv, n)) ||
+        (rc = device_alloc(&c->fac_off, n + 1)))
       return rc;
     c->pl_capacity = n;
   }
@@ -1105,10 +1263,11 @@ int build_plan(hybrid_mlem_ctx* c) {
     return rc;
   unsigned *keys = nullptr, *keys_sorted = nullptr;
   int *idx = nullptr, *order = nullptr, *count = nullptr, *stats = nullptr;
+  int64_t* n_info = nullptr;
   char* tmp = nullptr;
   auto cleanup = [&]() {
     (void)hipStreamSynchronize(c->stream);
-    void* ps[] = {keys, keys_sorted, idx, order, count, stats, tmp};
+    void* ps[] = {keys, keys_sorted, idx, order, count, stats, n_info, tmp};
     for (void* p : ps)
       if (p) (void)hipFree(p);
   };
@@ -1116,17 +1275,20 @@ int build_plan(hybrid_mlem_ctx* c) {
   if ((rc = device_alloc(&keys, nn)) || (rc = device_alloc(&keys_sorted, nn)) ||
       (rc = device_alloc(&idx, nn)) || (rc = device_alloc(&order, nn)) ||
       (rc = device_alloc(&count, (size_t)c->n_lors + 1)) ||
-      (rc = device_alloc(&stats, 4))) {
+      (rc = device_alloc(&stats, 4)) || (rc = device_alloc(&n_info, nn + 1))) {
     cleanup();
     return rc;
   }
-  size_t b1 = 0, b2 = 0;
+  size_t b1 = 0, b2 = 0, b3 = 0;
   hipError_t e = hipcub::DeviceRadixSort::SortPairs(
       nullptr, b1, keys, keys_sorted, idx, order, (int)nn, 0, 32, c->stream);
   if (e == hipSuccess)
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, b2, count, c->lor_events,
                                          (int)(c->n_lors + 1), c->stream);
-  const size_t tb = std::max(b1, b2);
+  if (e == hipSuccess)
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, b3, n_info, c->fac_off,
+                                         (int)(nn + 1), c->stream);
+  const size_t tb = std::max(b1, std::max(b2, b3));
   if (e == hipSuccess && (rc = device_alloc(&tmp, tb))) {
     cleanup();
     return rc;
@@ -1143,19 +1305,69 @@ int build_plan(hybrid_mlem_ctx* c) {
     if (e == hipSuccess)
       e = hipcub::DeviceRadixSort::SortPairs(tmp, t1, keys, keys_sorted, idx,
                                              order, (int)n, 0, 32, c->stream);
+    if (e == hipSuccess)
+      e = hipMemsetAsync(n_info + n, 0, sizeof(int64_t), c->stream);
     if (e == hipSuccess) {
       hipLaunchKernelGGL(permute_events_kernel, dim3(blocks_for(n)),
                          dim3(256), 0, c->stream, order, n, c->ev.ev_i,
                          c->ev.ev_f, c->ev.ev_dup, c->ranges, c->pl_i,
-                         c->pl_f, c->pl_dup, count, stats);
+                         c->pl_f, c->pl_dup, count, stats, n_info);
       e = hipGetLastError();
     }
     if (e == hipSuccess)
       e = hipcub::DeviceScan::ExclusiveSum(tmp, t2, count, c->lor_events,
                                            (int)(c->n_lors + 1), c->stream);
+    size_t t3 = tb;
+    if (e == hipSuccess)
+      e = hipcub::DeviceScan::ExclusiveSum(tmp, t3, n_info, c->fac_off,
+                                           (int)(n + 1), c->stream);
   } else if (e == hipSuccess) {
     e = hipMemsetAsync(c->lor_events, 0,
                        sizeof(int) * ((size_t)c->n_lors + 1), c->stream);
+  }
+  // the LOR queue: LORs with events, most events first
+  if (!c->lor_order && (rc = device_alloc(&c->lor_order, (size_t)c->n_lors))) {
+    cleanup();
+    return rc;
+  }
+  if (!c->queue && (rc = device_alloc(&c->queue, 1))) {
+    cleanup();
+    return rc;
+  }
+  if (e == hipSuccess && c->n_lors) {
+    unsigned* ck = nullptr;
+    unsigned* ck_sorted = nullptr;
+    int* li = nullptr;
+    char* t2 = nullptr;
+    size_t tb2 = 0;
+    if ((rc = device_alloc(&ck, (size_t)c->n_lors)) ||
+        (rc = device_alloc(&ck_sorted, (size_t)c->n_lors)) ||
+        (rc = device_alloc(&li, (size_t)c->n_lors))) {
+      if (ck) (void)hipFree(ck);
+      if (ck_sorted) (void)hipFree(ck_sorted);
+      cleanup();
+      return rc;
+    }
+    hipLaunchKernelGGL(lor_count_keys_kernel, dim3(blocks_for(c->n_lors)),
+                       dim3(256), 0, c->stream, c->lor_events, c->n_lors, ck,
+                       li, stats + 1);
+    e = hipGetLastError();
+    if (e == hipSuccess)
+      e = hipcub::DeviceRadixSort::SortPairsDescending(
+          nullptr, tb2, ck, ck_sorted, li, c->lor_order, (int)c->n_lors, 0,
+          32, c->stream);
+    if (e == hipSuccess && (rc = device_alloc(&t2, tb2)) == STRIP_MLEM_OK)
+      e = hipcub::DeviceRadixSort::SortPairsDescending(
+          t2, tb2, ck, ck_sorted, li, c->lor_order, (int)c->n_lors, 0, 32,
+          c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    void* ps[] = {ck, ck_sorted, li, t2};
+    for (void* q : ps)
+      if (q) (void)hipFree(q);
+    if (rc) {
+      cleanup();
+      return rc;
+    }
   }
   // a LOR whose events are gone keeps no stale rows
   if (e == hipSuccess && c->bp)
@@ -1171,7 +1383,20 @@ int build_plan(hybrid_mlem_ctx* c) {
   if (e != hipSuccess)
     return strip::fail(STRIP_MLEM_ERR_HIP,
                        std::string("event plan: ") + hipGetErrorString(e));
+  int64_t total = 0;
+  if (n > 0) {
+    HY_TRY(hipMemcpy(&total, c->fac_off + n, sizeof(int64_t),
+                     hipMemcpyDeviceToHost));
+  }
+  if (total > c->fac_capacity) {
+    if (c->fac) (void)hipFree(c->fac);
+    c->fac = nullptr;
+    c->fac_capacity = 0;
+    HY_CHECK(device_alloc(&c->fac, total));
+    c->fac_capacity = total;
+  }
   c->max_event_infos = host_stats[0];
+  c->n_active = n > 0 ? host_stats[1] : 0;
   c->plan_valid = true;
   return STRIP_MLEM_OK;
 }
@@ -1343,6 +1568,10 @@ int hybrid_mlem_create(const hybrid_mlem_geometry* G, int semantics,
       return bail(strip::fail(STRIP_MLEM_ERR_HIP, "per-info arrays"));
   }
   if ((rc = build_pixel_index(c))) return bail(rc);
+  if (hipDeviceGetAttribute(&c->n_cus, hipDeviceAttributeMultiprocessorCount,
+                            device) != hipSuccess ||
+      c->n_cus <= 0)
+    c->n_cus = 1;
   *out = c;
   return STRIP_MLEM_OK;
 }
@@ -1357,8 +1586,8 @@ int hybrid_mlem_prepare_weights(hybrid_mlem_ctx* c, int calculate_weight,
                                 int sensitivity_mode,
                                 const float* sensitivity) {
   if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
-  if (sensitivity_mode < 0 || sensitivity_mode > 2 ||
-      (sensitivity_mode == 2 && !sensitivity))
+  if (sensitivity_mode < 0 || sensitivity_mode > 3 ||
+      (sensitivity_mode >= 2 && !sensitivity))
     return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad sensitivity mode");
   HY_TRY(hipSetDevice(c->device));
   const Grid& g = c->grid;
@@ -1373,7 +1602,7 @@ int hybrid_mlem_prepare_weights(hybrid_mlem_ctx* c, int calculate_weight,
                        c->ranges, c->pix, c->n_lors, c->w);
     HY_TRY(hipGetLastError());
   }
-  if (sensitivity_mode == 2) {
+  if (sensitivity_mode >= 2) {
     HY_TRY(hipMemcpyAsync(c->sens, sensitivity, sizeof(float) * g.n_pixels,
                           hipMemcpyHostToDevice, c->stream));
   } else if (sensitivity_mode == 1) {
@@ -1386,7 +1615,7 @@ int hybrid_mlem_prepare_weights(hybrid_mlem_ctx* c, int calculate_weight,
                        c->w, g.n_pixels, c->sens);
     HY_TRY(hipGetLastError());
   }
-  if (n)
+  if (n && sensitivity_mode != HYBRID_MLEM_SENSITIVITY_NORMALISED)
     hipLaunchKernelGGL(normalize_kernel, dim3(blocks_for(n)), dim3(256), 0,
                        c->stream, g, c->pix, c->sens, (int64_t)n, c->w);
   HY_TRY(hipGetLastError());
@@ -1511,6 +1740,60 @@ int64_t hybrid_mlem_n_events(const hybrid_mlem_ctx* c) {
   return c ? c->ev.n : -1;
 }
 
+int hybrid_mlem_set_frame_events(hybrid_mlem_ctx* c,
+                                 const hybrid_mlem_frame_event* events,
+                                 int64_t n) {
+  if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (n < 0 || (n && !events))
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad frame events");
+  HY_TRY(hipSetDevice(c->device));
+  c->ev.n = 0;
+  c->plan_valid = false;
+  if (n == 0) return STRIP_MLEM_OK;
+  HY_CHECK(grow_events(c, n));
+  hybrid_mlem_frame_event* d = nullptr;
+  int* bad = nullptr;
+  int rc = STRIP_MLEM_OK;
+  if ((rc = device_alloc(&d, n)) || (rc = device_alloc(&bad, 1))) {
+    if (d) (void)hipFree(d);
+    return rc;
+  }
+  int host_bad = 0;
+  hipError_t e = hipMemcpyAsync(d, events, sizeof(hybrid_mlem_frame_event) * n,
+                                hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(int), c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(frame_import_kernel, dim3(blocks_for(n)), dim3(256), 0,
+                       c->stream, c->grid, d, n, c->n_lors, c->ranges, c->pix,
+                       c->n_infos, c->ev.ev_i, c->ev.ev_f, c->ev.ev_dup, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&host_bad, bad, sizeof(int), hipMemcpyDeviceToHost,
+                       c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  (void)hipFree(bad);
+  if (e != hipSuccess)
+    return strip::fail(STRIP_MLEM_ERR_HIP,
+                       std::string("frame events: ") + hipGetErrorString(e));
+  if (host_bad)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT,
+                       "a frame event's LOR, info or plane range is outside "
+                       "the geometry");
+  c->ev.n = n;
+  return STRIP_MLEM_OK;
+}
+
+int hybrid_mlem_device_name(hybrid_mlem_ctx* c, char* name, int size) {
+  if (!c || !name || size <= 0)
+    return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "bad arguments");
+  hipDeviceProp_t prop;
+  HY_TRY(hipGetDeviceProperties(&prop, c->device));
+  std::snprintf(name, (size_t)size, "%s (%s)", prop.name, prop.gcnArchName);
+  return STRIP_MLEM_OK;
+}
+
 int hybrid_mlem_get_frame_events(hybrid_mlem_ctx* c, int32_t* ints,
                                  float* floats) {
   if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
@@ -1625,33 +1908,24 @@ int iterate_lor_space(hybrid_mlem_ctx* c) {
   const float kexp = -c->kernel.iz * 1.44269504088896340736f;
   hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(nv)), dim3(256), 0,
                      c->stream, c->extra, nv, 0.0f);
-  const int64_t n = c->ev.n;
-  if (n && c->n_lors) {
-    const bool cpu = c->semantics == HYBRID_MLEM_CPU;
-    const unsigned fblocks = (unsigned)std::min<int64_t>(n, 65536);
-    if (cpu)
-      hipLaunchKernelGGL(lor_forward_kernel<true>, dim3(fblocks), dim3(64), 0,
+  if (c->ev.n && c->n_active) {
+    HY_TRY(hipMemsetAsync(c->queue, 0, sizeof(int), c->stream));
+    const dim3 grid((unsigned)std::max(1, std::min(c->n_cus, c->n_active))),
+        block(64 * c->lor_waves);
+    if (c->semantics == HYBRID_MLEM_CPU)
+      hipLaunchKernelGGL(lor_iteration_kernel<true>, grid, block, c->lor_lds,
                          c->stream, g, c->kernel, kexp, c->seg_a, c->seg_b,
                          c->ranges, c->pix, c->ipix, c->upi, c->w, c->sens,
-                         c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup, n, c->inv,
-                         c->extra);
+                         c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup,
+                         c->fac_off, c->lor_events, c->lor_order, c->n_active,
+                         c->queue, c->inv, c->fac, c->bp, c->extra);
     else
-      hipLaunchKernelGGL(lor_forward_kernel<false>, dim3(fblocks), dim3(64),
-                         0, c->stream, g, c->kernel, kexp, c->seg_a, c->seg_b,
-                         c->ranges, c->pix, c->ipix, c->upi, c->w, c->sens,
-                         c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup, n, c->inv,
-                         c->extra);
-    const dim3 grid((unsigned)c->n_lors), block(64 * c->lor_waves);
-    if (cpu)
-      hipLaunchKernelGGL(lor_backward_kernel<true>, grid, block, c->lor_lds,
-                         c->stream, g, c->kernel, kexp, c->seg_b, c->ranges,
-                         c->ipix, c->upi, c->pl_i, c->pl_f, c->pl_dup, c->inv,
-                         c->lor_events, c->bp);
-    else
-      hipLaunchKernelGGL(lor_backward_kernel<false>, grid, block, c->lor_lds,
-                         c->stream, g, c->kernel, kexp, c->seg_b, c->ranges,
-                         c->ipix, c->upi, c->pl_i, c->pl_f, c->pl_dup, c->inv,
-                         c->lor_events, c->bp);
+      hipLaunchKernelGGL(lor_iteration_kernel<false>, grid, block,
+                         c->lor_lds, c->stream, g, c->kernel, kexp, c->seg_a,
+                         c->seg_b, c->ranges, c->pix, c->ipix, c->upi, c->w,
+                         c->sens, c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup,
+                         c->fac_off, c->lor_events, c->lor_order, c->n_active,
+                         c->queue, c->inv, c->fac, c->bp, c->extra);
   }
   const unsigned gblocks =
       (unsigned)std::min<int64_t>((g.n_pixels + 3) / 4, 65536);

@@ -292,6 +292,14 @@ HYBRID_MLEM_LEGACY_GPU = 1
 
 RESPONSE_DTYPE = np.dtype([("first", "<i2"), ("second", "<i2"),
                            ("z_up", "<f4"), ("z_dn", "<f4"), ("dl", "<f4")])
+# hybrid_mlem_frame_event, with the C layout's padding
+FRAME_EVENT_DTYPE = np.dtype({
+    "names": ["first", "second", "up", "right", "tan", "sec", "info_begin",
+              "info_end", "plane_begin", "plane_end"],
+    "formats": ["<i2", "<i2", "<f4", "<f4", "<f4", "<f4", "<i8", "<i8",
+                "<i2", "<i2"],
+    "offsets": [0, 2, 4, 8, 12, 16, 24, 32, 40, 42],
+    "itemsize": 48})
 
 
 class GeometryC(ctypes.Structure):
@@ -325,7 +333,8 @@ def exported_symbols():
             "hybrid_mlem_get_rho", "hybrid_mlem_get_sensitivity",
             "hybrid_mlem_get_weights", "hybrid_mlem_iterate",
             "hybrid_mlem_last_iteration_ms", "hybrid_mlem_voxel_visits",
-            "hybrid_mlem_set_algorithm", "hybrid_mlem_get_algorithm"]
+            "hybrid_mlem_set_algorithm", "hybrid_mlem_get_algorithm",
+            "hybrid_mlem_set_frame_events", "hybrid_mlem_device_name"]
 
 
 def load_library():
@@ -359,6 +368,9 @@ def load_library():
         "hybrid_mlem_voxel_visits": (ctypes.c_int, [P, ctypes.POINTER(i64)]),
         "hybrid_mlem_set_algorithm": (ctypes.c_int, [P, ctypes.c_int]),
         "hybrid_mlem_get_algorithm": (ctypes.c_int, [P]),
+        "hybrid_mlem_set_frame_events": (ctypes.c_int, [P, P, i64]),
+        "hybrid_mlem_device_name": (ctypes.c_int, [P, ctypes.c_char_p,
+                                                   ctypes.c_int]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -454,6 +466,18 @@ class HybridReconstruction:
 
     def clear_events(self):
         self._check(self._lib.hybrid_mlem_clear_events(self._ctx))
+
+    def set_frame_events(self, frames):
+        """Replaces the events held by frame events (a FRAME_EVENT_DTYPE
+        array), with no filter."""
+        f = np.ascontiguousarray(frames, FRAME_EVENT_DTYPE)
+        self._check(self._lib.hybrid_mlem_set_frame_events(
+            self._ctx, f.ctypes.data if len(f) else None, len(f)))
+
+    def device_name(self):
+        buf = ctypes.create_string_buffer(256)
+        self._check(self._lib.hybrid_mlem_device_name(self._ctx, buf, 256))
+        return buf.value.decode()
 
     @property
     def n_events(self):

@@ -221,3 +221,47 @@ def test_full_ring_workload(oracle_mod, hybrid_mod):
         assert e <= 1e-4
         rec.close()
     assert oracle_mod.rel_l2(images[1], images[2]) <= 1e-4
+
+
+@pytest.mark.parametrize("algorithm", [1, 2])
+def test_frame_events_from_the_reader(oracle_mod, pystrip_mod, hybrid_mod,
+                                      ring, algorithm):
+    # the reference's GPU run receives the normalised weights, the
+    # sensitivity and the frame events of its CPU reader
+    # (reconstruction_cmd.cpp:596-600): sensitivity mode 3 and
+    # hybrid_mlem_set_frame_events, as include/hybrid_gpu_reconstruction.hpp
+    # drives them
+    from dataclasses import replace
+    print_case("hybrid frame events from the reader, algorithm", algorithm)
+    ev = hybrid_mod.hybrid_events(ring, 30000, seed=16)
+    _, _, sens, w = oracle_mod.hybrid_mlem(ring, ev[:0], iterations=0)
+    ints, floats, keep = oracle_mod.hybrid_frames(ring, ev, w)
+    frames = np.zeros(int(keep.sum()), hybrid_mod.FRAME_EVENT_DTYPE)
+    frames["first"] = ev[keep, 0]
+    frames["second"] = ev[keep, 1]
+    for c, name in enumerate(["up", "right", "tan", "sec"]):
+        frames[name] = floats[keep, c]
+    frames["info_begin"] = ints[keep, 1]
+    frames["info_end"] = ints[keep, 2]
+    frames["plane_begin"] = ints[keep, 3]
+    frames["plane_end"] = ints[keep, 4]
+    geo = replace(ring, weights=w[None].astype(np.float32))
+    rec = hybrid_mod.HybridReconstruction(geo)
+    rec.prepare_weights(False, 3, sens)
+    rec.set_algorithm(algorithm)
+    assert np.array_equal(rec.weights, geo.weights[0])
+    assert np.array_equal(rec.sensitivity, sens.astype(np.float32))
+    rec.set_frame_events(frames)
+    assert rec.n_events == len(frames)
+    assert "gfx950" in rec.device_name()
+    rec.iterate(3)
+    assert rec.algorithm == algorithm
+    ref = oracle_mod.hybrid_mlem(ring, ev, iterations=3)[0]
+    e = oracle_mod.rel_l2(rec.rho, ref)
+    print("rel-L2", e, flush=True)
+    assert e <= 1e-4
+    bad = frames[:4].copy()
+    bad["plane_end"][2] = ring.n_planes + 1
+    with pytest.raises(pystrip_mod.StripMLEMError):
+        rec.set_frame_events(bad)
+    rec.close()
