@@ -568,9 +568,21 @@ __device__ inline void wave_lds_sync() {
 struct BoxWalk {
   int q, r, sq, sr, n_pl;
   __device__ BoxWalk(int lane, int n_pl_) : n_pl(n_pl_) {
-    sq = n_pl > 0 ? 64 / n_pl : 0;
+    if (n_pl > 0) {
+      // quotients of 0..64 by n_pl from a float reciprocal (exact for these
+      // small integers after the one-step correction)
+      const float rcp = __builtin_amdgcn_rcpf((float)n_pl);
+      sq = (int)(64.0f * rcp);
+      sq += (sq + 1) * n_pl <= 64 ? 1 : 0;
+      sq -= sq * n_pl > 64 ? 1 : 0;
+      q = (int)((float)lane * rcp);
+      q += (q + 1) * n_pl <= lane ? 1 : 0;
+      q -= q * n_pl > lane ? 1 : 0;
+    } else {
+      sq = 0;
+      q = 0;
+    }
     sr = 64 - sq * n_pl;
-    q = n_pl > 0 ? lane / n_pl : 0;
     r = lane - q * n_pl;
   }
   __device__ void step() {
@@ -626,13 +638,14 @@ __device__ inline float lane_float(float v, int j) {
 //   to the image-space extra sums (twice, for the visit it repeats, under
 //   the kernel-cache semantics, which also leaves that visit's F at 0).
 //   Backward: the slab is cleared; its rows are dealt to the waves in
-//   blocks of kRowBlock (block b to wave b mod n_waves); every wave reads
+//   blocks (block b to wave b mod n_waves); every wave reads
 //   the events 64 at a time, picks by ballot those whose box meets its
 //   rows, and adds F exp(-iz b^2) over its rows of each box.  Each slab
 //   word has one writer, so the sums are plain LDS read-modify-writes (LDS
 //   float atomics run at about one lane per clock).
-// The slab is finally written to bp.  n_waves must be a power of two.
-constexpr int kRowBlock = 4;
+// The slab is finally written to bp.  n_waves must be a power of two; the
+// row blocks hold 4 to 16 rows (a power of two), as many as still give every
+// wave a block.
 
 template <bool LAST_WRITE>
 __global__ __launch_bounds__(1024) void lor_iteration_kernel(
@@ -652,7 +665,6 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
   const int np = g.n_planes;
   const int n_waves = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int span = kRowBlock * n_waves;
   for (;;) {
     if (threadIdx.x == 0) s_next = atomicAdd(queue, 1);
     __syncthreads();
@@ -663,6 +675,10 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     const int e0 = lor_events[lor], e1 = lor_events[lor + 1];
     const int2 lr = ranges[lor];
     const int n_lor = lr.y - lr.x;
+    int rb_shift = 2;  // row blocks of 1 << rb_shift rows
+    while (rb_shift < 4 && (n_lor >> (rb_shift + 1)) >= n_waves) ++rb_shift;
+    const int rb = 1 << rb_shift;
+    const int span = rb * n_waves;
     float* slab = lds;
     float* s_up = slab + (size_t)n_lor * np;
     float* wA = s_up + n_lor + (size_t)wave * 2 * kEventInfoCap;
@@ -807,10 +823,10 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
       const int pb = ei.w & 0xffff, pe = ei.w >> 16;
       const int ie_l = ei.z < lr.y ? ei.z : lr.y;
       const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
-      int blk = a0 / kRowBlock;
+      int blk = a0 >> rb_shift;
       blk += (wave - blk) & (n_waves - 1);
       const bool mine = e < e1 && iv > 0.0f && pe > pb && a1 > a0 &&
-                        blk * kRowBlock < a1;
+                        (blk << rb_shift) < a1;
       uint64_t mask = __ballot(mine);
       while (mask) {
         const int j = __builtin_ctzll(mask);
@@ -825,9 +841,9 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
                       (uint32_t)lane_int((int)off, j));
         const int n_pl = jpe - jpb;
         BoxWalk start(lane, n_pl);
-        for (int r0 = jblk * kRowBlock; r0 < ja1; r0 += span) {
+        for (int r0 = jblk << rb_shift; r0 < ja1; r0 += span) {
           const int lo = r0 > ja0 ? r0 : ja0;
-          const int hi = r0 + kRowBlock < ja1 ? r0 + kRowBlock : ja1;
+          const int hi = r0 + rb < ja1 ? r0 + rb : ja1;
           const int rows = hi - lo;
           float2 f = make_float2(0.f, 0.f);
           if (lane < rows) f = fac[joff + (lo - ja0) + lane];
@@ -884,14 +900,46 @@ __global__ __launch_bounds__(256) void lor_gather_kernel(
   for (int p = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
        p < n_pixels; p += n_waves) {
     const int k0 = offsets[p], k1 = offsets[p + 1];
-    for (int z = lane; z < n_planes; z += 64) {
-      float acc = 0.0f;
-      for (int k = k0; k < k1; ++k) {
-        const int64_t i = order[k];
-        acc += G[i * np + z] * bp[i * np + z];
+    for (int z0 = 0; z0 < n_planes; z0 += 128) {
+      // two plane chunks per pass; the pixel's info indices are read 64 at
+      // a time, one per lane, and four infos are summed per step
+      const int za = z0 + lane, zb = z0 + 64 + lane;
+      const bool in_a = za < n_planes, in_b = zb < n_planes;
+      float acc_a = 0.0f, acc_b = 0.0f;
+      for (int kb = k0; kb < k1; kb += 64) {
+        const int mine = kb + lane < k1 ? order[kb + lane] : 0;
+        const int cnt = k1 - kb < 64 ? k1 - kb : 64;
+        int j = 0;
+        for (; j + 4 <= cnt; j += 4) {
+          float ga[4], ba[4], gb[4], bb[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t row = (int64_t)lane_int(mine, j + u) * np;
+            ga[u] = in_a ? G[row + za] : 0.0f;
+            ba[u] = in_a ? bp[row + za] : 0.0f;
+            gb[u] = in_b ? G[row + zb] : 0.0f;
+            bb[u] = in_b ? bp[row + zb] : 0.0f;
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            acc_a += ga[u] * ba[u];
+            acc_b += gb[u] * bb[u];
+          }
+        }
+        for (; j < cnt; ++j) {
+          const int64_t row = (int64_t)lane_int(mine, j) * np;
+          if (in_a) acc_a += G[row + za] * bp[row + za];
+          if (in_b) acc_b += G[row + zb] * bp[row + zb];
+        }
       }
-      const int64_t v = (int64_t)p * np + z;
-      out[v] = rho[v] * acc + extra[v];
+      if (in_a) {
+        const int64_t v = (int64_t)p * np + za;
+        out[v] = rho[v] * acc_a + extra[v];
+      }
+      if (in_b) {
+        const int64_t v = (int64_t)p * np + zb;
+        out[v] = rho[v] * acc_b + extra[v];
+      }
     }
   }
 }
