@@ -292,6 +292,38 @@ def batched_timing(pystrip, phantom, n_members, iterations):
     return out
 
 
+# Algorithmic work of one hybrid voxel visit in the LOR-space iteration:
+# forward b = (z - right) - dy tan, b^2 kexp, exp2, A K, x P, += (8), and
+# back-projection b, b^2 kexp, exp2, F K, += (7)
+HYBRID_FLOPS_PER_VISIT = 15
+
+
+def hybrid_roofline(geo, visits, event_ms, gather_ms, algorithm):
+    """The event kernels against the FP32 peak (algorithmic FLOPs per
+    visit x visits per iteration / their time), and the gather against HBM
+    (G and bp rows of every info, rho, the extra sums and the new image)."""
+    out = {
+        "event_kernels": {
+            "bound": "valu",
+            "achieved": visits * HYBRID_FLOPS_PER_VISIT / (event_ms * 1e-3)
+            / 1e12 if event_ms > 0 else None,
+            "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "avg_ms": event_ms,
+            "flops_per_visit": HYBRID_FLOPS_PER_VISIT,
+        },
+    }
+    e = out["event_kernels"]
+    e["frac"] = e["achieved"] / PEAK_FP32_TFLOPS if e["achieved"] else None
+    if algorithm == 2 and gather_ms > 0:
+        nv = geo.n_rows * geo.n_columns * geo.n_planes
+        nbytes = 4 * (2 * geo.n_infos * geo.n_planes + 3 * nv)
+        gbps = nbytes / (gather_ms * 1e-3) / 1e9
+        out["gather"] = {"bound": "hbm", "achieved": gbps, "peak": 8000.0,
+                         "unit": "GB/s", "frac": gbps / 8000.0,
+                         "avg_ms": gather_ms, "bytes_per_launch": nbytes}
+    return out
+
+
 def hybrid_timing(name, steps, warmup, cpu_sample, with_cpu, algorithm=0):
     """The 3D hybrid reconstruction (include/hybrid_mlem.h) on a named
     workload: weights and sensitivity prepared on the device, events
@@ -320,6 +352,7 @@ def hybrid_timing(name, steps, warmup, cpu_sample, with_cpu, algorithm=0):
     rec.iterate(steps)
     elapsed = time.perf_counter() - t
     kernel_ms = rec.last_iteration_ms()
+    event_ms, gather_ms, timed = rec.timing()
     n = rec.n_events
     used_algorithm = rec.algorithm
     mass = float((rec.rho.astype(np.float64)).sum())
@@ -338,6 +371,9 @@ def hybrid_timing(name, steps, warmup, cpu_sample, with_cpu, algorithm=0):
         "event_kernel_ms_last_iteration": kernel_ms,
         "voxel_visits_per_iteration": visits,
         "voxel_visits_per_s": visits / (kernel_ms * 1e-3),
+        "roofline": hybrid_roofline(geo, visits, event_ms / max(timed, 1),
+                                    gather_ms / max(timed, 1),
+                                    used_algorithm),
         "timing_s": {"geometry_and_events": t_gen,
                      "weights_sensitivity_and_frames": t_setup,
                      "warmup_iterations_and_plan": t_first},

@@ -191,8 +191,15 @@ int hybrid_mlem_iterate(hybrid_mlem_ctx* ctx, int n_iterations);
 int hybrid_mlem_set_algorithm(hybrid_mlem_ctx* ctx, int algorithm);
 int hybrid_mlem_get_algorithm(hybrid_mlem_ctx* ctx);
 
-/* Device time of the last hybrid_mlem_iterate's event kernels, in ms. */
+/* Device time of the last iteration of the last hybrid_mlem_iterate, in
+ * ms (HIP events on the context stream around the iteration's kernels). */
 int hybrid_mlem_last_iteration_ms(hybrid_mlem_ctx* ctx, float* ms);
+/* Sums over the iterations of the last hybrid_mlem_iterate: the event
+ * kernels (algorithm 2: the LOR-space iteration kernel and the clearing of
+ * the extra sums; algorithm 1: the event kernel) and the gather (algorithm
+ * 2; 0 for algorithm 1), in ms, and the iteration count. */
+int hybrid_mlem_get_timing(hybrid_mlem_ctx* ctx, double* event_kernels_ms,
+                           double* gather_ms, int* n_iterations);
 
 /* Voxel visits (info x plane pairs) of one iteration over the events held:
  * the sum over events of (info end - info begin) (plane end - plane

@@ -1052,6 +1052,11 @@ struct hybrid_mlem_ctx {
   float* stage = nullptr;  // reference-order staging
   Events ev;
   float last_ms = 0.0f;
+  // per-iteration timing events of the last hybrid_mlem_iterate call:
+  // start, before the gather, end
+  std::vector<hipEvent_t> t_start, t_mid, t_end;
+  double timed_event_ms = 0.0, timed_gather_ms = 0.0;
+  int timed_iterations = 0;
   // per-info LOR, pixel index and up; infos sorted by pixel (CSR)
   int* info_lor = nullptr;
   int* ipix = nullptr;
@@ -1098,6 +1103,8 @@ struct hybrid_mlem_ctx {
       if (p) (void)hipFree(p);
     if (t0) (void)hipEventDestroy(t0);
     if (t1) (void)hipEventDestroy(t1);
+    for (auto* v : {&t_start, &t_mid, &t_end})
+      for (hipEvent_t e : *v) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
   int64_t n_voxels() const { return (int64_t)grid.n_pixels * grid.n_planes; }
@@ -1950,7 +1957,7 @@ int choose_algorithm(hybrid_mlem_ctx* c) {
   return STRIP_MLEM_OK;
 }
 
-int iterate_lor_space(hybrid_mlem_ctx* c) {
+int iterate_lor_space(hybrid_mlem_ctx* c, hipEvent_t mid) {
   const Grid& g = c->grid;
   const int64_t nv = c->n_voxels();
   const float kexp = -c->kernel.iz * 1.44269504088896340736f;
@@ -1975,6 +1982,7 @@ int iterate_lor_space(hybrid_mlem_ctx* c) {
                          c->fac_off, c->lor_events, c->lor_order, c->n_active,
                          c->queue, c->inv, c->fac, c->bp, c->extra);
   }
+  if (mid) HY_TRY(hipEventRecord(mid, c->stream));
   const unsigned gblocks =
       (unsigned)std::min<int64_t>((g.n_pixels + 3) / 4, 65536);
   hipLaunchKernelGGL(lor_gather_kernel, dim3(gblocks), dim3(256), 0,
@@ -1999,13 +2007,20 @@ int hybrid_mlem_iterate(hybrid_mlem_ctx* c, int n_iterations) {
   const int64_t n = c->ev.n;
   const unsigned blocks =
       (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + 3) / 4, 16384));
+  for (auto* v : {&c->t_start, &c->t_mid, &c->t_end})
+    while ((int)v->size() < n_iterations) {
+      hipEvent_t e;
+      HY_TRY(hipEventCreate(&e));
+      v->push_back(e);
+    }
   for (int it = 0; it < n_iterations; ++it) {
     if (c->algorithm == 1)
       hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(nv)), dim3(256), 0,
                          c->stream, c->out, nv, 0.0f);
+    HY_TRY(hipEventRecord(c->t_start[it], c->stream));
     HY_TRY(hipEventRecord(c->t0, c->stream));
     if (c->algorithm == 2) {
-      HY_CHECK(iterate_lor_space(c));
+      HY_CHECK(iterate_lor_space(c, c->t_mid[it]));
     } else if (n) {
       if (c->semantics == HYBRID_MLEM_CPU)
         hipLaunchKernelGGL(event_kernel<true>, dim3(blocks), dim3(256), 0,
@@ -2019,11 +2034,31 @@ int hybrid_mlem_iterate(hybrid_mlem_ctx* c, int n_iterations) {
                            n, c->rho, c->sens, c->out);
     }
     HY_TRY(hipGetLastError());
+    if (c->algorithm != 2) HY_TRY(hipEventRecord(c->t_mid[it], c->stream));
     HY_TRY(hipEventRecord(c->t1, c->stream));
+    HY_TRY(hipEventRecord(c->t_end[it], c->stream));
     std::swap(c->rho, c->out);
   }
   HY_TRY(hipStreamSynchronize(c->stream));
   if (n_iterations > 0) HY_TRY(hipEventElapsedTime(&c->last_ms, c->t0, c->t1));
+  c->timed_event_ms = c->timed_gather_ms = 0.0;
+  for (int it = 0; it < n_iterations; ++it) {
+    float a = 0.0f, b = 0.0f;
+    HY_TRY(hipEventElapsedTime(&a, c->t_start[it], c->t_mid[it]));
+    HY_TRY(hipEventElapsedTime(&b, c->t_mid[it], c->t_end[it]));
+    c->timed_event_ms += a;
+    c->timed_gather_ms += b;
+  }
+  c->timed_iterations = n_iterations;
+  return STRIP_MLEM_OK;
+}
+
+int hybrid_mlem_get_timing(hybrid_mlem_ctx* c, double* event_kernels_ms,
+                           double* gather_ms, int* n_iterations) {
+  if (!c) return strip::fail(STRIP_MLEM_ERR_ARGUMENT, "ctx is NULL");
+  if (event_kernels_ms) *event_kernels_ms = c->timed_event_ms;
+  if (gather_ms) *gather_ms = c->timed_gather_ms;
+  if (n_iterations) *n_iterations = c->timed_iterations;
   return STRIP_MLEM_OK;
 }
 

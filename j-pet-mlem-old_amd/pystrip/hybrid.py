@@ -334,7 +334,8 @@ def exported_symbols():
             "hybrid_mlem_get_weights", "hybrid_mlem_iterate",
             "hybrid_mlem_last_iteration_ms", "hybrid_mlem_voxel_visits",
             "hybrid_mlem_set_algorithm", "hybrid_mlem_get_algorithm",
-            "hybrid_mlem_set_frame_events", "hybrid_mlem_device_name"]
+            "hybrid_mlem_set_frame_events", "hybrid_mlem_device_name",
+            "hybrid_mlem_get_timing"]
 
 
 def load_library():
@@ -371,6 +372,9 @@ def load_library():
         "hybrid_mlem_set_frame_events": (ctypes.c_int, [P, P, i64]),
         "hybrid_mlem_device_name": (ctypes.c_int, [P, ctypes.c_char_p,
                                                    ctypes.c_int]),
+        "hybrid_mlem_get_timing": (ctypes.c_int, [
+            P, ctypes.POINTER(ctypes.c_double),
+            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -541,6 +545,14 @@ class HybridReconstruction:
         self._check(self._lib.hybrid_mlem_last_iteration_ms(
             self._ctx, ctypes.byref(ms)))
         return ms.value
+
+    def timing(self):
+        """(event kernels ms, gather ms, iterations) summed over the last
+        iterate call."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        self._check(self._lib.hybrid_mlem_get_timing(
+            self._ctx, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)))
+        return a.value, b.value, n.value
 
     def voxel_visits(self):
         v = ctypes.c_int64()
