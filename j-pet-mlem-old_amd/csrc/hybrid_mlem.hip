@@ -477,7 +477,9 @@ __global__ __launch_bounds__(256) void event_kernel(
 // atomics; so is a repeated visit's update under the kernel-cache
 // semantics.
 
-constexpr int kEventInfoCap = 256;  // infos of one event held in LDS
+// LDS words of a wavefront's area: an event's two factors per info and one
+// offset per plane
+constexpr int kWaveWords = 512;
 // dynamic LDS of the back-projection: 160 KiB less room for its static
 // queue slot
 constexpr size_t kLdsBytes = 160 * 1024 - 256;
@@ -626,26 +628,60 @@ __device__ inline float lane_float(float v, int j) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
+// The factors (A / D_e, dy tan) of a back-projection unit: event lane j,
+// rows r0 .. min(r0 + 64, hi_j) - 1 of its box; lane t holds row r0 + t.
+__device__ inline float2 unit_factors(const float2* __restrict__ fac, int a0,
+                                      int hi, int64_t off, int j, int r0,
+                                      int lane) {
+  const int ja0 = lane_int(a0, j), jhi = lane_int(hi, j);
+  const int64_t joff =
+      (int64_t)(((uint64_t)(uint32_t)lane_int((int)(off >> 32), j) << 32) |
+                (uint32_t)lane_int((int)off, j));
+  return lane < jhi - r0 ? fac[joff + (r0 - ja0) + lane]
+                         : make_float2(0.f, 0.f);
+}
+
+// Inclusive prefix sums over the 64 lanes.
+__device__ inline int wave_scan_int(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+__device__ inline float wave_scan_float(float v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const float u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
 // One LOR-space iteration's event work.  A persistent grid, one block per
 // CU, takes LORs from a queue (LORs with events, most events first).  For
 // each LOR the block's LDS slab (the LOR's infos x all planes, laid out as
 // its rows of bp) first holds P = G rho s of every (info, plane).
 //   Forward: the waves take the LOR's events in turn (parameters read 64 at
-//   a time, one per lane); per event the per-info factors go to the wave's
-//   LDS, D_e = sum A_ei exp(-iz b^2) P over the box (plus the extra info in
-//   full), then 1 / D_e and the back-projection's factors F = A_ei / D_e
-//   and dy tan are written to global memory, and the extra info's update
-//   to the image-space extra sums (twice, for the visit it repeats, under
-//   the kernel-cache semantics, which also leaves that visit's F at 0).
-//   Backward: the slab is cleared; its rows are dealt to the waves in
-//   blocks (block b to wave b mod n_waves); every wave reads
-//   the events 64 at a time, picks by ballot those whose box meets its
-//   rows, and adds F exp(-iz b^2) over its rows of each box.  Each slab
-//   word has one writer, so the sums are plain LDS read-modify-writes (LDS
-//   float atomics run at about one lane per clock).
-// The slab is finally written to bp.  n_waves must be a power of two; the
-// row blocks hold 4 to 16 rows (a power of two), as many as still give every
-// wave a block.
+//   a time, one per lane); per event the per-info factors (A, dy tan) and
+//   the box's plane offsets go to the wave's LDS area, D_e = sum A_ei
+//   exp(-iz b^2) P over the box (plus the extra info in full), then 1 / D_e
+//   and the back-projection's factors F = A_ei / D_e and dy tan are written
+//   to global memory, and the extra info's update to the image-space extra
+//   sums (twice, for the visit it repeats, under the kernel-cache
+//   semantics, which also leaves that visit's F at 0).
+//   Backward: the slab is cleared and its rows split into one contiguous
+//   range per wave with about equal (row, plane) pair counts over the
+//   events' boxes; every wave reads the events 64 at a time, picks by
+//   ballot those whose box meets its range, and adds F exp(-iz b^2) over
+//   its rows of each box, up to 64 rows per unit, loading the next unit's
+//   factors while it sums the current one.  Each slab word has one writer,
+//   so the sums are plain LDS read-modify-writes (LDS float atomics run at
+//   about one lane per clock), in event order per word.
+// The slab is finally written to bp.
 
 template <bool LAST_WRITE>
 __global__ __launch_bounds__(1024) void lor_iteration_kernel(
@@ -660,7 +696,7 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     const int* __restrict__ lor_order, int n_active, int* __restrict__ queue,
     float* __restrict__ inv, float2* __restrict__ fac, float* __restrict__ bp,
     float* __restrict__ extra) {
-  extern __shared__ float lds[];
+  extern __shared__ __align__(16) float lds[];
   __shared__ int s_next;
   const int np = g.n_planes;
   const int n_waves = blockDim.x >> 6;
@@ -675,19 +711,17 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     const int e0 = lor_events[lor], e1 = lor_events[lor + 1];
     const int2 lr = ranges[lor];
     const int n_lor = lr.y - lr.x;
-    int rb_shift = 2;  // row blocks of 1 << rb_shift rows
-    while (rb_shift < 4 && (n_lor >> (rb_shift + 1)) >= n_waves) ++rb_shift;
-    const int rb = 1 << rb_shift;
-    const int span = rb * n_waves;
     float* slab = lds;
     float* s_up = slab + (size_t)n_lor * np;
-    float* wA = s_up + n_lor + (size_t)wave * 2 * kEventInfoCap;
-    float* wT = wA + kEventInfoCap;
+    // the waves' areas of kWaveWords floats (8-byte aligned): an event's
+    // factors (A, dy tan) per info, then its planes' offsets
+    float* areas = lds + (((size_t)n_lor * np + n_lor + 2) & ~(size_t)1);
+    float* warea = areas + (size_t)wave * kWaveWords;
+    float2* wAT = (float2*)warea;
     const float4 sa = seg_a[lor];
     const float R = seg_b[lor].w;
-    // the rows' pixel and sensitivity, staged in the (still unused) factor
-    // area of the waves
-    int* s_pix = (int*)(s_up + n_lor);
+    // the rows' pixel and sensitivity, staged in the (still unused) areas
+    int* s_pix = (int*)areas;
     float* s_sens = (float*)(s_pix + n_lor);
     for (int t = threadIdx.x; t < n_lor; t += blockDim.x) {
       s_up[t] = upi[lr.x + t];
@@ -751,19 +785,25 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
         for (int t = lane; t < n_info; t += 64) {
           float A, T;
           info_factors(k, ef, R, s_up[a0 + t], A, T);
-          wA[t] = A;
-          wT[t] = T;
+          wAT[t] = make_float2(A, T);
         }
+        // plane offsets z_c - right of the box's planes
+        float* wc = warea + 2 * n_info;
+        for (int r = lane; r < n_pl; r += 64)
+          wc[r] = plane_centre(g, pb + r) - ef.y;
         wave_lds_sync();
         float den = 0.0f;
-        const int total = n_info * (n_pl > 0 ? n_pl : 0);
-        BoxWalk wk(lane, n_pl);
-        for (int idx = lane; idx < total; idx += 64) {
-          const int z = pb + wk.r;
-          const float d = (plane_centre(g, z) - ef.y) - wT[wk.q];
-          const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-          den += wA[wk.q] * K * slab[(a0 + wk.q) * np + z];
-          wk.step();
+        {
+          const int total = n_info * (n_pl > 0 ? n_pl : 0);
+          BoxWalk wk(lane, n_pl);
+          const float* pbox = slab + a0 * np + pb;
+          for (int idx = lane; idx < total; idx += 64) {
+            const float2 at = wAT[wk.q];
+            const float d = wc[wk.r] - at.y;
+            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+            den += at.x * K * pbox[__mul24(wk.q, np) + wk.r];
+            wk.step();
+          }
         }
         int xx = 0, yy = 0, xpixel = -1;
         if (ei.z > lr.y && n_pl > 0) {
@@ -784,9 +824,11 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
         if (lane == 0) inv[e] = iv;
         if (iv > 0.0f) {
           float2* f = fac + fac_off[e];
-          for (int t = lane; t < n_info; t += 64)
+          for (int t = lane; t < n_info; t += 64) {
+            const float2 at = wAT[t];
             f[t] = make_float2(
-                (LAST_WRITE && ei.y + t == dup) ? 0.0f : wA[t] * iv, wT[t]);
+                (LAST_WRITE && ei.y + t == dup) ? 0.0f : at.x * iv, at.y);
+          }
           if (xpixel >= 0) {
             const float mult = (LAST_WRITE && dup >= 0) ? 2.0f : 1.0f;
             for (int z = pb + lane; z < pe; z += 64) {
@@ -805,10 +847,58 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     // waves of this block read them back from global memory, which the
     // barrier's workgroup-scope release and acquire order (one CU, one L1)
     __syncthreads();
+    // back-projection over contiguous row ranges, one per wave, of about
+    // equal (row, plane) pair counts: the event boxes' row weights (n_pl per
+    // covered row) as differences, then their running sum C(r), in the
+    // (now free) factor area of the waves
+    int* s_dw = (int*)s_up;
+    float* s_C = (float*)s_dw;
     for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) slab[t] = 0.0f;
+    for (int t = threadIdx.x; t <= n_lor; t += blockDim.x) s_dw[t] = 0;
     __syncthreads();
-
-    // back-projection
+    for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+      const int4 ei = ev_i[e];
+      const int n_pl = (ei.w >> 16) - (ei.w & 0xffff);
+      const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+      const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
+      if (n_pl > 0 && a1 > a0 && inv[e] > 0.0f) {
+        atomicAdd(&s_dw[a0], n_pl);
+        atomicAdd(&s_dw[a1], -n_pl);
+      }
+    }
+    __syncthreads();
+    if (wave == 0) {
+      int carry_v = 0;
+      float carry_c = 0.0f;
+      for (int c0 = 0; c0 < n_lor; c0 += 64) {
+        const int r = c0 + lane;
+        const int v = wave_scan_int(r < n_lor ? s_dw[r] : 0) + carry_v;
+        carry_v = lane_int(v, 63);
+        const float c = wave_scan_float((float)v) + carry_c;
+        carry_c = lane_float(c, 63);
+        if (r < n_lor) s_C[r] = c;
+      }
+    }
+    __syncthreads();
+    // row r goes to wave floor(n_waves (C(r - 1) + C(r)) / 2 / C(n_lor - 1));
+    // wave w takes rows [#{owner < w}, #{owner <= w}), a partition of the
+    // rows into contiguous ranges whatever the rounding
+    int r_start = 0, r_end = 0;
+    {
+      const float W = n_lor > 0 ? s_C[n_lor - 1] : 0.0f;
+      const float scale = W > 0.0f ? (float)n_waves / W : 0.0f;
+      for (int c0 = 0; c0 < n_lor; c0 += 64) {
+        const int r = c0 + lane;
+        int o = n_waves;
+        if (r < n_lor) {
+          const float mid = 0.5f * ((r > 0 ? s_C[r - 1] : 0.0f) + s_C[r]);
+          o = (int)(mid * scale);
+          o = o < n_waves - 1 ? o : n_waves - 1;
+        }
+        r_start += __popcll(__ballot(r < n_lor && o < wave));
+        r_end += __popcll(__ballot(r < n_lor && o <= wave));
+      }
+    }
     for (int base = e0; base < e1; base += 64) {
       const int e = base + lane;
       int4 ei = make_int4(0, 0, 0, 0);
@@ -823,46 +913,56 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
       const int pb = ei.w & 0xffff, pe = ei.w >> 16;
       const int ie_l = ei.z < lr.y ? ei.z : lr.y;
       const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
-      int blk = a0 >> rb_shift;
-      blk += (wave - blk) & (n_waves - 1);
-      const bool mine = e < e1 && iv > 0.0f && pe > pb && a1 > a0 &&
-                        (blk << rb_shift) < a1;
+      // this wave's rows of the event's box
+      const int lo = a0 > r_start ? a0 : r_start;
+      const int hi = a1 < r_end ? a1 : r_end;
+      const bool mine = e < e1 && iv > 0.0f && pe > pb && hi > lo;
       uint64_t mask = __ballot(mine);
-      while (mask) {
-        const int j = __builtin_ctzll(mask);
+      // the wave's units (event j, up to 64 rows from r0) in order; the
+      // factors of the next unit are loaded while the current one is summed
+      int nj = -1, nr0 = 0;
+      float2 nf = make_float2(0.f, 0.f);
+      if (mask) {
+        nj = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const int ja0 = lane_int(a0, j), ja1 = lane_int(a1, j);
+        nr0 = lane_int(lo, nj);
+      }
+      if (nj >= 0) nf = unit_factors(fac, a0, hi, off, nj, nr0, lane);
+      while (nj >= 0) {
+        const int j = nj, r0 = nr0;
+        const float2 f = nf;
+        const int jhi = lane_int(hi, j);
         const int jpb = lane_int(pb, j), jpe = lane_int(pe, j);
-        const int jblk = lane_int(blk, j);
         const float jright = lane_float(right, j);
-        const int64_t joff =
-            (int64_t)(((uint64_t)(uint32_t)lane_int((int)(off >> 32), j)
-                       << 32) |
-                      (uint32_t)lane_int((int)off, j));
-        const int n_pl = jpe - jpb;
-        BoxWalk start(lane, n_pl);
-        for (int r0 = jblk << rb_shift; r0 < ja1; r0 += span) {
-          const int lo = r0 > ja0 ? r0 : ja0;
-          const int hi = r0 + rb < ja1 ? r0 + rb : ja1;
-          const int rows = hi - lo;
-          float2 f = make_float2(0.f, 0.f);
-          if (lane < rows) f = fac[joff + (lo - ja0) + lane];
-          const int total = rows * n_pl;
-          BoxWalk wk = start;
-          // a uniform loop: every lane takes part in the shuffles, which
-          // read lanes 0 .. rows - 1
-          for (int step = 0; step < total; step += 64) {
-            const float F = __shfl(f.x, wk.q & 63, 64);
-            const float T = __shfl(f.y, wk.q & 63, 64);
-            if (step + lane < total && F != 0.0f) {
-              const int z = jpb + wk.r;
-              const float d = (plane_centre(g, z) - jright) - T;
-              const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-              slab[(lo + wk.q) * np + z] += F * K;
-            }
-            wk.step();
-          }
+        if (r0 + 64 < jhi) {
+          nr0 = r0 + 64;
+        } else if (mask) {
+          nj = __builtin_ctzll(mask);
+          mask &= mask - 1;
+          nr0 = lane_int(lo, nj);
+        } else {
+          nj = -1;
         }
+        if (nj >= 0) nf = unit_factors(fac, a0, hi, off, nj, nr0, lane);
+        const int rows = (r0 + 64 < jhi ? r0 + 64 : jhi) - r0;
+        const int n_pl = jpe - jpb;
+        const int total = rows * n_pl;
+        // the unit's row factors and plane offsets in the wave's area
+        if (lane < rows) wAT[lane] = f;
+        float* wc = warea + 2 * rows;
+        for (int r = lane; r < n_pl; r += 64)
+          wc[r] = plane_centre(g, jpb + r) - jright;
+        wave_lds_sync();
+        BoxWalk wk(lane, n_pl);
+        float* sbox = slab + r0 * np + jpb;
+        for (int idx = lane; idx < total; idx += 64) {
+          const float2 ft = wAT[wk.q];
+          const float d = wc[wk.r] - ft.y;
+          const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+          sbox[__mul24(wk.q, np) + wk.r] += ft.x * K;
+          wk.step();
+        }
+        wave_lds_sync();  // the area is read before the next unit's
       }
     }
     __syncthreads();
@@ -972,7 +1072,8 @@ __global__ __launch_bounds__(256) void permute_events_kernel(
   atomicAdd(&lor_count[ei.x], 1);
   const int2 lr = ranges[ei.x];
   const int ie_l = ei.z < lr.y ? ei.z : lr.y;
-  atomicMax(max_infos, ie_l - ei.y);
+  const int n_pl = (ei.w >> 16) - (ei.w & 0xffff);
+  atomicMax(max_infos, 2 * (ie_l - ei.y) + (n_pl > 0 ? n_pl : 0));
   n_info[e] = ie_l - ei.y;
 }
 
@@ -1089,7 +1190,7 @@ struct hybrid_mlem_ctx {
   int* queue = nullptr;      // the back-projection's LOR counter
   int n_active = 0;
   int n_cus = 0;
-  int max_event_infos = 0;
+  int max_event_words = 0;
   int64_t pl_capacity = 0;
 
   ~hybrid_mlem_ctx() {
@@ -1238,17 +1339,17 @@ int build_pixel_index(hybrid_mlem_ctx* c) {
 // LOR-space tables: G (recomputed with every weight preparation) and bp
 // (rows of LORs without events stay zero), the extra accumulator, and the
 // iteration kernel's block: the largest LOR's slab (infos x planes) and
-// its infos' up with 2 KiB of per-event factors per wave, within 160 KiB
-// of LDS, 16 waves (or 8, 4, 2 where LDS is short); the tables' cell
-// indices must fit in 31 bits.
+// its infos' up with a 2 KiB area per wave (an event's factors and plane
+// offsets), within 160 KiB of LDS, 16 waves (or 8, 4, 2 where LDS is
+// short); the tables' cell indices must fit in 31 bits.
 // Where that does not fit, or the tables do not fit in device memory, the
 // context stays on the wavefront-per-event algorithm.
 int prepare_tables(hybrid_mlem_ctx* c) {
   const Grid& g = c->grid;
   const size_t slab =
-      sizeof(float) * (size_t)c->max_lor_infos * ((size_t)g.n_planes + 1);
-  const size_t per_wave = sizeof(float) * 2 * kEventInfoCap;
-  int waves = 16;  // a power of two (the row blocks' wave index)
+      sizeof(float) * ((size_t)c->max_lor_infos * ((size_t)g.n_planes + 1) + 2);
+  const size_t per_wave = sizeof(float) * kWaveWords;
+  int waves = 16;
   while (waves > 2 && slab + waves * per_wave > kLdsBytes) waves /= 2;
   // the factor area also stages the rows' pixel and sensitivity
   if (slab + waves * per_wave > kLdsBytes ||
@@ -1450,7 +1551,7 @@ int build_plan(hybrid_mlem_ctx* c) {
     HY_CHECK(device_alloc(&c->fac, total));
     c->fac_capacity = total;
   }
-  c->max_event_infos = host_stats[0];
+  c->max_event_words = host_stats[0];
   c->n_active = n > 0 ? host_stats[1] : 0;
   c->plan_valid = true;
   return STRIP_MLEM_OK;
@@ -1945,11 +2046,12 @@ int choose_algorithm(hybrid_mlem_ctx* c) {
     return STRIP_MLEM_OK;
   }
   if (!c->plan_valid) HY_CHECK(build_plan(c));
-  if (c->max_event_infos > kEventInfoCap) {
+  if (c->max_event_words > kWaveWords) {
     if (c->algorithm_request == 2)
       return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
-                         "an event spans more than " +
-                             std::to_string(kEventInfoCap) + " infos");
+                         "an event's infos and planes need more than " +
+                             std::to_string(kWaveWords) +
+                             " words of LDS (two per info, one per plane)");
     c->algorithm = 1;
     return STRIP_MLEM_OK;
   }
