@@ -186,8 +186,7 @@ int hybrid_mlem_iterate(hybrid_mlem_ctx* ctx, int n_iterations);
  *      infos x planes, then gathered per voxel over the pixel's infos;
  *   0: automatic (default): 2 where its tables fit in device memory, the
  *      largest LOR's infos x planes fit in LDS with room for two
- *      wavefronts, and no event needs more than 512 words of a wavefront's
- *      LDS area (two per info of its box, one per plane), else 1.
+ *      wavefronts, and no event spans more than 256 infos, else 1.
  * hybrid_mlem_get_algorithm returns the algorithm of the last iteration. */
 int hybrid_mlem_set_algorithm(hybrid_mlem_ctx* ctx, int algorithm);
 int hybrid_mlem_get_algorithm(hybrid_mlem_ctx* ctx);

@@ -477,8 +477,7 @@ __global__ __launch_bounds__(256) void event_kernel(
 // atomics; so is a repeated visit's update under the kernel-cache
 // semantics.
 
-// LDS words of a wavefront's area: an event's two factors per info and one
-// offset per plane
+// LDS words of a wavefront's area: an event's two factors per info
 constexpr int kWaveWords = 512;
 // dynamic LDS of the back-projection: 160 KiB less room for its static
 // queue slot
@@ -597,6 +596,28 @@ struct BoxWalk {
   }
 };
 
+// The walk of an event's rows x planes with each lane's plane fixed: for a
+// chunk of w <= 64 planes from plane chunk0 of the box, lane l takes plane
+// chunk0 + l mod w and rows l / w, l / w + rpi, ... (rpi = 64 / w rows per
+// step); lanes from rpi w on stay idle.
+struct PlaneWalk {
+  int q0, r, rpi;
+  bool active;
+  __device__ PlaneWalk(int lane, int n_pl, int chunk0) {
+    const int w = n_pl - chunk0 < 64 ? n_pl - chunk0 : 64;
+    // quotients by w from a float reciprocal, exact after one correction
+    const float rcp = __builtin_amdgcn_rcpf((float)w);
+    rpi = (int)(64.0f * rcp);
+    rpi += (rpi + 1) * w <= 64 ? 1 : 0;
+    rpi -= rpi * w > 64 ? 1 : 0;
+    q0 = (int)((float)lane * rcp);
+    q0 += (q0 + 1) * w <= lane ? 1 : 0;
+    q0 -= q0 * w > lane ? 1 : 0;
+    r = chunk0 + lane - q0 * w;
+    active = q0 < rpi;
+  }
+};
+
 // Event parameters of 64 events, one per lane, read back for one event at a
 // time with readlane (wave-uniform values in scalar registers).
 struct EventBatch {
@@ -666,13 +687,13 @@ __device__ inline float wave_scan_float(float v) {
 // each LOR the block's LDS slab (the LOR's infos x all planes, laid out as
 // its rows of bp) first holds P = G rho s of every (info, plane).
 //   Forward: the waves take the LOR's events in turn (parameters read 64 at
-//   a time, one per lane); per event the per-info factors (A, dy tan) and
-//   the box's plane offsets go to the wave's LDS area, D_e = sum A_ei
-//   exp(-iz b^2) P over the box (plus the extra info in full), then 1 / D_e
-//   and the back-projection's factors F = A_ei / D_e and dy tan are written
-//   to global memory, and the extra info's update to the image-space extra
-//   sums (twice, for the visit it repeats, under the kernel-cache
-//   semantics, which also leaves that visit's F at 0).
+//   a time, one per lane); per event the per-info factors (A, dy tan) go
+//   to the wave's LDS area, D_e = sum A_ei exp(-iz b^2) P over the box
+//   (each lane on one plane, rows in steps; plus the extra info in full),
+//   then 1 / D_e and the back-projection's factors F = A_ei / D_e and
+//   dy tan are written to global memory, and the extra info's update to the
+//   image-space extra sums (twice, for the visit it repeats, under the
+//   kernel-cache semantics, which also leaves that visit's F at 0).
 //   Backward: the slab is cleared and its rows split into one contiguous
 //   range per wave with about equal (row, plane) pair counts over the
 //   events' boxes; every wave reads the events 64 at a time, picks by
@@ -714,7 +735,7 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     float* slab = lds;
     float* s_up = slab + (size_t)n_lor * np;
     // the waves' areas of kWaveWords floats (8-byte aligned): an event's
-    // factors (A, dy tan) per info, then its planes' offsets
+    // factors (A, dy tan) per info
     float* areas = lds + (((size_t)n_lor * np + n_lor + 2) & ~(size_t)1);
     float* warea = areas + (size_t)wave * kWaveWords;
     float2* wAT = (float2*)warea;
@@ -787,22 +808,20 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
           info_factors(k, ef, R, s_up[a0 + t], A, T);
           wAT[t] = make_float2(A, T);
         }
-        // plane offsets z_c - right of the box's planes
-        float* wc = warea + 2 * n_info;
-        for (int r = lane; r < n_pl; r += 64)
-          wc[r] = plane_centre(g, pb + r) - ef.y;
         wave_lds_sync();
         float den = 0.0f;
-        {
-          const int total = n_info * (n_pl > 0 ? n_pl : 0);
-          BoxWalk wk(lane, n_pl);
-          const float* pbox = slab + a0 * np + pb;
-          for (int idx = lane; idx < total; idx += 64) {
-            const float2 at = wAT[wk.q];
-            const float d = wc[wk.r] - at.y;
+        for (int c0 = 0; c0 < n_pl; c0 += 64) {
+          const PlaneWalk pw(lane, n_pl, c0);
+          if (!pw.active) continue;
+          const float c = plane_centre(g, pb + pw.r) - ef.y;
+          const float* prow = slab + __mul24(a0 + pw.q0, np) + pb + pw.r;
+          const int stride = __mul24(pw.rpi, np);
+          for (int q = pw.q0; q < n_info; q += pw.rpi) {
+            const float2 at = wAT[q];
+            const float d = c - at.y;
             const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-            den += at.x * K * pbox[__mul24(wk.q, np) + wk.r];
-            wk.step();
+            den += at.x * K * *prow;
+            prow += stride;
           }
         }
         int xx = 0, yy = 0, xpixel = -1;
@@ -946,21 +965,22 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
         if (nj >= 0) nf = unit_factors(fac, a0, hi, off, nj, nr0, lane);
         const int rows = (r0 + 64 < jhi ? r0 + 64 : jhi) - r0;
         const int n_pl = jpe - jpb;
-        const int total = rows * n_pl;
-        // the unit's row factors and plane offsets in the wave's area
+        // the unit's row factors in the wave's area
         if (lane < rows) wAT[lane] = f;
-        float* wc = warea + 2 * rows;
-        for (int r = lane; r < n_pl; r += 64)
-          wc[r] = plane_centre(g, jpb + r) - jright;
         wave_lds_sync();
-        BoxWalk wk(lane, n_pl);
-        float* sbox = slab + r0 * np + jpb;
-        for (int idx = lane; idx < total; idx += 64) {
-          const float2 ft = wAT[wk.q];
-          const float d = wc[wk.r] - ft.y;
-          const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-          sbox[__mul24(wk.q, np) + wk.r] += ft.x * K;
-          wk.step();
+        for (int c0 = 0; c0 < n_pl; c0 += 64) {
+          const PlaneWalk pw(lane, n_pl, c0);
+          if (!pw.active) continue;
+          const float c = plane_centre(g, jpb + pw.r) - jright;
+          float* srow = slab + __mul24(r0 + pw.q0, np) + jpb + pw.r;
+          const int stride = __mul24(pw.rpi, np);
+          for (int q = pw.q0; q < rows; q += pw.rpi) {
+            const float2 ft = wAT[q];
+            const float d = c - ft.y;
+            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+            *srow += ft.x * K;
+            srow += stride;
+          }
         }
         wave_lds_sync();  // the area is read before the next unit's
       }
@@ -1072,8 +1092,7 @@ __global__ __launch_bounds__(256) void permute_events_kernel(
   atomicAdd(&lor_count[ei.x], 1);
   const int2 lr = ranges[ei.x];
   const int ie_l = ei.z < lr.y ? ei.z : lr.y;
-  const int n_pl = (ei.w >> 16) - (ei.w & 0xffff);
-  atomicMax(max_infos, 2 * (ie_l - ei.y) + (n_pl > 0 ? n_pl : 0));
+  atomicMax(max_infos, ie_l - ei.y);
   n_info[e] = ie_l - ei.y;
 }
 
@@ -1190,7 +1209,7 @@ struct hybrid_mlem_ctx {
   int* queue = nullptr;      // the back-projection's LOR counter
   int n_active = 0;
   int n_cus = 0;
-  int max_event_words = 0;
+  int max_event_infos = 0;
   int64_t pl_capacity = 0;
 
   ~hybrid_mlem_ctx() {
@@ -1339,9 +1358,9 @@ int build_pixel_index(hybrid_mlem_ctx* c) {
 // LOR-space tables: G (recomputed with every weight preparation) and bp
 // (rows of LORs without events stay zero), the extra accumulator, and the
 // iteration kernel's block: the largest LOR's slab (infos x planes) and
-// its infos' up with a 2 KiB area per wave (an event's factors and plane
-// offsets), within 160 KiB of LDS, 16 waves (or 8, 4, 2 where LDS is
-// short); the tables' cell indices must fit in 31 bits.
+// its infos' up with a 2 KiB area per wave (an event's factors), within
+// 160 KiB of LDS, 16 waves (or 8, 4, 2 where LDS is short); the tables'
+// cell indices must fit in 31 bits.
 // Where that does not fit, or the tables do not fit in device memory, the
 // context stays on the wavefront-per-event algorithm.
 int prepare_tables(hybrid_mlem_ctx* c) {
@@ -1551,7 +1570,7 @@ int build_plan(hybrid_mlem_ctx* c) {
     HY_CHECK(device_alloc(&c->fac, total));
     c->fac_capacity = total;
   }
-  c->max_event_words = host_stats[0];
+  c->max_event_infos = host_stats[0];
   c->n_active = n > 0 ? host_stats[1] : 0;
   c->plan_valid = true;
   return STRIP_MLEM_OK;
@@ -2046,12 +2065,11 @@ int choose_algorithm(hybrid_mlem_ctx* c) {
     return STRIP_MLEM_OK;
   }
   if (!c->plan_valid) HY_CHECK(build_plan(c));
-  if (c->max_event_words > kWaveWords) {
+  if (2 * c->max_event_infos > kWaveWords) {
     if (c->algorithm_request == 2)
       return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
-                         "an event's infos and planes need more than " +
-                             std::to_string(kWaveWords) +
-                             " words of LDS (two per info, one per plane)");
+                         "an event spans more than " +
+                             std::to_string(kWaveWords / 2) + " infos");
     c->algorithm = 1;
     return STRIP_MLEM_OK;
   }
