@@ -34,7 +34,8 @@ constexpr int kMoments = 3;  // per image: sum K x, sum K x Q, sum K x (w - Qd)
 // moments[(e * max_bands + band) * 3 * NIMG + 3 i + {0, 1, 2}] =
 //   sum K x_i, sum K x_i Q, sum K x_i (w - Qd)   over the band's pixels.
 // Same tiling as band_forward_kernel: one workgroup per task, the windows of
-// the NIMG images staged in LDS, one image row per lane.
+// the NIMG images (2H rows each, the pairs' bands being aligned to their
+// events) staged in LDS, one row of a pair's band per lane.
 template <int H, int NIMG, bool SPANS>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
     DeviceGeometry g, const BandRecord* __restrict__ rec,
@@ -47,10 +48,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
   const int b = task.x & 0xffff;
   const int c0 = task.w & 0xffff;
   const int y_base = b * H;
+  constexpr int WH = 2 * H;  // window rows
   // the columns the task's pairs reach (Wt: the window stride)
-  const int n_words = min(task.w >> 16, Wt) * H;
+  const int n_words = min(task.w >> 16, Wt) * WH;
   for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
-    const int col = c0 + i / H, iy = y_base + (i & (H - 1));
+    const int col = c0 + i / WH, iy = y_base + (i & (WH - 1));
     const bool in = col < g.n_z && iy < g.n_y;
     const int64_t at = (int64_t)col * g.n_y + iy;
     win[i] = in ? x0T[at] : 0.0f;
@@ -81,19 +83,24 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
       next = gather_batch<false>(rec, pair_event, nullptr, base + 64, end, lane);
     const int n_here = min(64, end - base);
     uint32_t sp = 0;
-    if (SPANS)
-      sp = fetch_span(spans, span_stride, lane_int(cur.e, 0),
-                      (uint32_t)lane_int((int)cur.by, 0), y_base + r);
+    if (SPANS) {
+      const uint32_t by0 = (uint32_t)lane_int((int)cur.by, 0);
+      sp = fetch_span(spans, span_stride, lane_int(cur.e, 0), by0,
+                      y_base + (int)(by0 & (H - 1)) + r);
+    }
     for (int j = 0; j < n_here; ++j) {
       const PairView v = pick(cur, j);
-      const int iy = y_base + r;
+      // this lane's row of the pair's band and its row in the window
+      const int wrow = (v.y0 & (H - 1)) + r;
+      const int iy = y_base + wrow;
       RowSpan s;
       if (SPANS) {
         const uint32_t word = (iy >= v.y0 && iy < v.y1) ? sp : 0u;
         __builtin_amdgcn_sched_barrier(0);
         const int j1 = min(j + 1, n_here - 1);
-        sp = fetch_span(spans, span_stride, lane_int(cur.e, j1),
-                        (uint32_t)lane_int((int)cur.by, j1), y_base + r);
+        const uint32_t by1 = (uint32_t)lane_int((int)cur.by, j1);
+        sp = fetch_span(spans, span_stride, lane_int(cur.e, j1), by1,
+                        y_base + (int)(by1 & (H - 1)) + r);
         s = stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy);
       } else {
         s = row_span(g, v.cz, v.cy, v.A, v.B, v.P, v.Q0, v.x0, v.x1, v.y0,
@@ -110,7 +117,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
       const float qa = s.qd * kExp2Scale;
       const float qb = 0.0f;
       const float db = __builtin_fmaf(qc * step, step, qb * step);
-      const float* src0 = win + (col0 - c0) * H + r;
+      const float* src0 = win + (col0 - c0) * WH + wrow;
       const float* src1 = src0 + n_words;
       float a0 = 0.f, aq0 = 0.f, a1 = 0.f, aq1 = 0.f;
       int k = 0;
@@ -123,11 +130,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
         float rr = __builtin_amdgcn_exp2f(dq);
 #pragma unroll
         for (int kk = 0; kk < 8; ++kk) {
-          const float t0 = e * src0[(k + kk) * LR * H];
+          const float t0 = e * src0[(k + kk) * LR * WH];
           a0 += t0;
           aq0 = __builtin_fmaf(t0, qq, aq0);
           if (NIMG == 2) {
-            const float t1 = e * src1[(k + kk) * LR * H];
+            const float t1 = e * src1[(k + kk) * LR * WH];
             a1 += t1;
             aq1 = __builtin_fmaf(t1, qq, aq1);
           }
@@ -141,11 +148,11 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_moments_kernel(
       for (; k < nq; ++k) {
         const float qq = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
         const float e = __builtin_amdgcn_exp2f(qq);
-        const float t0 = e * src0[k * LR * H];
+        const float t0 = e * src0[k * LR * WH];
         a0 += t0;
         aq0 = __builtin_fmaf(t0, qq, aq0);
         if (NIMG == 2) {
-          const float t1 = e * src1[k * LR * H];
+          const float t1 = e * src1[k * LR * WH];
           a1 += t1;
           aq1 = __builtin_fmaf(t1, qq, aq1);
         }
@@ -204,13 +211,11 @@ __global__ __launch_bounds__(256) void adjoint_event_kernel(
   int x0, x1, y0, y1;
   record_box(rec[e], x0, x1, y0, y1);
   float m[kMoments * 2] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (x1 > x0 && y1 > y0) {
-    const int nb = (y1 - 1) / H - y0 / H + 1;
-    for (int k = 0; k < nb; ++k) {
-      const float* src = moments + (e * max_bands + k) * (kMoments * NIMG);
+  const int nb = band_count(y0, y1, x1 > x0, H, true);
+  for (int k = 0; k < nb; ++k) {
+    const float* src = moments + (e * max_bands + k) * (kMoments * NIMG);
 #pragma unroll
-      for (int i = 0; i < kMoments * NIMG; ++i) m[i] += src[i];
-    }
+    for (int i = 0; i < kMoments * NIMG; ++i) m[i] += src[i];
   }
   const float D = m[0];
   float v1 = 0.f, v2 = 0.f, v = 0.f, vz = 0.f, vd = 0.f;
@@ -374,7 +379,7 @@ int launch_moments(BandPlan& plan, const DeviceGeometry& g, AdjointWork& wk,
                    std::string* err) {
   const PairList& L = plan.fwd;
   if (L.n_tasks == 0) return 0;
-  const size_t lds = (size_t)L.Wt * L.H * sizeof(float) * NIMG;
+  const size_t lds = (size_t)L.Wt * 2 * L.H * sizeof(float) * NIMG;
   if (lds > 160 * 1024) {
     if (err) *err = "moment windows do not fit in LDS";
     return -1;

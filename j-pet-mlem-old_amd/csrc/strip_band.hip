@@ -35,36 +35,35 @@ __global__ __launch_bounds__(256) void sum_counts_kernel(
 }
 
 __global__ __launch_bounds__(256) void count_pairs_kernel(
-    const BandRecord* __restrict__ rec, int64_t n, int H,
+    const BandRecord* __restrict__ rec, int64_t n, int H, int aligned,
     int* __restrict__ counts) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   int x0, x1, y0, y1;
   record_box(rec[e], x0, x1, y0, y1);
-  int c = 0;
-  if (x1 > x0 && y1 > y0) c = (y1 - 1) / H - y0 / H + 1;
-  counts[e] = c;
+  counts[e] = band_count(y0, y1, x1 > x0, H, aligned != 0);
 }
 
 // bounds[e * max_bands + k] = lo | hi << 16: the columns [lo, hi) reached
-// by the runs of band y0 / H + k of record e (the span table's runs, or the
-// bbox columns without one; an empty slice gives [x0, x0 + 1)).  One
-// wavefront per record, lane = row offset from the start of its first band,
-// so a band's rows are H consecutive lanes (H divides 64).
+// by the runs of band k of record e (the span table's runs, or the bbox
+// columns without one; an empty slice gives [x0, x0 + 1)).  One wavefront
+// per record, lane = row offset from the start of its first band, so a
+// band's rows are H consecutive lanes (H divides 64).
 __global__ __launch_bounds__(256) void band_bounds_kernel(
     const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
-    int stride, int64_t n, int H, int max_bands, int* __restrict__ bounds) {
+    int stride, int64_t n, int H, int aligned, int max_bands,
+    int* __restrict__ bounds) {
   const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (e >= n) return;  // uniform over the wavefront
   const int lane = threadIdx.x & 63;
   int x0, x1, y0, y1;
   record_box(rec[e], x0, x1, y0, y1);
   if (x1 <= x0 || y1 <= y0) return;
-  const int b0 = y0 / H;
-  const int rows = ((y1 - 1) / H - b0 + 1) * H;
+  const int row0 = band_first_row(y0, y0 / H, H, aligned != 0);
+  const int rows = band_count(y0, y1, true, H, aligned != 0) * H;
   for (int base = 0; base < rows; base += 64) {
     const int i = base + lane;
-    const int iy = b0 * H + i;
+    const int iy = row0 + i;
     int lo = 0x7fff, hi = 0;
     if (i < rows && iy >= y0 && iy < y1) {
       if (spans) {
@@ -317,22 +316,23 @@ __global__ __launch_bounds__(256) void band_inverse_kernel(
   int x0, x1, y0, y1;
   record_box(rec[e], x0, x1, y0, y1);
   float d = 0.0f;
-  if (x1 > x0 && y1 > y0) {
-    const int nb = (y1 - 1) / H - y0 / H + 1;
-    for (int k = 0; k < nb; ++k) d += partial[e * max_bands + k];
-  }
+  const int nb = band_count(y0, y1, x1 > x0, H, true);
+  for (int k = 0; k < nb; ++k) d += partial[e * max_bands + k];
   if (denominators) denominators[e] = d;
   if (inv) inv[e] = guard ? ((d > 0.0f) ? 1.0f / d : 0.0f) : 1.0f / d;
 }
 
 // Forward projection.  Workgroup = one task (pairs of one band and one x0
-// bucket); its rho window of Wt columns from c0 lives in LDS column-major
-// (word = (col - c0) * H + row; lanes of a column touch distinct rows, hence
-// distinct banks).  Wave w takes a contiguous share of the task's pairs, G
-// pairs per step: lane = g*(H*LR) + q*H + r covers row r of pair g of the
-// step and every LR-th column of the row's span from offset q (G = 64 /
-// (H*LR); with G > 1 the lane groups read their pairs' fields through the
-// LDS crossbar, so one instruction stream serves G pairs).
+// bucket); the pairs' bands are aligned to their events (a pair's H rows
+// start at the event's bbox top row, plus a multiple of H, inside image band
+// b), so the rho window holds 2H rows from b H, Wt columns from c0, in LDS
+// column-major (word = (col - c0) * 2H + row; lanes of a column touch
+// distinct rows, hence distinct banks).  Wave w takes a contiguous share of
+// the task's pairs, G pairs per step: lane = g*(H*LR) + q*H + r covers row r
+// of the band of pair g of the step and every LR-th column of the row's span
+// from offset q (G = 64 / (H*LR); with G > 1 the lane groups read their
+// pairs' fields through the LDS crossbar, so one instruction stream serves G
+// pairs).
 // MODE 0: partial D per pair; MODE 1: count the span pixels instead.
 template <int H, int LR, int MODE, bool REC, bool SPANS>
 __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
@@ -370,16 +370,16 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
   const int span_stride = d.span_stride;
   float* __restrict__ partial = d.partial;
   const int max_bands = d.fwd_max_bands;
+  constexpr int WH = 2 * H;  // window rows
   const int c0 = task.w & 0xffff;
   const int y_base = b * H;
-  const int iy = y_base + r;
   if (MODE == 0) {
     const float* __restrict__ rhoT = d.rhoT;
     // the columns the task's pairs reach
-    const int n_words = (task.w >> 16) * H;
+    const int n_words = (task.w >> 16) * WH;
     __syncthreads();  // the previous task's window is no longer read
     for (int i = threadIdx.x; i < n_words; i += blockDim.x) {
-      const int col = c0 + i / H, row = y_base + (i & (H - 1));
+      const int col = c0 + i / WH, row = y_base + (i & (WH - 1));
       win[i] = (col < g.n_z && row < g.n_y)
                    ? gload(rhoT + (int64_t)col * g.n_y + row)
                    : 0.0f;
@@ -407,20 +407,26 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
     // a step reads one set and fills the other with the next step's
     PairView va = step_pair<G, false>(cur, 0, grp), vb = va;
     uint32_t spa = 0, spb = 0;
-    if (SPANS) spa = fetch_span(spans, span_stride, va.e, (uint32_t)va.y0, iy);
+    if (SPANS)
+      spa = fetch_span(spans, span_stride, va.e, (uint32_t)va.y0,
+                       y_base + (va.y0 & (H - 1)) + r);
     auto run_step = [&](int j, const PairView& v, uint32_t sp, PairView& nv,
                         uint32_t& sp_next) {
       const bool valid = G == 1 || j + grp < n_here;
+      // this lane's row of the pair's band and its row in the window
+      const int wrow = (v.y0 & (H - 1)) + r;
+      const int iy = y_base + wrow;
       RowSpan s;
       if (SPANS) {
-        // rows outside the bbox read 0 (an empty run)
-        const uint32_t word = valid ? sp : 0u;
+        // rows past the bbox (the last band's) hold an empty run
+        const uint32_t word = (valid && iy < v.y1) ? sp : 0u;
         // the word is consumed before the next load is issued, so that load
         // stays in flight through this step
         __builtin_amdgcn_sched_barrier(0);
         const int jn = G == 1 ? min(j + 1, n_here - 1) : min(j + G, 64 - G);
         nv = step_pair<G, false>(cur, jn, grp);
-        sp_next = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0, iy);
+        sp_next = fetch_span(spans, span_stride, nv.e, (uint32_t)nv.y0,
+                             y_base + (nv.y0 & (H - 1)) + r);
         s = stored_span(g, word, v.cy, v.A, v.B, v.P, v.Q0, v.x0, iy);
       } else {
         if (G > 1) nv = step_pair<G, false>(cur, min(j + G, 64 - G), grp);
@@ -446,7 +452,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
                  s.tdy;
       const float qa = s.qd * kExp2Scale;
       const float qb = 0.0f;
-      const float* src = win + (col0 - c0) * H + r;
+      const float* src = win + (col0 - c0) * WH + wrow;
       float acc = 0.0f;
       int k = 0;
       if (REC) {
@@ -460,7 +466,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
         const float Ginv = __builtin_amdgcn_exp2f(-two_qc_step * step);
         const float db6 = __builtin_fmaf(6.0f * two_qc_step, step, db);
         const float db2 = __builtin_fmaf(2.0f * two_qc_step, step, db);
-        constexpr int S = LR * H;
+        constexpr int S = LR * WH;
         for (; k + 8 <= nq; k += 8) {
           const float e0 = __builtin_amdgcn_exp2f(
               __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa));
@@ -495,7 +501,7 @@ __global__ __launch_bounds__(64 * kFwdWaves) void band_forward_kernel(
       }
       for (; k < nq; ++k) {
         const float qq = __builtin_fmaf(__builtin_fmaf(qc, dz, qb), dz, qa);
-        acc = __builtin_fmaf(__builtin_amdgcn_exp2f(qq), src[k * LR * H],
+        acc = __builtin_fmaf(__builtin_amdgcn_exp2f(qq), src[k * LR * WH],
                              acc);
         dz += step;
       }
@@ -761,10 +767,8 @@ __global__ __launch_bounds__(256) void batch_inverse_kernel(
   int x0, x1, y0, y1;
   record_box(d.rec[e], x0, x1, y0, y1);
   float sum = 0.0f;
-  if (x1 > x0 && y1 > y0) {
-    const int nb = (y1 - 1) / H - y0 / H + 1;
-    for (int k = 0; k < nb; ++k) sum += d.partial[e * d.fwd_max_bands + k];
-  }
+  const int nb = band_count(y0, y1, x1 > x0, H, true);
+  for (int k = 0; k < nb; ++k) sum += d.partial[e * d.fwd_max_bands + k];
   d.inv[e] = (sum > 0.0f) ? 1.0f / sum : 0.0f;
 }
 
@@ -947,8 +951,10 @@ size_t backprojection_budget(int cu_waves) {
 namespace {
 
 // Sorted (event, band) pairs of band height H for the resident events, and
-// the index of the first pair of every (band, x0 >> shift) group.
-int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
+// the index of the first pair of every (band, x0 >> shift) group; aligned:
+// the bands start at each event's bbox top (forward pair lists).
+int build_pairs(PairList& L, int H, bool aligned, int shift,
+                const DeviceGeometry& g,
                 const BandRecord* rec, const uint16_t* spans, int stride,
                 int64_t n, hipStream_t stream, std::vector<int>& first,
                 std::string* err) {
@@ -965,7 +971,7 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
   int last_count = 0, last_offset = 0, max_count = 0;
   if (n > 0) {
     hipLaunchKernelGGL(count_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
-                       rec, n, H, counts);
+                       rec, n, H, aligned ? 1 : 0, counts);
     BAND_TRY(hipGetLastError());
     // the pair offsets below are 32-bit: refuse before they could wrap
     unsigned long long* d_total = nullptr;
@@ -1028,7 +1034,7 @@ int build_pairs(PairList& L, int H, int shift, const DeviceGeometry& g,
     BAND_TRY(hipMalloc(&L.bounds, sizeof(int) * n * L.max_bands));
     hipLaunchKernelGGL(band_bounds_kernel, dim3((unsigned)((n + 3) / 4)),
                        dim3(256), 0, stream, rec, spans, stride, n, H,
-                       L.max_bands, L.bounds);
+                       aligned ? 1 : 0, L.max_bands, L.bounds);
     BAND_TRY(hipGetLastError());
     hipLaunchKernelGGL(emit_pairs_kernel, dim3(blocks), dim3(256), 0, stream,
                        rec, L.bounds, L.max_bands, n, H, shift, counts,
@@ -1281,36 +1287,29 @@ int band_prepare_once(BandPlan& plan, const DeviceGeometry& g,
   // to pick band heights); its actual window is set by its pairs' runs
   plan.Wt = bucket + max_width;
   // band heights as asked, halved (down to 16 rows) where a task window of
-  // the widest bbox would not fit in LDS
+  // the widest bbox would not fit in LDS (the forward window holds 2 Hf
+  // rows: its pairs' bands start anywhere in the task's image band)
   int Hf = tu.fwd_rows == 16 ? 16 : (tu.fwd_rows == 32 ? 32 : 64);
   int Hb = tu.bwd_rows == 64 ? 64 : (tu.bwd_rows == 16 ? 16 : 32);
-  while (Hf > 16 && (size_t)plan.Wt * Hf * sizeof(float) > kLdsLimit) Hf /= 2;
+  while (Hf > 16 && (size_t)plan.Wt * 2 * Hf * sizeof(float) > kLdsLimit)
+    Hf /= 2;
   while (Hb > 16 && (size_t)plan.Wt * Hb * sizeof(float) > kLdsLimit) Hb /= 2;
-  if ((size_t)plan.Wt * Hf * sizeof(float) > kLdsLimit) {
+  if ((size_t)plan.Wt * 2 * Hf * sizeof(float) > kLdsLimit) {
     if (err) *err = "task window does not fit in LDS";
     return -1;
   }
   const uint16_t* sp = plan.spans;
   const int ss = plan.span_stride;
   std::vector<int> first;
-  if (build_pairs(plan.fwd, Hf, shift, g, plan.records, sp, ss, n, stream,
-                  first, err))
+  if (build_pairs(plan.fwd, Hf, true, shift, g, plan.records, sp, ss, n,
+                  stream, first, err))
     return -1;
   if (build_tasks(plan.fwd, shift, first, tu.fwd_pairs, plan.records,
                   plan.Wt, stream, err))
     return -1;
-  if (Hb == Hf) {
-    plan.bwd.H = Hf;
-    plan.bwd.n_bands = plan.fwd.n_bands;
-    plan.bwd.max_bands = plan.fwd.max_bands;
-    plan.bwd.n_pairs = plan.fwd.n_pairs;
-    plan.bwd.pair_event = plan.fwd.pair_event;
-    plan.bwd.bounds = plan.fwd.bounds;
-    plan.bwd.owns_pairs = false;
-  } else if (build_pairs(plan.bwd, Hb, shift, g, plan.records, sp, ss, n,
-                         stream, first, err)) {
+  if (build_pairs(plan.bwd, Hb, false, shift, g, plan.records, sp, ss, n,
+                  stream, first, err))
     return -1;
-  }
   if (build_tasks(plan.bwd, shift, first, tu.bwd_pairs, plan.records,
                   plan.Wt, stream, err))
     return -1;
@@ -1324,7 +1323,7 @@ int band_prepare_once(BandPlan& plan, const DeviceGeometry& g,
   plan.bwd_group = std::max(1, std::min(tu.bwd_group, 64 / Hb));
   if (plan.fwd_group == 3) plan.fwd_group = 2;
   if (plan.bwd_group == 3) plan.bwd_group = 2;
-  plan.fwd_lds = (size_t)plan.fwd.Wt * Hf * sizeof(float);
+  plan.fwd_lds = (size_t)plan.fwd.Wt * 2 * Hf * sizeof(float);
   // back-projection: every wavefront owns a fixed LDS budget and runs each
   // task with the most pairs per step (up to bwd_group, halving) whose
   // windows fit in it.  Automatic groups: the budget keeps bwd_cu_waves

@@ -147,6 +147,23 @@ __device__ inline RowSpan row_span(const DeviceGeometry& g, float cz,
   return s;
 }
 
+// The bands of an event's bbox rows [y0, y1) (none for an empty box).  A
+// forward pair list aligns its bands to the event: band k holds rows
+// y0 + k H .. y0 + k H + H - 1, so only the last band is partial.  The
+// back-projection's bands are the image's, rows b H .. b H + H - 1.  Either
+// way band k belongs to the tasks of image band y0 / H + k, the band of its
+// first row.
+__device__ inline int band_count(int y0, int y1, bool has_columns, int H,
+                                 bool aligned) {
+  if (!has_columns || y1 <= y0) return 0;
+  return aligned ? (y1 - y0 + H - 1) / H : (y1 - 1) / H - y0 / H + 1;
+}
+
+// First row of the event's band in image band b (H a power of two).
+__device__ inline int band_first_row(int y0, int b, int H, bool aligned) {
+  return b * H + (aligned ? (y0 & (H - 1)) : 0);
+}
+
 __device__ inline void record_box(const BandRecord& rec, int& x0, int& x1,
                                   int& y0, int& y1) {
   x0 = rec.bx & 0xffff;

@@ -1874,7 +1874,8 @@ int batch_merge(strip_mlem_batch* b) {
     }
     b->generations.push_back(P.generation);
   }
-  b->fwd_lds = sizeof(float) * (size_t)b->fwd_Wt * b->H_fwd;
+  // forward windows hold 2 H rows (bands aligned to their events)
+  b->fwd_lds = sizeof(float) * (size_t)b->fwd_Wt * 2 * b->H_fwd;
   const size_t window = sizeof(float) * (size_t)b->bwd_wave_words;
   if (b->fwd_lds > 160 * 1024 || window > 160 * 1024)
     return strip::fail(STRIP_MLEM_ERR_UNSUPPORTED,
