@@ -11,6 +11,9 @@ rel-L2 <= 1e-4 (the GPU sums each event's denominator over a wavefront and
 adds its updates with float atomics, in another order than the reference's
 sequential loop).
 """
+import ctypes
+import os
+
 import numpy as np
 import pytest
 
@@ -265,3 +268,70 @@ def test_frame_events_from_the_reader(oracle_mod, pystrip_mod, hybrid_mod,
     with pytest.raises(pystrip_mod.StripMLEMError):
         rec.set_frame_events(bad)
     rec.close()
+
+
+DROPIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(
+    __file__))), "tests", "dropin", "_build", "libhybrid_dropin.so")
+
+
+@pytest.mark.skipif(not os.path.exists(DROPIN),
+                    reason="drop-in run harness not built (it is compiled "
+                           "from the reference's headers at build time)")
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_dropin_run(oracle_mod, hybrid_mod, ring, semantics):
+    # the reference's own preparation and event reader, then the C++ drop-in
+    # run() (include/hybrid_gpu_reconstruction.hpp) as the command line's
+    # --gpu branch calls it: 2 blocks of 3 iterations, so the run outputs
+    # iterations 1, 2 (early outputs of block 0), 3 and 6 (block ends)
+    print_case("hybrid drop-in run", "semantics", semantics)
+    ev = hybrid_mod.hybrid_events(ring, 20000, seed=21)
+    lib = ctypes.CDLL(DROPIN)
+    P = ctypes.c_void_p
+    f = ctypes.c_float
+    i64 = ctypes.c_int64
+    fn = lib.dropin_hybrid_run
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_int, P, P, i64, P, P, P, P, P, ctypes.c_int,
+                   ctypes.c_int, f, ctypes.c_int, f, f, f, f, P, i64,
+                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                   ctypes.c_int, P, P, P, P, P, ctypes.c_int, P,
+                   ctypes.c_int]
+    seg, widths, pixels, positions, weights, begin, end = \
+        oracle_mod._geometry_args(ring)
+    evf = np.ascontiguousarray(ev, np.float32)
+    rho = np.ones(ring.shape, np.float32)
+    nv = rho.size
+    max_out = 8
+    iters = np.zeros(max_out, np.int32)
+    outs = np.zeros((max_out, nv), np.float32)
+    n_out = ctypes.c_int()
+    n_frames = ctypes.c_int64()
+    name = ctypes.create_string_buffer(256)
+    err = ctypes.create_string_buffer(512)
+    rc = fn(ring.n_detectors, seg.ctypes.data, widths.ctypes.data,
+            ring.n_infos, pixels.ctypes.data, positions.ctypes.data,
+            weights.ctypes.data, begin.ctypes.data, end.ctypes.data,
+            ring.n_columns, ring.n_rows, ring.pixel_size, ring.n_planes,
+            ring.z_left, ring.sigma_z, ring.sigma_dl, ring.length,
+            evf.ctypes.data, evf.shape[0], 2, 3, 0, semantics,
+            rho.ctypes.data, max_out, iters.ctypes.data, outs.ctypes.data,
+            ctypes.byref(n_out), ctypes.byref(n_frames), name, 256, err, 512)
+    assert rc == 0, err.value.decode()
+    assert b"gfx950" in name.value
+    assert list(iters[:n_out.value]) == [1, 2, 3, 6]
+    legacy = semantics == 1
+    ref6, n_used, _, _ = oracle_mod.hybrid_mlem(ring, ev, iterations=6,
+                                                legacy=legacy)
+    ref3 = oracle_mod.hybrid_mlem(ring, ev, iterations=3, legacy=legacy)[0]
+    assert n_frames.value == n_used
+    e6 = oracle_mod.rel_l2(rho, ref6)
+    e3 = oracle_mod.rel_l2(outs[2].reshape(rho.shape), ref3)
+    assert np.array_equal(outs[3].reshape(rho.shape), rho)
+    msg = f"rel-L2 vs oracle: iteration 3 {e3:.3e}, iteration 6 {e6:.3e}"
+    if oracle_mod.ref_available() and not legacy:
+        r6 = oracle_mod.ref_hybrid_mlem(ring, ev, iterations=6)[0]
+        er = oracle_mod.rel_l2(rho, r6)
+        msg += f", vs reference build {er:.3e}"
+        assert er <= 1e-4
+    print(msg, flush=True)
+    assert e3 <= 1e-4 and e6 <= 1e-4
