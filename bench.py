@@ -37,8 +37,8 @@ FLOPS_PER_KERNEL_EVAL = 16
 # by the counter's reading for a plain 256 MiB copy kernel (x2.0 and x1.0);
 # recorded in profiles/band_kernels_strip-1000x400-1mm-2e7_counters.md
 MEASURED_TRAFFIC_BYTES = {
-    "back-projection": (41.50 + 0.90) * 1e9,
-    "forward projection": (20.51 + 2.66) * 1e9,
+    "back-projection": (41.51 + 0.90) * 1e9,
+    "forward projection": (22.49 + 2.24) * 1e9,
 }
 TRAFFIC_WORKLOAD = "strip-1000x400-1mm-2e7"
 
