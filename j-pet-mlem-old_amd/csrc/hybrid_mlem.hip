@@ -564,38 +564,6 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// The (info, plane) walk of an event's box, plane-fastest: lane `lane`
-// starts at pair `lane` and steps 64 pairs at a time.
-struct BoxWalk {
-  int q, r, sq, sr, n_pl;
-  __device__ BoxWalk(int lane, int n_pl_) : n_pl(n_pl_) {
-    if (n_pl > 0) {
-      // quotients of 0..64 by n_pl from a float reciprocal (exact for these
-      // small integers after the one-step correction)
-      const float rcp = __builtin_amdgcn_rcpf((float)n_pl);
-      sq = (int)(64.0f * rcp);
-      sq += (sq + 1) * n_pl <= 64 ? 1 : 0;
-      sq -= sq * n_pl > 64 ? 1 : 0;
-      q = (int)((float)lane * rcp);
-      q += (q + 1) * n_pl <= lane ? 1 : 0;
-      q -= q * n_pl > lane ? 1 : 0;
-    } else {
-      sq = 0;
-      q = 0;
-    }
-    sr = 64 - sq * n_pl;
-    r = lane - q * n_pl;
-  }
-  __device__ void step() {
-    q += sq;
-    r += sr;
-    if (r >= n_pl) {
-      r -= n_pl;
-      ++q;
-    }
-  }
-};
-
 // The walk of an event's rows x planes with each lane's plane fixed: for a
 // chunk of w <= 64 planes from plane chunk0 of the box, lane l takes plane
 // chunk0 + l mod w and rows l / w, l / w + rpi, ... (rpi = 64 / w rows per
@@ -605,15 +573,13 @@ struct PlaneWalk {
   bool active;
   __device__ PlaneWalk(int lane, int n_pl, int chunk0) {
     const int w = n_pl - chunk0 < 64 ? n_pl - chunk0 : 64;
-    // quotients by w from a float reciprocal, exact after one correction
+    // l / w for 0 <= l <= 64 as the integer part of (l + 1/2) / w, which
+    // lies at least 1 / (2 w) >= 1 / 128 from an integer: far more than the
+    // error of the hardware reciprocal and of the product
     const float rcp = __builtin_amdgcn_rcpf((float)w);
-    rpi = (int)(64.0f * rcp);
-    rpi += (rpi + 1) * w <= 64 ? 1 : 0;
-    rpi -= rpi * w > 64 ? 1 : 0;
-    q0 = (int)((float)lane * rcp);
-    q0 += (q0 + 1) * w <= lane ? 1 : 0;
-    q0 -= q0 * w > lane ? 1 : 0;
-    r = chunk0 + lane - q0 * w;
+    rpi = (int)(64.5f * rcp);
+    q0 = (int)(((float)lane + 0.5f) * rcp);
+    r = chunk0 + lane - __mul24(q0, w);
     active = q0 < rpi;
   }
 };
