@@ -685,6 +685,7 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     float* __restrict__ extra) {
   extern __shared__ __align__(16) float lds[];
   __shared__ int s_next;
+  __shared__ unsigned long long s_zw[2];  // box pairs, pairs x (pb + pe)
   const int np = g.n_planes;
   const int n_waves = blockDim.x >> 6;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -832,23 +833,31 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     // waves of this block read them back from global memory, which the
     // barrier's workgroup-scope release and acquire order (one CU, one L1)
     __syncthreads();
-    // back-projection over contiguous row ranges, one per wave, of about
-    // equal (row, plane) pair counts: the event boxes' row weights (n_pl per
-    // covered row) as differences, then their running sum C(r), in the
-    // (now free) factor area of the waves
+    // back-projection over rectangles of the slab, one per wave: the rows in
+    // n_waves / 2 contiguous ranges of about equal (row, plane) pair counts
+    // of the event boxes, each range split in two at the boxes' mean plane
+    // (weighted by their pairs).  The rows' weights (n_pl per covered row)
+    // go in as differences, then their running sum C(r), where the infos'
+    // up was (no longer read)
     int* s_dw = (int*)s_up;
     float* s_C = (float*)s_dw;
     for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) slab[t] = 0.0f;
     for (int t = threadIdx.x; t <= n_lor; t += blockDim.x) s_dw[t] = 0;
+    if (threadIdx.x == 0) s_zw[0] = s_zw[1] = 0;
     __syncthreads();
     for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
       const int4 ei = ev_i[e];
-      const int n_pl = (ei.w >> 16) - (ei.w & 0xffff);
+      const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+      const int n_pl = pe - pb;
       const int ie_l = ei.z < lr.y ? ei.z : lr.y;
       const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
       if (n_pl > 0 && a1 > a0 && inv[e] > 0.0f) {
         atomicAdd(&s_dw[a0], n_pl);
         atomicAdd(&s_dw[a1], -n_pl);
+        const unsigned long long pairs =
+            (unsigned long long)(a1 - a0) * (unsigned long long)n_pl;
+        atomicAdd(&s_zw[0], pairs);
+        atomicAdd(&s_zw[1], pairs * (unsigned long long)(pb + pe));
       }
     }
     __syncthreads();
@@ -865,25 +874,37 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
       }
     }
     __syncthreads();
-    // row r goes to wave floor(n_waves (C(r - 1) + C(r)) / 2 / C(n_lor - 1));
-    // wave w takes rows [#{owner < w}, #{owner <= w}), a partition of the
-    // rows into contiguous ranges whatever the rounding
+    // row r goes to range floor(n_ranges (C(r - 1) + C(r)) / 2 /
+    // C(n_lor - 1)); range k takes rows [#{owner < k}, #{owner <= k}), a
+    // partition of the rows into contiguous ranges whatever the rounding.
+    // Wave w takes range w / 2 and the planes below (w even) or from (w odd)
+    // the split plane.
+    const int n_ranges = n_waves >> 1;
+    const int range = wave >> 1;
     int r_start = 0, r_end = 0;
     {
       const float W = n_lor > 0 ? s_C[n_lor - 1] : 0.0f;
-      const float scale = W > 0.0f ? (float)n_waves / W : 0.0f;
+      const float scale = W > 0.0f ? (float)n_ranges / W : 0.0f;
       for (int c0 = 0; c0 < n_lor; c0 += 64) {
         const int r = c0 + lane;
-        int o = n_waves;
+        int o = n_ranges;
         if (r < n_lor) {
           const float mid = 0.5f * ((r > 0 ? s_C[r - 1] : 0.0f) + s_C[r]);
           o = (int)(mid * scale);
-          o = o < n_waves - 1 ? o : n_waves - 1;
+          o = o < n_ranges - 1 ? o : n_ranges - 1;
         }
-        r_start += __popcll(__ballot(r < n_lor && o < wave));
-        r_end += __popcll(__ballot(r < n_lor && o <= wave));
+        r_start += __popcll(__ballot(r < n_lor && o < range));
+        r_end += __popcll(__ballot(r < n_lor && o <= range));
       }
     }
+    int z_split = np >> 1;
+    if (s_zw[0] > 0) {
+      // the pair-weighted mean of the boxes' plane midpoints (pb + pe) / 2
+      z_split = (int)((double)s_zw[1] / (2.0 * (double)s_zw[0]) + 0.5);
+      z_split = z_split < 1 ? 1 : (z_split > np - 1 ? np - 1 : z_split);
+    }
+    const int z_lo = (wave & 1) ? z_split : 0;
+    const int z_hi = (wave & 1) ? np : z_split;
     for (int base = e0; base < e1; base += 64) {
       const int e = base + lane;
       int4 ei = make_int4(0, 0, 0, 0);
@@ -895,12 +916,13 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
         right = ev_f[e].y;
         off = fac_off[e];
       }
-      const int pb = ei.w & 0xffff, pe = ei.w >> 16;
       const int ie_l = ei.z < lr.y ? ei.z : lr.y;
       const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
-      // this wave's rows of the event's box
+      // this wave's rows and planes of the event's box
       const int lo = a0 > r_start ? a0 : r_start;
       const int hi = a1 < r_end ? a1 : r_end;
+      const int pb = (ei.w & 0xffff) > z_lo ? (ei.w & 0xffff) : z_lo;
+      const int pe = (ei.w >> 16) < z_hi ? (ei.w >> 16) : z_hi;
       const bool mine = e < e1 && iv > 0.0f && pe > pb && hi > lo;
       uint64_t mask = __ballot(mine);
       // the wave's units (event j, up to 64 rows from r0) in order; the
