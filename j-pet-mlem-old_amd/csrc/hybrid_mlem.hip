@@ -660,11 +660,12 @@ __device__ inline float wave_scan_float(float v) {
 //   dy tan are written to global memory, and the extra info's update to the
 //   image-space extra sums (twice, for the visit it repeats, under the
 //   kernel-cache semantics, which also leaves that visit's F at 0).
-//   Backward: the slab is cleared and its rows split into one contiguous
-//   range per wave with about equal (row, plane) pair counts over the
-//   events' boxes; every wave reads the events 64 at a time, picks by
-//   ballot those whose box meets its range, and adds F exp(-iz b^2) over
-//   its rows of each box, up to 64 rows per unit, loading the next unit's
+//   Backward: the slab is cleared and split into one rectangle per wave
+//   (n_waves / 2 contiguous row ranges with about equal (row, plane) pair
+//   counts over the events' boxes, each cut in two at the boxes' mean
+//   plane); every wave reads the events 64 at a time, picks by ballot
+//   those whose box meets its rectangle, and adds F exp(-iz b^2) over its
+//   part of each box, up to 64 rows per unit, loading the next unit's
 //   factors while it sums the current one.  Each slab word has one writer,
 //   so the sums are plain LDS read-modify-writes (LDS float atomics run at
 //   about one lane per clock), in event order per word.
