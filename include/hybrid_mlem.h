@@ -184,9 +184,10 @@ int hybrid_mlem_iterate(hybrid_mlem_ctx* ctx, int n_iterations);
  *      again for the back-projection's LOR-space sums), one workgroup per
  *      LOR taking its events, their updates summed in LDS over the LOR's
  *      infos x planes, then gathered per voxel over the pixel's infos;
- *   0: automatic (default): 2 where its tables fit in device memory, the
- *      largest LOR's infos x planes fit in LDS with room for two
- *      wavefronts, and no event spans more than 256 infos, else 1.
+ *   0: automatic (default): 2 where its tables fit in device memory, a
+ *      plane of the largest LOR's infos fits in LDS (a LOR whose infos x
+ *      planes do not is summed in chunks of planes) and no event spans
+ *      more than 256 infos, else 1.
  * hybrid_mlem_get_algorithm returns the algorithm of the last iteration. */
 int hybrid_mlem_set_algorithm(hybrid_mlem_ctx* ctx, int algorithm);
 int hybrid_mlem_get_algorithm(hybrid_mlem_ctx* ctx);

@@ -648,10 +648,67 @@ __device__ inline float wave_scan_float(float v) {
   return v;
 }
 
+// The end of an event's forward step: D_e = the lanes' partial sums plus
+// the extra info's terms (the info one past the LOR's range, with the full
+// kernel), then 1 / D_e, the back-projection's factors F = A / D_e (0 for
+// the visit the extra info repeats under the kernel-cache semantics) and
+// dy tan, and the extra info's update added to the image-space sums (twice
+// for that repeated visit).  wAT holds the event's (A, dy tan) per info.
+template <bool LAST_WRITE>
+__device__ inline void finish_event(
+    const Grid& g, const Kernel& k, const float4& sa, float R,
+    const float4& ef, const int4& ei, int e, int dup, int lor_end, int pb,
+    int pe, int n_info, int np, int lane, float den,
+    const int* __restrict__ pix, const float* __restrict__ w,
+    const float* __restrict__ rho, const float* __restrict__ sens,
+    const int64_t* __restrict__ fac_off, const float2* wAT,
+    float* __restrict__ inv, float2* __restrict__ fac,
+    float* __restrict__ extra) {
+  int xx = 0, yy = 0, xpixel = -1;
+  if (ei.z > lor_end && pe > pb) {
+    const int pxy = pix[ei.z - 1];
+    xx = pixel_x(pxy);
+    yy = pixel_y(pxy);
+    if (in_grid(g, xx, yy)) xpixel = yy * g.n_columns + xx;
+  }
+  if (xpixel >= 0) {
+    for (int z = pb + lane; z < pe; z += 64) {
+      const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
+      if (!(kv <= 0.0f))
+        den += kv * w[ei.z - 1] * rho[xpixel * np + z] * sens[xpixel];
+    }
+  }
+  den = strip::wave_sum(den);
+  const float iv = den > 0.0f ? 1.0f / den : 0.0f;
+  if (lane == 0) inv[e] = iv;
+  if (iv > 0.0f) {
+    float2* f = fac + fac_off[e];
+    for (int t = lane; t < n_info; t += 64) {
+      const float2 at = wAT[t];
+      f[t] = make_float2(
+          (LAST_WRITE && ei.y + t == dup) ? 0.0f : at.x * iv, at.y);
+    }
+    if (xpixel >= 0) {
+      const float mult = (LAST_WRITE && dup >= 0) ? 2.0f : 1.0f;
+      for (int z = pb + lane; z < pe; z += 64) {
+        const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
+        if (!(kv <= 0.0f)) {
+          const int v = xpixel * np + z;
+          atomicAdd(&extra[v], mult * (kv * w[ei.z - 1] * rho[v] * iv));
+        }
+      }
+    }
+  }
+}
+
 // One LOR-space iteration's event work.  A persistent grid, one block per
 // CU, takes LORs from a queue (LORs with events, most events first).  For
 // each LOR the block's LDS slab (the LOR's infos x all planes, laid out as
-// its rows of bp) first holds P = G rho s of every (info, plane).
+// its rows of bp) first holds P = G rho s of every (info, plane); where the
+// LOR's infos x planes exceed slab_budget floats (with its infos' up), the
+// slab holds a chunk of the planes at a time and each step below runs chunk
+// by chunk, the forward pass summing each event's D_e over the chunks in
+// memory before a last pass forms 1 / D_e and the factors.
 //   Forward: the waves take the LOR's events in turn (parameters read 64 at
 //   a time, one per lane); per event the per-info factors (A, dy tan) go
 //   to the wave's LDS area, D_e = sum A_ei exp(-iz b^2) P over the box
@@ -683,7 +740,7 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     const int64_t* __restrict__ fac_off, const int* __restrict__ lor_events,
     const int* __restrict__ lor_order, int n_active, int* __restrict__ queue,
     float* __restrict__ inv, float2* __restrict__ fac, float* __restrict__ bp,
-    float* __restrict__ extra) {
+    float* __restrict__ extra, int slab_budget) {
   extern __shared__ __align__(16) float lds[];
   __shared__ int s_next;
   __shared__ unsigned long long s_zw[2];  // box pairs, pairs x (pb + pe)
@@ -700,134 +757,171 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     const int e0 = lor_events[lor], e1 = lor_events[lor + 1];
     const int2 lr = ranges[lor];
     const int n_lor = lr.y - lr.x;
+    // planes per slab chunk: the slab holds the LOR's infos x pc planes (all
+    // of them where they fit), then the infos' up, then the waves' areas
+    int pc = n_lor > 0 ? (slab_budget - n_lor - 2) / n_lor : np;
+    pc = pc < np ? pc : np;
+    const int n_chunks = (np + pc - 1) / pc;
     float* slab = lds;
-    float* s_up = slab + (size_t)n_lor * np;
+    float* s_up = slab + (size_t)n_lor * pc;
     // the waves' areas of kWaveWords floats (8-byte aligned): an event's
     // factors (A, dy tan) per info
-    float* areas = lds + (((size_t)n_lor * np + n_lor + 2) & ~(size_t)1);
+    float* areas = lds + (((size_t)n_lor * pc + n_lor + 2) & ~(size_t)1);
     float* warea = areas + (size_t)wave * kWaveWords;
     float2* wAT = (float2*)warea;
     const float4 sa = seg_a[lor];
     const float R = seg_b[lor].w;
-    // the rows' pixel and sensitivity, staged in the (still unused) areas
+    for (int t = threadIdx.x; t < n_lor; t += blockDim.x)
+      s_up[t] = upi[lr.x + t];
+    // the rows' pixel and sensitivity, staged for each chunk in the areas
+    // (the forward pass then takes them over)
     int* s_pix = (int*)areas;
     float* s_sens = (float*)(s_pix + n_lor);
-    for (int t = threadIdx.x; t < n_lor; t += blockDim.x) {
-      s_up[t] = upi[lr.x + t];
-      const int p = ipix[lr.x + t];
-      s_pix[t] = p;
-      s_sens[t] = p >= 0 ? sens[p] : 0.0f;
-    }
-    __syncthreads();
-    // P = G rho s, eight cells per thread in flight
-    {
-      const int n_cells = n_lor * np;
-      const float* Grows = G + lr.x * np;
-      const float inv_np = 1.0f / (float)np;
-      for (int c0 = threadIdx.x; c0 < n_cells; c0 += 8 * blockDim.x) {
-        float gv[8], rv[8], sv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int c = c0 + u * blockDim.x;
-          gv[u] = rv[u] = sv[u] = 0.0f;
-          if (c < n_cells) {
-            int row = (int)((float)c * inv_np);
-            if (row * np > c) --row;
-            if ((row + 1) * np <= c) ++row;
-            const int p = s_pix[row];
-            if (p >= 0) {
-              gv[u] = Grows[c];
-              rv[u] = rho[p * np + (c - row * np)];
-              sv[u] = s_sens[row];
-            }
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const int c = c0 + u * blockDim.x;
-          if (c < n_cells) slab[c] = gv[u] * rv[u] * sv[u];
-        }
+    for (int ch = 0; ch < n_chunks; ++ch) {
+      const int z0 = ch * pc;
+      const int cw = pc < np - z0 ? pc : np - z0;  // the chunk's planes
+      __syncthreads();  // the last chunk's P and areas are no longer read
+      for (int t = threadIdx.x; t < n_lor; t += blockDim.x) {
+        const int p = ipix[lr.x + t];
+        s_pix[t] = p;
+        s_sens[t] = p >= 0 ? sens[p] : 0.0f;
       }
-    }
-    __syncthreads();
-
-    // forward: wave w takes events e0 + w, e0 + w + n_waves, ...
-    for (int base = e0 + wave; base < e1; base += 64 * n_waves) {
-      EventBatch bt;
-      const int mine = base + lane * n_waves;
-      bt.load(ev_i, ev_f, nullptr, ev_dup, mine, mine < e1);
-      const int cnt = min(64, (e1 - base + n_waves - 1) / n_waves);
-      for (int j = 0; j < cnt; ++j) {
-        const int e = base + j * n_waves;
-        const int4 ei = make_int4(lane_int(bt.i.x, j), lane_int(bt.i.y, j),
-                                  lane_int(bt.i.z, j), lane_int(bt.i.w, j));
-        const float4 ef = make_float4(lane_float(bt.f.x, j),
-                                      lane_float(bt.f.y, j),
-                                      lane_float(bt.f.z, j),
-                                      lane_float(bt.f.w, j));
-        const int dup = LAST_WRITE ? lane_int(bt.dup, j) : -1;
-        const int pb = ei.w & 0xffff, pe = ei.w >> 16;
-        const int n_pl = pe - pb;
-        const int ie_l = ei.z < lr.y ? ei.z : lr.y;
-        const int a0 = ei.y - lr.x;
-        const int n_info = ie_l - ei.y;
-        for (int t = lane; t < n_info; t += 64) {
-          float A, T;
-          info_factors(k, ef, R, s_up[a0 + t], A, T);
-          wAT[t] = make_float2(A, T);
-        }
-        wave_lds_sync();
-        float den = 0.0f;
-        for (int c0 = 0; c0 < n_pl; c0 += 64) {
-          const PlaneWalk pw(lane, n_pl, c0);
-          if (!pw.active) continue;
-          const float c = plane_centre(g, pb + pw.r) - ef.y;
-          const float* prow = slab + __mul24(a0 + pw.q0, np) + pb + pw.r;
-          const int stride = __mul24(pw.rpi, np);
-          for (int q = pw.q0; q < n_info; q += pw.rpi) {
-            const float2 at = wAT[q];
-            const float d = c - at.y;
-            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-            den += at.x * K * *prow;
-            prow += stride;
-          }
-        }
-        int xx = 0, yy = 0, xpixel = -1;
-        if (ei.z > lr.y && n_pl > 0) {
-          const int pxy = pix[ei.z - 1];
-          xx = pixel_x(pxy);
-          yy = pixel_y(pxy);
-          if (in_grid(g, xx, yy)) xpixel = yy * g.n_columns + xx;
-        }
-        if (xpixel >= 0) {
-          for (int z = pb + lane; z < pe; z += 64) {
-            const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
-            if (!(kv <= 0.0f))
-              den += kv * w[ei.z - 1] * rho[xpixel * np + z] * sens[xpixel];
-          }
-        }
-        den = strip::wave_sum(den);
-        const float iv = den > 0.0f ? 1.0f / den : 0.0f;
-        if (lane == 0) inv[e] = iv;
-        if (iv > 0.0f) {
-          float2* f = fac + fac_off[e];
-          for (int t = lane; t < n_info; t += 64) {
-            const float2 at = wAT[t];
-            f[t] = make_float2(
-                (LAST_WRITE && ei.y + t == dup) ? 0.0f : at.x * iv, at.y);
-          }
-          if (xpixel >= 0) {
-            const float mult = (LAST_WRITE && dup >= 0) ? 2.0f : 1.0f;
-            for (int z = pb + lane; z < pe; z += 64) {
-              const float kv = kernel2d(g, k, sa, R, ef, xx, yy, z);
-              if (!(kv <= 0.0f)) {
-                const int v = xpixel * np + z;
-                atomicAdd(&extra[v], mult * (kv * w[ei.z - 1] * rho[v] * iv));
+      __syncthreads();
+      // P = G rho s of the chunk's planes (row pitch cw), eight cells per
+      // thread in flight
+      {
+        const int n_cells = n_lor * cw;
+        const float* Grows = G + lr.x * np + z0;
+        const float inv_cw = 1.0f / (float)cw;
+        for (int c0 = threadIdx.x; c0 < n_cells; c0 += 8 * blockDim.x) {
+          float gv[8], rv[8], sv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int c = c0 + u * blockDim.x;
+            gv[u] = rv[u] = sv[u] = 0.0f;
+            if (c < n_cells) {
+              // the row from a float estimate (within one of it; the slab
+              // has fewer than 2^16 cells)
+              int row = (int)((float)c * inv_cw);
+              if (row * cw > c) --row;
+              if ((row + 1) * cw <= c) ++row;
+              const int p = s_pix[row];
+              if (p >= 0) {
+                const int zz = c - row * cw;
+                gv[u] = Grows[row * np + zz];
+                rv[u] = rho[p * np + z0 + zz];
+                sv[u] = s_sens[row];
               }
             }
           }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            const int c = c0 + u * blockDim.x;
+            if (c < n_cells) slab[c] = gv[u] * rv[u] * sv[u];
+          }
         }
-        wave_lds_sync();  // this event's factors are read before the next's
+      }
+      __syncthreads();
+
+      // forward: wave w takes events e0 + w, e0 + w + n_waves, ... (the
+      // same events in every chunk)
+      for (int base = e0 + wave; base < e1; base += 64 * n_waves) {
+        EventBatch bt;
+        const int mine = base + lane * n_waves;
+        bt.load(ev_i, ev_f, nullptr, ev_dup, mine, mine < e1);
+        const int cnt = min(64, (e1 - base + n_waves - 1) / n_waves);
+        for (int j = 0; j < cnt; ++j) {
+          const int e = base + j * n_waves;
+          const int4 ei = make_int4(lane_int(bt.i.x, j), lane_int(bt.i.y, j),
+                                    lane_int(bt.i.z, j), lane_int(bt.i.w, j));
+          const float4 ef = make_float4(lane_float(bt.f.x, j),
+                                        lane_float(bt.f.y, j),
+                                        lane_float(bt.f.z, j),
+                                        lane_float(bt.f.w, j));
+          const int dup = LAST_WRITE ? lane_int(bt.dup, j) : -1;
+          const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+          const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+          const int a0 = ei.y - lr.x;
+          const int n_info = ie_l - ei.y;
+          // the box's planes in this chunk
+          const int qb = pb > z0 ? pb : z0;
+          const int qe = pe < z0 + cw ? pe : z0 + cw;
+          if (n_chunks > 1 && qe <= qb) {
+            // D_e is summed over the chunks in inv[e], by this wave's lane 0
+            if (ch == 0 && lane == 0) inv[e] = 0.0f;
+            continue;
+          }
+          for (int t = lane; t < n_info; t += 64) {
+            float A, T;
+            info_factors(k, ef, R, s_up[a0 + t], A, T);
+            wAT[t] = make_float2(A, T);
+          }
+          wave_lds_sync();
+          float den = 0.0f;
+          const int n_q = qe - qb;
+          for (int c0 = 0; c0 < n_q; c0 += 64) {
+            const PlaneWalk pw(lane, n_q, c0);
+            if (!pw.active) continue;
+            const int z = qb + pw.r;
+            const float c = plane_centre(g, z) - ef.y;
+            const float* prow = slab + __mul24(a0 + pw.q0, cw) + (z - z0);
+            const int stride = __mul24(pw.rpi, cw);
+            for (int q = pw.q0; q < n_info; q += pw.rpi) {
+              const float2 at = wAT[q];
+              const float d = c - at.y;
+              const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+              den += at.x * K * *prow;
+              prow += stride;
+            }
+          }
+          if (n_chunks > 1) {
+            den = strip::wave_sum(den);
+            if (lane == 0) inv[e] = (ch == 0 ? 0.0f : inv[e]) + den;
+          } else {
+            finish_event<LAST_WRITE>(g, k, sa, R, ef, ei, e, dup, lr.y, pb, pe,
+                                     n_info, np, lane, den, pix, w, rho,
+                                     sens, fac_off, wAT, inv, fac, extra);
+          }
+          wave_lds_sync();  // this event's factors are read before the next's
+        }
+      }
+    }
+    if (n_chunks > 1) {
+      // D_e summed over the chunks: 1 / D_e, the factors and the extra
+      // info's update, each event by the wave that summed it
+      for (int base = e0 + wave; base < e1; base += 64 * n_waves) {
+        EventBatch bt;
+        const int mine = base + lane * n_waves;
+        bt.load(ev_i, ev_f, nullptr, ev_dup, mine, mine < e1);
+        const int cnt = min(64, (e1 - base + n_waves - 1) / n_waves);
+        for (int j = 0; j < cnt; ++j) {
+          const int e = base + j * n_waves;
+          const int4 ei = make_int4(lane_int(bt.i.x, j), lane_int(bt.i.y, j),
+                                    lane_int(bt.i.z, j), lane_int(bt.i.w, j));
+          const float4 ef = make_float4(lane_float(bt.f.x, j),
+                                        lane_float(bt.f.y, j),
+                                        lane_float(bt.f.z, j),
+                                        lane_float(bt.f.w, j));
+          const int dup = LAST_WRITE ? lane_int(bt.dup, j) : -1;
+          const int pb = ei.w & 0xffff, pe = ei.w >> 16;
+          const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+          const int a0 = ei.y - lr.x;
+          const int n_info = ie_l - ei.y;
+          for (int t = lane; t < n_info; t += 64) {
+            float A, T;
+            info_factors(k, ef, R, s_up[a0 + t], A, T);
+            wAT[t] = make_float2(A, T);
+          }
+          wave_lds_sync();
+          // the chunks' sum enters through lane 0, the extra info's terms
+          // through every lane
+          const float acc = inv[e];
+          finish_event<LAST_WRITE>(g, k, sa, R, ef, ei, e, dup, lr.y, pb, pe,
+                                   n_info, np, lane, lane == 0 ? acc : 0.0f,
+                                   pix, w, rho, sens, fac_off, wAT, inv, fac,
+                                   extra);
+          wave_lds_sync();
+        }
       }
     }
     // every 1 / D_e and factor is written and P is no longer read; other
@@ -837,12 +931,12 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
     // back-projection over rectangles of the slab, one per wave: the rows in
     // n_waves / 2 contiguous ranges of about equal (row, plane) pair counts
     // of the event boxes, each range split in two at the boxes' mean plane
-    // (weighted by their pairs).  The rows' weights (n_pl per covered row)
-    // go in as differences, then their running sum C(r), where the infos'
-    // up was (no longer read)
+    // (weighted by their pairs; the chunk's middle plane where the slab
+    // holds a chunk of the planes).  The rows' weights (n_pl per covered
+    // row) go in as differences, then their running sum C(r), where the
+    // infos' up was (no longer read)
     int* s_dw = (int*)s_up;
     float* s_C = (float*)s_dw;
-    for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x) slab[t] = 0.0f;
     for (int t = threadIdx.x; t <= n_lor; t += blockDim.x) s_dw[t] = 0;
     if (threadIdx.x == 0) s_zw[0] = s_zw[1] = 0;
     __syncthreads();
@@ -904,81 +998,102 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
       z_split = (int)((double)s_zw[1] / (2.0 * (double)s_zw[0]) + 0.5);
       z_split = z_split < 1 ? 1 : (z_split > np - 1 ? np - 1 : z_split);
     }
-    const int z_lo = (wave & 1) ? z_split : 0;
-    const int z_hi = (wave & 1) ? np : z_split;
-    for (int base = e0; base < e1; base += 64) {
-      const int e = base + lane;
-      int4 ei = make_int4(0, 0, 0, 0);
-      float iv = 0.0f, right = 0.0f;
-      int64_t off = 0;
-      if (e < e1) {
-        ei = ev_i[e];
-        iv = inv[e];
-        right = ev_f[e].y;
-        off = fac_off[e];
-      }
-      const int ie_l = ei.z < lr.y ? ei.z : lr.y;
-      const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
-      // this wave's rows and planes of the event's box
-      const int lo = a0 > r_start ? a0 : r_start;
-      const int hi = a1 < r_end ? a1 : r_end;
-      const int pb = (ei.w & 0xffff) > z_lo ? (ei.w & 0xffff) : z_lo;
-      const int pe = (ei.w >> 16) < z_hi ? (ei.w >> 16) : z_hi;
-      const bool mine = e < e1 && iv > 0.0f && pe > pb && hi > lo;
-      uint64_t mask = __ballot(mine);
-      // the wave's units (event j, up to 64 rows from r0) in order; the
-      // factors of the next unit are loaded while the current one is summed
-      int nj = -1, nr0 = 0;
-      float2 nf = make_float2(0.f, 0.f);
-      if (mask) {
-        nj = __builtin_ctzll(mask);
-        mask &= mask - 1;
-        nr0 = lane_int(lo, nj);
-      }
-      if (nj >= 0) nf = unit_factors(fac, a0, hi, off, nj, nr0, lane);
-      while (nj >= 0) {
-        const int j = nj, r0 = nr0;
-        const float2 f = nf;
-        const int jhi = lane_int(hi, j);
-        const int jpb = lane_int(pb, j), jpe = lane_int(pe, j);
-        const float jright = lane_float(right, j);
-        if (r0 + 64 < jhi) {
-          nr0 = r0 + 64;
-        } else if (mask) {
+    for (int ch = 0; ch < n_chunks; ++ch) {
+      const int z0 = ch * pc;
+      const int cw = pc < np - z0 ? pc : np - z0;  // the chunk's planes
+      const int zs = n_chunks > 1 ? z0 + (cw >> 1) : z_split;
+      const int z_lo = (wave & 1) ? zs : z0;
+      const int z_hi = (wave & 1) ? z0 + cw : zs;
+      for (int t = threadIdx.x; t < n_lor * cw; t += blockDim.x)
+        slab[t] = 0.0f;
+      __syncthreads();
+      for (int base = e0; base < e1; base += 64) {
+        const int e = base + lane;
+        int4 ei = make_int4(0, 0, 0, 0);
+        float iv = 0.0f, right = 0.0f;
+        int64_t off = 0;
+        if (e < e1) {
+          ei = ev_i[e];
+          iv = inv[e];
+          right = ev_f[e].y;
+          off = fac_off[e];
+        }
+        const int ie_l = ei.z < lr.y ? ei.z : lr.y;
+        const int a0 = ei.y - lr.x, a1 = ie_l - lr.x;
+        // this wave's rows and planes of the event's box
+        const int lo = a0 > r_start ? a0 : r_start;
+        const int hi = a1 < r_end ? a1 : r_end;
+        const int pb = (ei.w & 0xffff) > z_lo ? (ei.w & 0xffff) : z_lo;
+        const int pe = (ei.w >> 16) < z_hi ? (ei.w >> 16) : z_hi;
+        const bool mine = e < e1 && iv > 0.0f && pe > pb && hi > lo;
+        uint64_t mask = __ballot(mine);
+        // the wave's units (event j, up to 64 rows from r0) in order; the
+        // factors of the next unit are loaded while the current one is
+        // summed
+        int nj = -1, nr0 = 0;
+        float2 nf = make_float2(0.f, 0.f);
+        if (mask) {
           nj = __builtin_ctzll(mask);
           mask &= mask - 1;
           nr0 = lane_int(lo, nj);
-        } else {
-          nj = -1;
         }
         if (nj >= 0) nf = unit_factors(fac, a0, hi, off, nj, nr0, lane);
-        const int rows = (r0 + 64 < jhi ? r0 + 64 : jhi) - r0;
-        const int n_pl = jpe - jpb;
-        // the unit's row factors in the wave's area
-        if (lane < rows) wAT[lane] = f;
-        wave_lds_sync();
-        for (int c0 = 0; c0 < n_pl; c0 += 64) {
-          const PlaneWalk pw(lane, n_pl, c0);
-          if (!pw.active) continue;
-          const float c = plane_centre(g, jpb + pw.r) - jright;
-          float* srow = slab + __mul24(r0 + pw.q0, np) + jpb + pw.r;
-          const int stride = __mul24(pw.rpi, np);
-          for (int q = pw.q0; q < rows; q += pw.rpi) {
-            const float2 ft = wAT[q];
-            const float d = c - ft.y;
-            const float K = __builtin_amdgcn_exp2f(kexp * d * d);
-            *srow += ft.x * K;
-            srow += stride;
+        while (nj >= 0) {
+          const int j = nj, r0 = nr0;
+          const float2 f = nf;
+          const int jhi = lane_int(hi, j);
+          const int jpb = lane_int(pb, j), jpe = lane_int(pe, j);
+          const float jright = lane_float(right, j);
+          if (r0 + 64 < jhi) {
+            nr0 = r0 + 64;
+          } else if (mask) {
+            nj = __builtin_ctzll(mask);
+            mask &= mask - 1;
+            nr0 = lane_int(lo, nj);
+          } else {
+            nj = -1;
           }
+          if (nj >= 0) nf = unit_factors(fac, a0, hi, off, nj, nr0, lane);
+          const int rows = (r0 + 64 < jhi ? r0 + 64 : jhi) - r0;
+          const int n_pl = jpe - jpb;
+          // the unit's row factors in the wave's area
+          if (lane < rows) wAT[lane] = f;
+          wave_lds_sync();
+          for (int c0 = 0; c0 < n_pl; c0 += 64) {
+            const PlaneWalk pw(lane, n_pl, c0);
+            if (!pw.active) continue;
+            const int z = jpb + pw.r;
+            const float c = plane_centre(g, z) - jright;
+            float* srow = slab + __mul24(r0 + pw.q0, cw) + (z - z0);
+            const int stride = __mul24(pw.rpi, cw);
+            for (int q = pw.q0; q < rows; q += pw.rpi) {
+              const float2 ft = wAT[q];
+              const float d = c - ft.y;
+              const float K = __builtin_amdgcn_exp2f(kexp * d * d);
+              *srow += ft.x * K;
+              srow += stride;
+            }
+          }
+          wave_lds_sync();  // the area is read before the next unit's
         }
-        wave_lds_sync();  // the area is read before the next unit's
       }
+      __syncthreads();
+      // the chunk's columns of the LOR's rows of bp
+      float* rows_out = bp + (int64_t)lr.x * np + z0;
+      if (cw == np) {
+        for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x)
+          rows_out[t] = slab[t];
+      } else {
+        const float inv_cw = 1.0f / (float)cw;
+        for (int t = threadIdx.x; t < n_lor * cw; t += blockDim.x) {
+          int row = (int)((float)t * inv_cw);
+          if (row * cw > t) --row;
+          if ((row + 1) * cw <= t) ++row;
+          rows_out[(int64_t)row * np + (t - row * cw)] = slab[t];
+        }
+      }
+      __syncthreads();  // the slab is read before the next chunk or LOR
     }
-    __syncthreads();
-    float* rows_out = bp + (int64_t)lr.x * np;
-    for (int t = threadIdx.x; t < n_lor * np; t += blockDim.x)
-      rows_out[t] = slab[t];
-    __syncthreads();  // the slab is read before the next LOR fills it
   }
 }
 
@@ -1180,6 +1295,7 @@ struct hybrid_mlem_ctx {
   bool tables = false;
   int lor_waves = 0;  // waves per block of the LOR-space event kernel
   size_t lor_lds = 0;
+  int lor_slab_budget = 0;  // LDS floats for a LOR's slab and infos' up
   // algorithm: 0 automatic, 1 wavefront per event with image atomics,
   // 2 LOR space
   int algorithm_request = 0;
@@ -1348,25 +1464,36 @@ int build_pixel_index(hybrid_mlem_ctx* c) {
 // (rows of LORs without events stay zero), the extra accumulator, and the
 // iteration kernel's block: the largest LOR's slab (infos x planes) and
 // its infos' up with a 2 KiB area per wave (an event's factors), within
-// 160 KiB of LDS, 16 waves (or 8, 4, 2 where LDS is short); the tables'
-// cell indices must fit in 31 bits.
-// Where that does not fit, or the tables do not fit in device memory, the
-// context stays on the wavefront-per-event algorithm.
+// 160 KiB of LDS, 16 waves (or 8); else all of LDS with 16 waves, the slabs
+// then holding chunks of the planes (at least one plane of the largest
+// LOR).  The tables' cell indices must fit in 31 bits.  Where that does
+// not fit, or the tables do not fit in device memory, the context stays on
+// the wavefront-per-event algorithm.
 int prepare_tables(hybrid_mlem_ctx* c) {
   const Grid& g = c->grid;
   const size_t slab =
       sizeof(float) * ((size_t)c->max_lor_infos * ((size_t)g.n_planes + 1) + 2);
   const size_t per_wave = sizeof(float) * kWaveWords;
+  // every plane of the largest LOR with 16 (or 8) waves where that fits,
+  // else all of LDS with 16 waves and the slab in chunks of planes
   int waves = 16;
-  while (waves > 2 && slab + waves * per_wave > kLdsBytes) waves /= 2;
-  // the factor area also stages the rows' pixel and sensitivity
-  if (slab + waves * per_wave > kLdsBytes ||
+  if (slab + waves * per_wave > kLdsBytes) waves = 8;
+  if (slab + waves * per_wave <= kLdsBytes) {
+    c->lor_lds = slab + waves * per_wave;
+  } else {
+    waves = 16;
+    c->lor_lds = kLdsBytes;
+  }
+  const size_t budget = c->lor_lds / sizeof(float) - (size_t)waves * kWaveWords;
+  // at least one plane of the largest LOR; the factor area also stages the
+  // rows' pixel and sensitivity
+  if (budget < 2 * (size_t)c->max_lor_infos + 2 ||
       2 * sizeof(float) * (size_t)c->max_lor_infos > waves * per_wave) {
     c->tables = false;
     return STRIP_MLEM_OK;
   }
   c->lor_waves = waves;
-  c->lor_lds = slab + waves * per_wave;
+  c->lor_slab_budget = (int)budget;
   const int64_t cells = (int64_t)c->n_infos * g.n_planes;
   if (!c->G) {
     const bool ok =
@@ -2082,14 +2209,16 @@ int iterate_lor_space(hybrid_mlem_ctx* c, hipEvent_t mid) {
                          c->ranges, c->pix, c->ipix, c->upi, c->w, c->sens,
                          c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup,
                          c->fac_off, c->lor_events, c->lor_order, c->n_active,
-                         c->queue, c->inv, c->fac, c->bp, c->extra);
+                         c->queue, c->inv, c->fac, c->bp, c->extra,
+                         c->lor_slab_budget);
     else
       hipLaunchKernelGGL(lor_iteration_kernel<false>, grid, block,
                          c->lor_lds, c->stream, g, c->kernel, kexp, c->seg_a,
                          c->seg_b, c->ranges, c->pix, c->ipix, c->upi, c->w,
                          c->sens, c->G, c->rho, c->pl_i, c->pl_f, c->pl_dup,
                          c->fac_off, c->lor_events, c->lor_order, c->n_active,
-                         c->queue, c->inv, c->fac, c->bp, c->extra);
+                         c->queue, c->inv, c->fac, c->bp, c->extra,
+                         c->lor_slab_budget);
   }
   if (mid) HY_TRY(hipEventRecord(mid, c->stream));
   const unsigned gblocks =

@@ -270,26 +270,29 @@ def test_frame_events_from_the_reader(oracle_mod, pystrip_mod, hybrid_mod,
     rec.close()
 
 
-def test_automatic_choice_where_the_slab_exceeds_lds(oracle_mod, hybrid_mod,
-                                                   pystrip_mod):
-    # 200 planes: the ring's largest LOR (327 infos) needs 327 x 201 floats
-    # of LDS, more than a workgroup has, so the automatic choice runs the
-    # wavefront-per-event algorithm and an explicit LOR-space request fails
+@pytest.mark.parametrize("semantics", [0, 1])
+def test_lor_space_in_plane_chunks(oracle_mod, hybrid_mod, semantics):
+    # 200 planes: the ring's largest LOR (327 infos) needs 327 x 201 floats,
+    # more than a workgroup's LDS, so the LOR-space iteration holds the
+    # slabs in chunks of planes, summing each event's D_e over the chunks;
+    # against the oracle and the wavefront-per-event algorithm
     geo = hybrid_mod.ring_geometry(n_planes=200)
-    print_case("hybrid automatic choice", "200 planes")
+    print_case("hybrid plane chunks", "200 planes", "semantics", semantics)
     ev = hybrid_mod.hybrid_events(geo, 20000, seed=31)
-    rec = _rec(hybrid_mod, geo)
-    rec.add_responses(ev)
-    rec.iterate(2)
-    assert rec.algorithm == 1
-    ref = oracle_mod.hybrid_mlem(geo, ev, iterations=2)[0]
-    e = oracle_mod.rel_l2(rec.rho, ref)
-    print("rel-L2", e, flush=True)
-    assert e <= 1e-4
-    rec.set_algorithm(2)
-    with pytest.raises(pystrip_mod.StripMLEMError):
-        rec.iterate(1)
-    rec.close()
+    legacy = semantics == 1
+    ref = oracle_mod.hybrid_mlem(geo, ev, iterations=3, legacy=legacy)[0]
+    images = {}
+    for algorithm in (0, 1):
+        rec = _rec(hybrid_mod, geo, semantics, algorithm=algorithm)
+        rec.add_responses(ev)
+        rec.iterate(3)
+        assert rec.algorithm == (2 if algorithm == 0 else 1)
+        images[rec.algorithm] = rec.rho
+        e = oracle_mod.rel_l2(rec.rho, ref)
+        print("algorithm", rec.algorithm, "rel-L2", e, flush=True)
+        assert e <= 1e-4
+        rec.close()
+    assert oracle_mod.rel_l2(images[1], images[2]) <= 1e-4
 
 
 DROPIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(
