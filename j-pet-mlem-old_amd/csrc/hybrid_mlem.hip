@@ -540,19 +540,23 @@ __global__ __launch_bounds__(256) void frame_table_kernel(
     const int* __restrict__ info_lor, const int* __restrict__ ipix,
     const float* __restrict__ upi, const float* __restrict__ w, int n_infos,
     float* __restrict__ G) {
-  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)n_infos * g.n_planes) return;
-  const int i = (int)(idx / g.n_planes);
-  const int z = (int)(idx - (int64_t)i * g.n_planes);
-  const int l = info_lor[i];
-  float v = 0.0f;
-  if (l >= 0 && ipix[i] >= 0) {
-    const float S =
-        frame_sensitivity(plane_centre(g, z), upi[i], seg_b[l].w, k.length);
-    // kernel2d = un / S is skipped where <= 0 (S negative or -0)
-    v = signbit(S) ? 0.0f : w[i] / S;
+  // a grid-stride loop: the table can hold more cells than one launch has
+  // work-items (a dispatch counts them in 32 bits)
+  const int64_t n_cells = (int64_t)n_infos * g.n_planes;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+       idx < n_cells; idx += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(idx / g.n_planes);
+    const int z = (int)(idx - (int64_t)i * g.n_planes);
+    const int l = info_lor[i];
+    float v = 0.0f;
+    if (l >= 0 && ipix[i] >= 0) {
+      const float S =
+          frame_sensitivity(plane_centre(g, z), upi[i], seg_b[l].w, k.length);
+      // kernel2d = un / S is skipped where <= 0 (S negative or -0)
+      v = signbit(S) ? 0.0f : w[i] / S;
+    }
+    G[idx] = v;
   }
-  G[idx] = v;
 }
 
 // Orders this wavefront's LDS accesses before and after (a wavefront's
@@ -791,7 +795,7 @@ __global__ __launch_bounds__(1024) void lor_iteration_kernel(
       // thread in flight
       {
         const int n_cells = n_lor * cw;
-        const float* Grows = G + lr.x * np + z0;
+        const float* Grows = G + (int64_t)lr.x * np + z0;
         const float inv_cw = 1.0f / (float)cw;
         for (int c0 = threadIdx.x; c0 < n_cells; c0 += 8 * blockDim.x) {
           float gv[8], rv[8], sv[8];
@@ -1466,9 +1470,8 @@ int build_pixel_index(hybrid_mlem_ctx* c) {
 // its infos' up with a 2 KiB area per wave (an event's factors), within
 // 160 KiB of LDS, 16 waves (or 8); else all of LDS with 16 waves, the slabs
 // then holding chunks of the planes (at least one plane of the largest
-// LOR).  The tables' cell indices must fit in 31 bits.  Where that does
-// not fit, or the tables do not fit in device memory, the context stays on
-// the wavefront-per-event algorithm.
+// LOR).  Where that does not fit, or the tables do not fit in device
+// memory, the context stays on the wavefront-per-event algorithm.
 int prepare_tables(hybrid_mlem_ctx* c) {
   const Grid& g = c->grid;
   const size_t slab =
@@ -1497,7 +1500,6 @@ int prepare_tables(hybrid_mlem_ctx* c) {
   const int64_t cells = (int64_t)c->n_infos * g.n_planes;
   if (!c->G) {
     const bool ok =
-        cells < ((int64_t)1 << 31) &&
         hipFuncSetAttribute((const void*)lor_iteration_kernel<true>,
                             hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)kLdsBytes) == hipSuccess &&
@@ -1519,7 +1521,9 @@ int prepare_tables(hybrid_mlem_ctx* c) {
     HY_TRY(hipMemsetAsync(c->bp, 0, sizeof(float) * cells, c->stream));
   }
   if (cells)
-    hipLaunchKernelGGL(frame_table_kernel, dim3(blocks_for(cells)),
+    hipLaunchKernelGGL(frame_table_kernel,
+                       dim3((unsigned)std::min<int64_t>(blocks_for(cells),
+                                                        (int64_t)1 << 20)),
                        dim3(256), 0, c->stream, g, c->kernel, c->seg_b,
                        c->info_lor, c->ipix, c->upi, c->w, c->n_infos, c->G);
   HY_TRY(hipGetLastError());

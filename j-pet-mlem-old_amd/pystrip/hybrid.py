@@ -211,6 +211,12 @@ WORKLOADS = {w.name: w for w in [
     HybridWorkload("hybrid-ring384-160x160x100-5mm-1e7", 384, 0.4278, 0.007,
                    160, 0.005, 100, 0.4, 0.015, 0.06, 0.5, 0.25, 0.2,
                    10_000_000),
+    # the same ring and phantom on 320 x 320 x 200 voxels of 2.5 mm: 3.7e7
+    # pixel infos, LORs of up to 960 infos, whose infos x planes exceed LDS
+    # (the LOR-space iteration then works in chunks of planes)
+    HybridWorkload("hybrid-ring384-320x320x200-2.5mm-1e7", 384, 0.4278,
+                   0.007, 320, 0.0025, 200, 0.4, 0.015, 0.06, 0.5, 0.25, 0.2,
+                   10_000_000),
 ]}
 
 

@@ -295,6 +295,32 @@ def test_lor_space_in_plane_chunks(oracle_mod, hybrid_mod, semantics):
     assert oracle_mod.rel_l2(images[1], images[2]) <= 1e-4
 
 
+def test_tables_beyond_2_32_cells(oracle_mod, hybrid_mod):
+    # the full ring with 480 planes of 5 mm: 9.2e6 infos x 480 planes = 4.4e9
+    # cells per LOR-space table, more than one launch has work-items (a
+    # dispatch counts them in 32 bits); LOR space against the
+    # wavefront-per-event algorithm on 2e5 events
+    w = hybrid_mod.WORKLOADS["hybrid-ring384-160x160x100-5mm-1e7"]
+    print_case("hybrid tables beyond 2^32 cells", "480 planes")
+    geo = hybrid_mod.ring_geometry(w.n_detectors, w.ring_radius, w.lor_width,
+                                   w.n_pixels, w.pixel_size, 480,
+                                   w.fov_radius, w.sigma_z, w.sigma_dl,
+                                   w.length)
+    assert geo.n_infos * 480 > 2 ** 32
+    ev = w.events(geo, 200_000, seed=17)
+    images = {}
+    for algorithm in (1, 2):
+        rec = _rec(hybrid_mod, geo, algorithm=algorithm)
+        rec.add_responses(ev)
+        rec.iterate(2)
+        assert rec.algorithm == algorithm
+        images[algorithm] = rec.rho
+        rec.close()
+    e = oracle_mod.rel_l2(images[2], images[1])
+    print("rel-L2 LOR space vs wavefront per event", e, flush=True)
+    assert e <= 1e-4
+
+
 DROPIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(
     __file__))), "tests", "dropin", "_build", "libhybrid_dropin.so")
 
