@@ -47,18 +47,19 @@ __global__ __launch_bounds__(256) void count_pairs_kernel(
 // bounds[e * max_bands + k] = lo | hi << 16: the columns [lo, hi) reached
 // by the runs of band k of record e (the span table's runs, or the bbox
 // columns without one; an empty slice gives [x0, x0 + 1)).  One wavefront
-// per record, lane = row offset from the start of its first band, so a
-// band's rows are H consecutive lanes (H divides 64).
+// per record (records in grid strides: a launch counts its work-items in 32
+// bits), lane = row offset from the start of its first band, so a band's
+// rows are H consecutive lanes (H divides 64).
 __global__ __launch_bounds__(256) void band_bounds_kernel(
     const BandRecord* __restrict__ rec, const uint16_t* __restrict__ spans,
     int stride, int64_t n, int H, int aligned, int max_bands,
     int* __restrict__ bounds) {
-  const int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (e >= n) return;  // uniform over the wavefront
   const int lane = threadIdx.x & 63;
+  for (int64_t e = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); e < n;
+       e += (int64_t)gridDim.x * 4) {
   int x0, x1, y0, y1;
   record_box(rec[e], x0, x1, y0, y1);
-  if (x1 <= x0 || y1 <= y0) return;
+  if (x1 <= x0 || y1 <= y0) continue;  // uniform over the wavefront
   const int row0 = band_first_row(y0, y0 / H, H, aligned != 0);
   const int rows = band_count(y0, y1, true, H, aligned != 0) * H;
   for (int base = 0; base < rows; base += 64) {
@@ -88,6 +89,7 @@ __global__ __launch_bounds__(256) void band_bounds_kernel(
       }
       bounds[e * max_bands + i / H] = lo | (hi << 16);
     }
+  }
   }
 }
 
@@ -183,9 +185,10 @@ __global__ __launch_bounds__(256) void unpermute_kernel(
 __global__ __launch_bounds__(256) void span_table_kernel(
     DeviceGeometry g, const BandRecord* __restrict__ rec, int64_t n,
     int stride, uint16_t* __restrict__ spans) {
-  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
   const int lane = threadIdx.x & 63;
+  // records in grid strides (a launch counts its work-items in 32 bits)
+  for (int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < n;
+       i += (int64_t)gridDim.x * 4) {
   const BandRecord r = rec[i];
   int x0, x1, y0, y1;
   record_box(r, x0, x1, y0, y1);
@@ -200,6 +203,7 @@ __global__ __launch_bounds__(256) void span_table_kernel(
       if (s.n > 0) word = (uint16_t)((s.lo - x0) | (s.n << 8));
     }
     out[k] = word;
+  }
   }
 }
 
@@ -1032,7 +1036,8 @@ int build_pairs(PairList& L, int H, bool aligned, int shift,
     BAND_TRY(hipMalloc(&L.pair_event, sizeof(int) * n_pairs));
     BAND_TRY(hipMalloc(&vals_alt, sizeof(int) * n_pairs));
     BAND_TRY(hipMalloc(&L.bounds, sizeof(int) * n * L.max_bands));
-    hipLaunchKernelGGL(band_bounds_kernel, dim3((unsigned)((n + 3) / 4)),
+    hipLaunchKernelGGL(band_bounds_kernel,
+                       dim3((unsigned)std::min<int64_t>((n + 3) / 4, 1 << 20)),
                        dim3(256), 0, stream, rec, spans, stride, n, H,
                        aligned ? 1 : 0, L.max_bands, L.bounds);
     BAND_TRY(hipGetLastError());
@@ -1271,7 +1276,8 @@ int band_prepare_once(BandPlan& plan, const DeviceGeometry& g,
     if (bytes <= free_bytes / 2 && bytes + rest <= free_bytes) {
       *built_spans = true;
       BAND_TRY(hipMalloc(&plan.spans, bytes));
-      const unsigned span_blocks = (unsigned)((n + 3) / 4);
+      const unsigned span_blocks =
+          (unsigned)std::min<int64_t>((n + 3) / 4, 1 << 20);
       hipLaunchKernelGGL(span_table_kernel, dim3(span_blocks), dim3(256), 0,
                          stream, g, plan.records, n, plan.span_stride,
                          plan.spans);
