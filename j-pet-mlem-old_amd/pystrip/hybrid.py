@@ -170,6 +170,143 @@ def ring_geometry(n_detectors=48, ring_radius=0.4, lor_width=0.02,
                           begin, end)
 
 
+GEOMETRY_MAGIC = b"PETg"
+_GRID_HEADER = np.dtype([("n_columns", "<i2"), ("n_rows", "<i2"),
+                         ("pixel_size", "<f4"), ("llx", "<f4"),
+                         ("lly", "<f4")])
+_LOR_HEADER = np.dtype([("first", "<i2"), ("second", "<i2"),
+                        ("seg", "<f4", (4,)), ("width", "<f4"),
+                        ("n_pixels", "<u8")])
+_PIXEL_INFO = np.dtype([("x", "<i2"), ("y", "<i2"), ("t", "<f4"),
+                        ("distance", "<f4"), ("fill", "<f4"),
+                        ("weight", "<f4")])
+
+
+def read_geometry(path, n_planes, z_left=None, sigma_z=0.015, sigma_dl=0.06,
+                  length=0.5) -> HybridGeometry:
+    """A PET2D::Barrel::Geometry<float, short> binary file (the reference's
+    2d_barrel_geometry output) flattened as GeometrySOA(geometry, 1) does.
+
+    File (src/2d/barrel/geometry.h:56-105, native little-endian): the magic
+    "PETg", int16 n_detectors, the grid (pixel_grid.h:82-101: int16
+    n_columns, n_rows, float pixel_size, lower-left x, y), then LOR records
+    to the end (lor_geometry.h:58-84: int16 first, second, the segment's
+    start and end points (line_segment.h:38-56), float width, uint64
+    n_pixels, n_pixels x {int16 x, y, float t, distance, fill, weight}); a
+    later record for the same LOR (index first (first + 1) / 2 + second,
+    lor.h:36-38) replaces the earlier one.  The flattening
+    (geometry_soa.h:187-206) walks the LOR slots in index order and lays the
+    infos out slot after slot; a slot no record filled is a zero LOR
+    (first = second = 0), so it empties LOR 0, and keeps info range [0, 0)
+    (its segment and width, which the reference leaves unset, are 0 here).
+    The padding info of the module docstring (centre pixel, weight 0,
+    position 2) follows the file's infos, so that the info one past the last
+    LOR's stays inside the arrays.  The voxel grid's axis comes
+    from the caller as the reference's command line does (--n-planes,
+    --z-left; z_left None centres it).  The reconstruction's grid is centred
+    on the scanner axis (PixelGrid(n_columns, n_rows, pixel_size)), so a file
+    whose lower-left corner is not (-n_columns, -n_rows) pixel_size / 2 is
+    refused.  ring_radius (used only to draw events) is the mean distance of
+    the segments' end points from the axis."""
+    data = np.fromfile(path, np.uint8)
+    if data.size < 4 + 2 + _GRID_HEADER.itemsize or \
+            data[:4].tobytes() != GEOMETRY_MAGIC:
+        raise ValueError(f"{path}: not a binary float barrel geometry file")
+    n_det = int(np.frombuffer(data, "<i2", 1, 4)[0])
+    grid = np.frombuffer(data, _GRID_HEADER, 1, 6)[0]
+    n_columns, n_rows = int(grid["n_columns"]), int(grid["n_rows"])
+    ps = np.float32(grid["pixel_size"])
+    ll = (np.float32(-ps * np.float32(n_columns) / np.float32(2)),
+          np.float32(-ps * np.float32(n_rows) / np.float32(2)))
+    if n_det <= 0 or n_columns <= 0 or n_rows <= 0 or not ps > 0:
+        raise ValueError(f"{path}: bad header")
+    for got, want in ((grid["llx"], ll[0]), (grid["lly"], ll[1])):
+        if abs(float(got) - float(want)) > 1e-6 * abs(float(want)):
+            raise ValueError(f"{path}: pixel grid not centred on the axis "
+                             f"(lower-left {grid['llx']}, {grid['lly']})")
+    n_lors = (n_det + 1) * n_det // 2
+    slot = [None] * n_lors
+    off = 6 + _GRID_HEADER.itemsize
+    while off + _LOR_HEADER.itemsize <= data.size:
+        h = np.frombuffer(data, _LOR_HEADER, 1, off)[0]
+        off += _LOR_HEADER.itemsize
+        m = int(h["n_pixels"])
+        if m * _PIXEL_INFO.itemsize > data.size - off:
+            break  # a short record ends the stream, as a failed read does
+        infos = np.frombuffer(data, _PIXEL_INFO, m, off)
+        off += m * _PIXEL_INFO.itemsize
+        f, s = int(h["first"]), int(h["second"])
+        li = f * (f + 1) // 2 + s
+        if not (0 <= s <= f < n_det):
+            raise ValueError(f"{path}: LOR ({f}, {s}) outside the "
+                             f"{n_det}-detector scanner")
+        slot[li] = (h["seg"], h["width"], infos)
+    seg = np.zeros((n_lors, 4), np.float32)
+    widths = np.zeros(n_lors, np.float32)
+    begin = np.zeros(n_lors, np.int64)
+    end = np.zeros(n_lors, np.int64)
+    parts, size = [], 0
+    for li, rec in enumerate(slot):
+        target = li if rec is not None else 0
+        sg, w, infos = rec if rec is not None else \
+            (np.zeros(4, np.float32), np.float32(0), None)
+        seg[target], widths[target] = sg, w
+        begin[target] = size
+        if infos is not None and len(infos):
+            parts.append(infos)
+            size += len(infos)
+        end[target] = size
+    pad = np.zeros(1, _PIXEL_INFO)
+    pad["x"], pad["y"], pad["t"] = n_columns // 2, n_rows // 2, 2.0
+    infos = np.concatenate(parts + [pad])
+    pixels = np.stack([infos["x"], infos["y"]], 1).astype(np.int16)
+    ends = np.concatenate([seg[:, :2], seg[:, 2:]])
+    r = np.hypot(ends[:, 0].astype(np.float64), ends[:, 1])
+    ring_radius = float(r[r > 0].mean()) if np.any(r > 0) else 0.0
+    z_left = -n_planes * float(ps) / 2 if z_left is None else z_left
+    return HybridGeometry(n_det, ring_radius, n_columns, n_rows, float(ps),
+                          n_planes, z_left, sigma_z, sigma_dl, length, seg,
+                          widths, pixels,
+                          np.ascontiguousarray(infos["t"], np.float32),
+                          np.ascontiguousarray(infos["weight"],
+                                               np.float32)[None, :],
+                          begin, end)
+
+
+def write_geometry(geo: HybridGeometry, path):
+    """geo written as a PET2D::Barrel::Geometry<float, short> binary file
+    (the layout read_geometry describes, geometry.h:91-103): every LOR slot
+    in index order with its own infos and first plane-half's weights;
+    infos outside every LOR's range (ring_geometry's padding) are not
+    written.  The file keeps no distance or fill (the reconstruction reads
+    neither): they are written as 0 and 1."""
+    n_det = geo.n_detectors
+    ps = np.float32(geo.pixel_size)
+    head = np.zeros(1, _GRID_HEADER)
+    head["n_columns"], head["n_rows"], head["pixel_size"] = \
+        geo.n_columns, geo.n_rows, ps
+    head["llx"] = -ps * np.float32(geo.n_columns) / np.float32(2)
+    head["lly"] = -ps * np.float32(geo.n_rows) / np.float32(2)
+    first = np.repeat(np.arange(n_det), np.arange(1, n_det + 1))
+    second = np.concatenate([np.arange(f + 1) for f in range(n_det)])
+    with open(path, "wb") as out:
+        out.write(GEOMETRY_MAGIC)
+        out.write(np.int16(n_det).tobytes())
+        out.write(head.tobytes())
+        for li in range(geo.n_lors):
+            b, e = int(geo.info_begin[li]), int(geo.info_end[li])
+            h = np.zeros(1, _LOR_HEADER)
+            h["first"], h["second"] = first[li], second[li]
+            h["seg"], h["width"], h["n_pixels"] = geo.seg[li], \
+                geo.widths[li], e - b
+            out.write(h.tobytes())
+            rec = np.zeros(e - b, _PIXEL_INFO)
+            rec["x"], rec["y"] = geo.pixels[b:e, 0], geo.pixels[b:e, 1]
+            rec["t"], rec["fill"] = geo.positions[b:e], 1.0
+            rec["weight"] = geo.weights[0, b:e]
+            out.write(rec.tobytes())
+
+
 @dataclass(frozen=True)
 class HybridWorkload:
     """A named hybrid reconstruction workload: ring geometry parameters,

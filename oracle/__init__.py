@@ -419,6 +419,51 @@ def ref_hybrid_mlem(geo, events, iterations=1, rho0=None, weights_mode=1,
     return rho, used.value, sens, wout
 
 
+def ref_read_geometry(path):
+    """The reference's own reader of a binary barrel geometry file
+    (PET2D::Barrel::Geometry<float, short>(util::ibstream&) flattened by
+    GeometrySOA(geometry, 1), oracle/_ref).  Returns a dict of n_detectors,
+    n_columns, n_rows, grid (pixel_size, lower-left x, y) and the arrays
+    seg, widths, pixels, positions, weights, info_begin, info_end."""
+    L = ref()
+    fn = L.ref_read_geometry
+    fp, i16p, i64p, ip = (ctypes.POINTER(ctypes.c_float),
+                          ctypes.POINTER(ctypes.c_int16),
+                          ctypes.POINTER(ctypes.c_int64),
+                          ctypes.POINTER(ctypes.c_int))
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_char_p, ip, i64p, ip, ip, fp, fp, fp, i16p, fp,
+                   fp, i64p, i64p]
+    n_det, n_infos = ctypes.c_int(), ctypes.c_int64()
+    n_columns, n_rows = ctypes.c_int(), ctypes.c_int()
+    grid = np.zeros(4, np.float32)
+    name = os.fsencode(path)
+    if fn(name, ctypes.byref(n_det), ctypes.byref(n_infos),
+          ctypes.byref(n_columns), ctypes.byref(n_rows),
+          _p(grid, ctypes.c_float), None, None, None, None, None, None, None):
+        raise RuntimeError(f"ref_read_geometry could not read {path}")
+    n_lors = (n_det.value + 1) * n_det.value // 2
+    out = dict(seg=np.empty((n_lors, 4), np.float32),
+               widths=np.empty(n_lors, np.float32),
+               pixels=np.empty((n_infos.value, 2), np.int16),
+               positions=np.empty(n_infos.value, np.float32),
+               weights=np.empty(n_infos.value, np.float32),
+               info_begin=np.empty(n_lors, np.int64),
+               info_end=np.empty(n_lors, np.int64))
+    if fn(name, ctypes.byref(n_det), ctypes.byref(n_infos),
+          ctypes.byref(n_columns), ctypes.byref(n_rows),
+          _p(grid, ctypes.c_float), _p(out["seg"], ctypes.c_float),
+          _p(out["widths"], ctypes.c_float), _p(out["pixels"], ctypes.c_int16),
+          _p(out["positions"], ctypes.c_float),
+          _p(out["weights"], ctypes.c_float),
+          _p(out["info_begin"], ctypes.c_int64),
+          _p(out["info_end"], ctypes.c_int64)):
+        raise RuntimeError(f"ref_read_geometry could not read {path}")
+    out.update(n_detectors=n_det.value, n_columns=n_columns.value,
+               n_rows=n_rows.value, grid=grid[:3].copy())
+    return out
+
+
 class OracleHybrid(ctypes.Structure):
     _fields_ = [("n_detectors", ctypes.c_int), ("n_infos", ctypes.c_int64),
                 ("n_planes_half", ctypes.c_int),

@@ -14,6 +14,10 @@
 //   PET3D::Hybrid::Reconstruction<Scanner, GaussianKernel<float>>
 //                                                        3d/hybrid/reconstruction.h:32-549
 //   PET2D::Barrel::GeometrySOA (filled from caller arrays) 2d/barrel/geometry_soa.h:31-287
+//   PET2D::Barrel::Geometry read from its binary file and flattened by
+//   GeometrySOA(geometry, 1)                             2d/barrel/geometry.h:56-105,
+//                                                        lor_geometry.h:58-84,
+//                                                        geometry_soa.h:187-206
 //   PET2D::PixelMap text/binary streams, util::nrrd_writer,
 //   util::obstream / ibstream (image outputs and --rho inputs)
 //                                                        pixel_map.h:113-157, nrrd_writer.h, bstream.h
@@ -36,6 +40,8 @@
 #include "2d/barrel/generic_scanner.h"
 #include "3d/hybrid/scanner.h"
 #include "3d/hybrid/reconstruction.h"
+#include "2d/barrel/geometry.h"
+#include "2d/barrel/geometry_soa.h"
 
 #include "strip_oracle.h"  // oracle_geometry only (plain struct)
 
@@ -341,6 +347,53 @@ int ref_hybrid_mlem(int n_detectors, const float* seg, const float* widths,
     for (int64_t i = 0; i < n_infos * n_planes_half; ++i)
       weights_out[i] = geo.pixel_weights[i];
   return 0;
+}
+
+// A PET2D::Barrel::Geometry<float, short> binary file (the reference's
+// barrel geometry tool writes it) read by the reference's own constructor
+// and flattened by GeometrySOA(geometry, 1).  Call first with null arrays:
+// *n_detectors, *n_infos and the grid (n_columns, n_rows, pixel_size,
+// lower-left x, y) are set; then with arrays of those sizes (n_lors x 4
+// segments, n_lors widths, n_infos x 2 pixels, positions, weights, n_lors
+// begin / end).  Returns -1 where the file cannot be read.
+int ref_read_geometry(const char* path, int* n_detectors, int64_t* n_infos,
+                      int* n_columns, int* n_rows, float* grid4,
+                      float* seg, float* widths, int16_t* pixels,
+                      float* positions, float* weights, int64_t* begin,
+                      int64_t* end) {
+  try {
+    util::ibstream in(path);
+    if (!in) return -1;
+    PET2D::Barrel::Geometry<float, short> geometry(in);
+    PET2D::Barrel::GeometrySOA<float, short> soa(geometry, 1);
+    *n_detectors = soa.n_detectors;
+    *n_infos = (int64_t)soa.n_pixel_infos;
+    *n_columns = geometry.grid.n_columns;
+    *n_rows = geometry.grid.n_rows;
+    grid4[0] = geometry.grid.pixel_size;
+    grid4[1] = geometry.grid.lower_left.x;
+    grid4[2] = geometry.grid.lower_left.y;
+    grid4[3] = 0.0f;
+    if (!seg) return 0;
+    for (size_t l = 0; l < soa.n_lors; ++l) {
+      seg[4 * l] = soa.lor_line_segments[l].start.x;
+      seg[4 * l + 1] = soa.lor_line_segments[l].start.y;
+      seg[4 * l + 2] = soa.lor_line_segments[l].end.x;
+      seg[4 * l + 3] = soa.lor_line_segments[l].end.y;
+      widths[l] = soa.lor_widths[l];
+      begin[l] = (int64_t)soa.lor_pixel_info_begin[l];
+      end[l] = (int64_t)soa.lor_pixel_info_end[l];
+    }
+    for (size_t i = 0; i < soa.n_pixel_infos; ++i) {
+      pixels[2 * i] = soa.pixels[i].x;
+      pixels[2 * i + 1] = soa.pixels[i].y;
+      positions[i] = soa.pixel_positions[i];
+      weights[i] = soa.pixel_weights[i];
+    }
+    return 0;
+  } catch (...) {
+    return -1;
+  }
 }
 
 }  // extern "C"

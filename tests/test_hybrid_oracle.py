@@ -182,3 +182,124 @@ def test_legacy_update_equals_the_cached_one_without_repeats(
     a = oracle_mod.hybrid_mlem(ring, ev, iterations=2)[0]
     b = oracle_mod.hybrid_mlem(ring, ev, iterations=2, legacy=True)[0]
     assert oracle_mod.rel_l2(a, b) < 1e-6
+
+
+def _geometry_file(hybrid_mod, path, geo, order=None, extra=()):
+    """geo's LOR records in the given slot order (default: all, by index),
+    then the extra records (first, second, seg, width, infos)."""
+    n = geo.n_detectors
+    head = np.zeros(1, hybrid_mod._GRID_HEADER)
+    head["n_columns"], head["n_rows"] = geo.n_columns, geo.n_rows
+    head["pixel_size"] = np.float32(geo.pixel_size)
+    head["llx"] = -np.float32(geo.pixel_size) * geo.n_columns / 2
+    head["lly"] = -np.float32(geo.pixel_size) * geo.n_rows / 2
+    first = np.repeat(np.arange(n), np.arange(1, n + 1))
+    second = np.concatenate([np.arange(f + 1) for f in range(n)])
+    recs = []
+    for li in (range(geo.n_lors) if order is None else order):
+        b, e = geo.info_begin[li], geo.info_end[li]
+        infos = np.zeros(e - b, hybrid_mod._PIXEL_INFO)
+        infos["x"], infos["y"] = geo.pixels[b:e, 0], geo.pixels[b:e, 1]
+        infos["t"], infos["weight"] = geo.positions[b:e], geo.weights[0, b:e]
+        recs.append((first[li], second[li], geo.seg[li], geo.widths[li],
+                     infos))
+    with open(path, "wb") as out:
+        out.write(b"PETg" + np.int16(n).tobytes() + head.tobytes())
+        for f, s, sg, w, infos in list(recs) + list(extra):
+            h = np.zeros(1, hybrid_mod._LOR_HEADER)
+            h["first"], h["second"], h["seg"], h["width"] = f, s, sg, w
+            h["n_pixels"] = len(infos)
+            out.write(h.tobytes() + infos.tobytes())
+
+
+def test_geometry_file_round_trip(hybrid_mod, ring, tmp_path):
+    """write_geometry then read_geometry gives back the arrays (the padding
+    info, in no LOR, is not written, and the reader appends it), and
+    writing what was read gives the same bytes."""
+    geo, p = ring, tmp_path / "ring.geom"
+    hybrid_mod.write_geometry(geo, p)
+    back = hybrid_mod.read_geometry(p, geo.n_planes, geo.z_left)
+    n = geo.n_infos - 1
+    assert back.n_detectors == geo.n_detectors and back.n_infos == n + 1
+    assert (back.n_columns, back.n_rows, back.n_planes) == \
+        (geo.n_columns, geo.n_rows, geo.n_planes)
+    assert back.pixel_size == np.float32(geo.pixel_size)
+    assert abs(back.ring_radius - geo.ring_radius) < 1e-6
+    for name in ("seg", "widths", "info_begin", "info_end"):
+        assert np.array_equal(getattr(back, name), getattr(geo, name)), name
+    assert np.array_equal(back.pixels, geo.pixels)
+    assert np.array_equal(back.positions, geo.positions)
+    assert np.array_equal(back.weights, geo.weights)
+    # 4 + 2 + 16 bytes of header, 32 per LOR record, 20 per info
+    size = 22 + 32 * geo.n_lors + 20 * n
+    assert p.stat().st_size == size
+    q = tmp_path / "again.geom"
+    hybrid_mod.write_geometry(back, q)
+    assert p.read_bytes() == q.read_bytes()
+
+
+def test_geometry_file_refusals(hybrid_mod, ring, tmp_path):
+    p = tmp_path / "bad.geom"
+    hybrid_mod.write_geometry(ring, p)
+    raw = bytearray(p.read_bytes())
+    raw[3] = ord("G")  # the double-precision magic
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ValueError, match="not a binary float"):
+        hybrid_mod.read_geometry(p, 10)
+    raw[3] = ord("g")
+    raw[14:18] = np.float32(0.0).tobytes()  # lower-left x at 0: off-centre
+    p.write_bytes(bytes(raw))
+    with pytest.raises(ValueError, match="not centred"):
+        hybrid_mod.read_geometry(p, 10)
+
+
+def test_geometry_file_against_reference_reader(oracle_mod, hybrid_mod, ring,
+                                                tmp_path):
+    """read_geometry against the reference's own Geometry(ibstream) +
+    GeometrySOA(geometry, 1) (oracle/_ref) on a file whose records come in
+    shuffled order, with a later record replacing an earlier one for the
+    same LOR, a missing slot (which the flattening turns into an empty
+    LOR 0) and a last record cut short (which ends the stream)."""
+    if not oracle_mod.ref_available():
+        pytest.skip("oracle/_ref not built")
+    geo = ring
+    rng = np.random.default_rng(3)
+    order = rng.permutation(geo.n_lors)
+    order = order[order != 700]
+    li = int(order[5])
+    f = int((np.sqrt(8 * li + 1) - 1) // 2)
+    s = li - f * (f + 1) // 2
+    infos = np.zeros(3, hybrid_mod._PIXEL_INFO)
+    infos["x"], infos["y"] = [1, 2, 3], [4, 5, 6]
+    infos["t"], infos["weight"] = [0.1, 0.2, 0.3], [0.5, 0.25, 0.125]
+    seg = np.array([0.1, 0.2, 0.3, 0.4], np.float32)
+    short = (f, s, seg, np.float32(0.02), infos)
+    p = tmp_path / "shuffled.geom"
+    _geometry_file(hybrid_mod, p, geo, order,
+                   extra=[(f, s, seg, np.float32(0.03), infos), short])
+    raw = p.read_bytes()
+    p.write_bytes(raw[:-10])  # the last record loses part of its infos
+    mine = hybrid_mod.read_geometry(p, geo.n_planes)
+    ref = oracle_mod.ref_read_geometry(p)
+    assert ref["n_detectors"] == mine.n_detectors
+    assert (ref["n_columns"], ref["n_rows"]) == (mine.n_columns, mine.n_rows)
+    assert ref["grid"][0] == np.float32(mine.pixel_size)
+    n = len(ref["positions"])
+    assert mine.n_infos == n + 1 and mine.weights[0, n] == 0
+    for name in ("pixels", "positions"):
+        assert np.array_equal(ref[name], getattr(mine, name)[:n]), name
+    for name in ("info_begin", "info_end"):
+        assert np.array_equal(ref[name], getattr(mine, name)), name
+    # the reference leaves a missing slot's segment and width unset
+    kept = np.arange(geo.n_lors) != 700
+    for name in ("seg", "widths"):
+        assert np.array_equal(ref[name][kept], getattr(mine, name)[kept])
+    assert mine.info_begin[700] == mine.info_end[700] == 0
+    assert np.array_equal(ref["weights"], mine.weights[0, :n])
+    # the replaced LOR holds the later record, the missing slot emptied LOR 0
+    assert mine.widths[li] == np.float32(0.03)
+    assert mine.info_end[li] - mine.info_begin[li] == 3
+    assert mine.info_end[0] == mine.info_begin[0] and mine.widths[0] == 0
+    assert mine.n_infos == geo.n_infos - (geo.info_end[li] -
+                                             geo.info_begin[li]) - \
+        (geo.info_end[700] - geo.info_begin[700]) + 3
