@@ -134,6 +134,32 @@ def test_mlem_parity(oracle_mod, hybrid_mod, ring, weights_mode, sens_mode,
 
 
 @pytest.mark.parametrize("algorithm", [1, 2])
+def test_geometry_from_file(oracle_mod, hybrid_mod, ring, algorithm,
+                            tmp_path):
+    """The reconstruction of a geometry read from a file in the reference's
+    binary barrel geometry format (write_geometry, read_geometry) matches
+    the oracle on the arrays read, and the file holds the same geometry."""
+    print_case("hybrid geometry file", "algorithm", algorithm)
+    path = tmp_path / "ring.geom"
+    hybrid_mod.write_geometry(ring, path)
+    geo = hybrid_mod.read_geometry(path, ring.n_planes, ring.z_left,
+                                   ring.sigma_z, ring.sigma_dl, ring.length)
+    assert np.array_equal(geo.positions, ring.positions)
+    assert np.array_equal(geo.info_end, ring.info_end)
+    ev = hybrid_mod.hybrid_events(geo, 30000, seed=21)
+    rec = _rec(hybrid_mod, geo, algorithm=algorithm)
+    rec.add_responses(ev)
+    rec.iterate(2)
+    gpu = rec.rho
+    ref, n_used, _, _ = oracle_mod.hybrid_mlem(geo, ev, iterations=2)
+    assert rec.n_events == n_used
+    e = oracle_mod.rel_l2(gpu, ref)
+    print(f"rel-L2 vs oracle {e:.3e}", flush=True)
+    assert e <= 1e-4 and np.all(np.isfinite(gpu))
+    rec.close()
+
+
+@pytest.mark.parametrize("algorithm", [1, 2])
 @pytest.mark.parametrize("semantics", [0, 1])
 def test_repeated_visit(oracle_mod, hybrid_mod, ring, semantics, algorithm):
     from hybrid_cases import repeated_pixel_events, repeated_pixel_geometry
