@@ -134,6 +134,52 @@ def test_mlem_parity(oracle_mod, hybrid_mod, ring, weights_mode, sens_mode,
 
 
 @pytest.mark.parametrize("algorithm", [1, 2])
+def test_no_events_and_empty_lors(oracle_mod, hybrid_mod, ring, algorithm):
+    """Edge inputs: no events at all (the update gives a zero image, as the
+    reference's rho * 0 / sensitivity does), then responses on every LOR
+    with no pixel infos (first > second), which keep or drop the event by
+    the info after the range, the next LOR's first (the upper_bound + 1 of
+    the reference's event reader): event count and image against the
+    oracle and the reference build."""
+    print_case("hybrid edge events", "algorithm", algorithm)
+    rec = _rec(hybrid_mod, ring, algorithm=algorithm)
+    rec.add_responses(np.zeros((0, 5), np.float32))
+    rec.iterate(2)
+    assert rec.n_events == 0
+    img = rec.rho
+    assert np.all(img == 0.0)
+    ref0, n0, _, _ = oracle_mod.hybrid_mlem(
+        ring, np.zeros((0, 5), np.float32), iterations=2)
+    assert n0 == 0 and np.all(ref0 == 0.0)
+    rec.close()
+    li = np.nonzero(ring.info_end == ring.info_begin)[0]
+    f = ((np.sqrt(8 * li + 1) - 1) // 2).astype(np.int64)
+    s = li - f * (f + 1) // 2
+    sel = f > s
+    f, s = f[sel], s[sel]
+    rng = np.random.default_rng(2)
+    ev = np.stack([f, s, rng.uniform(-0.1, 0.1, len(f)),
+                   rng.uniform(-0.1, 0.1, len(f)),
+                   rng.uniform(-0.1, 0.1, len(f))], 1).astype(np.float32)
+    rec = _rec(hybrid_mod, ring, algorithm=algorithm)
+    rec.add_responses(ev)
+    rec.iterate(2)
+    gpu = rec.rho
+    ref, n_used, _, _ = oracle_mod.hybrid_mlem(ring, ev, iterations=2)
+    assert 0 < n_used < len(ev)
+    assert rec.n_events == n_used
+    e = oracle_mod.rel_l2(gpu, ref)
+    msg = f"{len(ev)} events on empty LORs, {n_used} kept, rel-L2 {e:.3e}"
+    if oracle_mod.ref_available():
+        r, nr, _, _ = oracle_mod.ref_hybrid_mlem(ring, ev, iterations=2)
+        assert nr == n_used
+        assert oracle_mod.rel_l2(gpu, r) <= 1e-4
+    print(msg, flush=True)
+    assert e <= 1e-4 and np.all(np.isfinite(gpu))
+    rec.close()
+
+
+@pytest.mark.parametrize("algorithm", [1, 2])
 def test_geometry_from_file(oracle_mod, hybrid_mod, ring, algorithm,
                             tmp_path):
     """The reconstruction of a geometry read from a file in the reference's
